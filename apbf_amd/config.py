@@ -1,0 +1,201 @@
+"""Scene parameters and initial particle states for the benchmark workloads.
+
+The settings dict carries the fields of the reference's ``apbf_settings``
+uniform block (shaders/cpu_gpu_shared_config.h:74-94) with the defaults of
+source/settings.cpp:5-31 for a 3D build. Scene dicts add the host-side knobs of
+source/settings.cpp (basicPbf, solverIterations, splitDuration) and the scene
+wiring of source/pool.cpp / source/waterdrop.cpp (walls, gravity, hash domain,
+capacities).
+
+Initial states are plain numpy dicts (see ``STATE_KEYS``): positions are int32
+fixed point with 2^18 units per world unit (cpu_gpu_shared_config.h:7), h = 1
+world unit.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+POS_RESOLUTION = 262144.0
+KERNEL_SCALE = 4.0
+
+REFERENCE_SETTINGS = dict(
+    mHeightKernelId=1, mGradientKernelId=1, mMerge=1, mSplit=1,
+    mBaseKernelWidthOnTargetRadius=1, mBaseKernelWidthOnBoundaryDistance=1,
+    mUpdateTargetRadius=1, mUpdateBoundariness=1, mNeighborListSorted=0,
+    mBoundarinessCalculationMethod=2, mBoundarinessAdaptionSpeed=0.5,
+    mKernelWidthAdaptionSpeed=0.01, mBoundarinessSelfGradLengthFactor=2.0,
+    mBoundarinessUnderpressureFactor=4.0, mMergeDuration=2.0,
+    mSmallestTargetRadius=1.0, mTargetRadiusOffset=10.0, mTargetRadiusScaleFactor=0.3,
+)
+
+STATE_KEYS = ("position", "velocity", "inverse_mass", "radius", "pos_backup", "transferring",
+              "fluid_index", "target_radius", "kernel_width", "boundariness",
+              "boundary_distance", "t_source", "t_target", "t_time_left")
+
+
+def walls(box_min, box_max, radius, ceiling=True, thickness=4.0):
+    """Wall boxes around [box_min, box_max] (pool.cpp:30-40, waterdrop.cpp:28-38)."""
+    mn = np.asarray(box_min, np.float32)
+    mx = np.asarray(box_max, np.float32)
+    t = np.float32(thickness)
+    wmin = mn - t
+    wmax = mx + t + np.array([0, radius * 2, 0], np.float32)
+    b = [
+        (wmin, np.array([wmin[0] + t, wmax[1], wmax[2]], np.float32)),
+        (wmin, np.array([wmax[0], wmin[1] + t, wmax[2]], np.float32)),
+        (np.array([wmax[0] - t, wmin[1], wmin[2]], np.float32), wmax),
+    ]
+    if ceiling:
+        b.append((np.array([wmin[0], wmax[1] - t, wmin[2]], np.float32), wmax))
+    b.append((wmin, np.array([wmax[0], wmax[1], wmin[2] + t], np.float32)))
+    b.append((np.array([wmin[0], wmin[1], wmax[2] - t], np.float32), wmax))
+    return [(tuple(float(x) for x in lo), tuple(float(x) for x in hi)) for lo, hi in b]
+
+
+def base_params(**kw) -> dict:
+    p = dict(
+        settings=dict(REFERENCE_SETTINGS),
+        basic_pbf=0, solver_iterations=3, split_duration=0.0,
+        gravity=(0.0, -9.81, 0.0),
+        hash_min=(0.0, 0.0, 0.0), hash_max=(1.0, 1.0, 1.0), hash_res_log2=4,
+        boxes=[], cap_particles=0, cap_transfers=10000, cap_pairs=0,
+        dt=1.0 / 120.0,
+    )
+    p.update(kw)
+    return p
+
+
+def _fixed(xyz: np.ndarray) -> np.ndarray:
+    """World float -> int32 fixed point (truncation, as ivec3(pos * 2^18))."""
+    return np.trunc(xyz.astype(np.float64) * POS_RESOLUTION).astype(np.int32)
+
+
+def make_state(pos_world: np.ndarray, radius: np.ndarray, bd: np.ndarray | None = None,
+               boundariness: np.ndarray | None = None, target_radius: np.ndarray | None = None,
+               velocity: np.ndarray | None = None) -> dict:
+    n = len(pos_world)
+    radius = np.asarray(radius, np.float32)
+    pos = np.zeros((n, 4), np.int32)
+    pos[:, :3] = _fixed(pos_world)
+    vel = np.zeros((n, 4), np.float32)
+    if velocity is not None:
+        vel[:, :3] = velocity
+    inv_mass = (np.float32(1.0) / (np.float32(2.0) * radius) ** 3).astype(np.float32)
+    st = dict(
+        position=pos, velocity=vel, inverse_mass=inv_mass, radius=radius.copy(),
+        pos_backup=pos.copy(), transferring=np.zeros(n, np.uint32),
+        fluid_index=np.arange(n, dtype=np.uint32),
+        target_radius=(radius if target_radius is None else target_radius).astype(np.float32).copy(),
+        kernel_width=(radius * np.float32(KERNEL_SCALE)).astype(np.float32),
+        boundariness=(np.ones(n, np.float32) if boundariness is None else boundariness.astype(np.float32)),
+        boundary_distance=(np.trunc(radius.astype(np.float64) * POS_RESOLUTION).astype(np.uint32)
+                           if bd is None else np.trunc(bd * POS_RESOLUTION).astype(np.uint32)),
+        t_source=np.zeros(0, np.uint32), t_target=np.zeros(0, np.uint32),
+        t_time_left=np.zeros(0, np.float32),
+        last_delta_time=1.0,
+    )
+    return st
+
+
+def lattice(lo, counts, spacing, jitter, rng) -> np.ndarray:
+    """Particle centres lo + spacing/2 + spacing*(i,j,k) (initialize_box.comp:33-38),
+    each coordinate jittered by U(-jitter, jitter)."""
+    counts = [int(c) for c in counts]
+    g = np.stack(np.meshgrid(*[np.arange(c) for c in counts], indexing="ij"), -1).reshape(-1, 3)
+    p = np.asarray(lo, np.float64) + spacing * 0.5 + spacing * g
+    if jitter > 0:
+        p = p + rng.uniform(-jitter, jitter, p.shape)
+    return p
+
+
+# ------------------------------------------------------------------ workloads
+def dam_break(n_side: int = 32, spacing: float = 0.5, seed: int = 0, box_scale=(4, 2, 2),
+              iterations: int = 3, res: int | None = None):
+    """Plain PBF dam break: an n_side^3 lattice at `spacing` (jitter +-0.05h) in
+    the low corner of a box whose interior is box_scale times the lattice extent.
+    basicPbf, Gauss kernel, r = spacing/2, kernel width 4r = 2*spacing."""
+    rng = np.random.default_rng(seed)
+    r = spacing / 2.0
+    extent = n_side * spacing
+    box_max = (extent * box_scale[0], extent * box_scale[1], extent * box_scale[2])
+    pos = lattice((0.0, 0.0, 0.0), (n_side, n_side, n_side), spacing, 0.05, rng)
+    n = len(pos)
+    st = make_state(pos, np.full(n, r, np.float32))
+    if res is None:
+        # largest z-order resolution whose cells are at least one kernel width wide on every axis
+        res = int(math.floor(math.log2(min(box_max) / (4 * r))))
+    s = dict(REFERENCE_SETTINGS)
+    s["mSmallestTargetRadius"] = r
+    p = base_params(
+        settings=s, basic_pbf=1, solver_iterations=iterations,
+        hash_min=(0.0, 0.0, 0.0), hash_max=box_max, hash_res_log2=res,
+        boxes=walls((0, 0, 0), box_max, r, ceiling=True),
+        cap_particles=n, cap_transfers=16, cap_pairs=n * 64,
+        workload=f"dam_break_{n}",
+    )
+    return p, st
+
+
+def _sphere_points(center, radius_world, spacing, jitter, rng):
+    m = int(math.ceil(radius_world / spacing)) + 1
+    g = np.arange(-m, m + 1) * spacing
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
+    pts = pts[np.linalg.norm(pts, axis=1) <= radius_world - spacing * 0.5]
+    if jitter > 0:
+        pts = pts + rng.uniform(-jitter, jitter, pts.shape)
+    return pts + np.asarray(center, np.float64)
+
+
+def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coarse: float = 1.0,
+               r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6,
+               cap_factor: float = 3.0, pairs_per_particle: int = 160):
+    """Adaptive water drop: a sphere of fine particles (radius r_fine) above a pool
+    of coarse particles (radius r_coarse) filling the box floor. Split, merge and
+    kernel-width spreading on (baseKernelWidthOnBoundaryDistance off, neighbour
+    range scale 1.5), released at t = 0. Target radius grows as
+    r_fine + 0.05 * boundary distance (targetRadiusOffset 0), which keeps radii
+    within [r_fine, 4 r_fine] for a pool 80h deep."""
+    rng = np.random.default_rng(seed)
+    sc = 2 * r_coarse
+    pool_max = (pool_counts[0] * sc, pool_counts[1] * sc, pool_counts[2] * sc)
+    pool = lattice((0, 0, 0), pool_counts, sc, 0.05, rng)
+    sf = 2 * r_fine
+    drop_r = (drop_particles * sf ** 3 * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
+    drop_r += sf * 0.5
+    center = (pool_max[0] * 0.5, pool_max[1] + 8.0 + drop_r, pool_max[2] * 0.5)
+    drop = _sphere_points(center, drop_r, sf, 0.05, rng)
+    pos = np.concatenate([pool, drop], 0)
+    radius = np.concatenate([np.full(len(pool), r_coarse), np.full(len(drop), r_fine)]).astype(np.float32)
+    # boundary distance: distance to the pool's free surface or walls, resp. to the drop surface
+    d_pool = np.minimum.reduce([pool[:, 0], pool_max[0] - pool[:, 0], pool[:, 1],
+                                pool_max[1] - pool[:, 1], pool[:, 2], pool_max[2] - pool[:, 2]])
+    d_drop = drop_r - np.linalg.norm(drop - np.asarray(center), axis=1)
+    bd = np.maximum(np.concatenate([d_pool, d_drop]), radius)
+    s = dict(REFERENCE_SETTINGS)
+    s.update(mBaseKernelWidthOnBoundaryDistance=0, mSmallestTargetRadius=r_fine,
+             mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=0.05)
+    tr = np.maximum(r_fine + np.maximum(0.0, bd * 0.05), r_fine).astype(np.float32)
+    bness = (bd < 2.0 * radius).astype(np.float32)
+    st = make_state(pos, radius, bd=bd, boundariness=bness, target_radius=tr)
+    n = len(pos)
+    top = center[1] + drop_r + 4
+    hmin = (-4.0, -4.0, -4.0)
+    hmax = (pool_max[0] + 4.0, top, pool_max[2] + 4.0)
+    cap = int(n * cap_factor)
+    p = base_params(
+        settings=s, basic_pbf=0, solver_iterations=iterations,
+        hash_min=hmin, hash_max=hmax, hash_res_log2=res,
+        boxes=walls((0, 0, 0), pool_max, r_coarse, ceiling=False),
+        cap_particles=cap, cap_transfers=max(1000, n // 5), cap_pairs=cap * pairs_per_particle,
+        workload=f"water_drop_{n}",
+    )
+    return p, st
+
+
+def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: int = 2000, res: int = 4):
+    """A few-thousand-particle version of :func:`water_drop` for parity tests."""
+    return water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, res=res,
+                      pairs_per_particle=400)

@@ -1,0 +1,64 @@
+// Launch wrappers of the plain-substep kernels (definitions in plain_kernels.hip).
+#pragma once
+#include "scene_kernels.h"
+
+namespace apbf {
+
+// One buffer set of the particle SoA (hidden particles + fluid attributes,
+// list_definitions.h:9-13) indexed by particle slot.
+struct ParticleArrays {
+    int4* pos;
+    float4* vel;
+    float* inv_mass;
+    float* radius;
+    int4* backup;
+    uint32_t* transferring;
+    float* target_radius;
+    float* kernel_width;
+    float* boundariness;
+    uint32_t* boundary_distance;
+    uint32_t* prev_old;  // optional: receives the source slot of every gathered particle
+};
+
+void launch_integrate(int4* pos, int4* backup, float4* vel, const uint32_t* counts, uint32_t cap,
+                      float last_dt, float dt, const float acc[3], hipStream_t s);
+void launch_copy_int4(int4* dst, const int4* src, const uint32_t* counts, uint32_t cap, hipStream_t s);
+void launch_hash_iota(const int4* pos, const uint32_t* d_len, uint32_t cap, const GridDesc& g,
+                      uint32_t* keys, uint32_t* vals, hipStream_t s);
+void launch_position_code(const int4* pos, const uint32_t* d_len, uint32_t cap, uint32_t section,
+                          uint32_t* out, hipStream_t s);
+void launch_permute(const uint32_t* perm, const uint32_t* counts, uint32_t cap,
+                    const ParticleArrays& src, const ParticleArrays& dst, hipStream_t s);
+void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, uint32_t* start,
+                   uint32_t* end, hipStream_t s);
+void launch_green_count(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
+                        float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
+                        uint32_t* cnt, hipStream_t s);
+void launch_green_fill(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
+                       float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
+                       const uint32_t* row, uint32_t* col, uint32_t pair_cap, hipStream_t s);
+void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
+void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
+void launch_rev_count(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                      uint32_t pair_cap, uint32_t* rcnt, hipStream_t s);
+void launch_rev_fill(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                     uint32_t pair_cap, uint32_t* rcur, uint32_t* rcol, hipStream_t s);
+void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
+                          hipStream_t s);
+void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
+                const BoxSet& boxes, hipStream_t s);
+void launch_solve_a(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                    const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                    const uint32_t* col, uint32_t pair_cap, const SolverArgs& a, float* boundariness,
+                    float* lambda, int4* shift, hipStream_t s);
+void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const float* radius,
+                    const uint32_t* counts, uint32_t cap, const uint32_t* rrow, const uint32_t* rcol,
+                    int hid, int gid, const float* lambda, const int4* shift, const BoxSet& boxes,
+                    int4* pos_out, hipStream_t s);
+uint32_t density_error_blocks(uint32_t cap);
+void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                          const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                          const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,
+                          float* part_max, hipStream_t s);
+
+}  // namespace apbf

@@ -1,0 +1,539 @@
+// Kernels of the plain (non-adaptive) part of a substep.
+//
+// Particles are stored so that fluid id == hidden slot (the fluid index list
+// of the reference is the identity once every hidden particle is a fluid
+// particle and index lists are rebuilt in ascending hidden-slot order), so the
+// per-fluid-id and per-hidden-slot arrays share one index.
+//
+// Neighbour lists are CSR (row offsets + neighbour ids) instead of the
+// reference's unordered uvec2 pair list: row i holds the neighbours of
+// particle i in the order of the reference's cell walk, which makes every
+// per-pair shader a gather over the row (no atomics) and keeps the sums exact
+// because the reference accumulates per-pair truncated integers.
+#include "plain_kernels.h"
+
+namespace apbf {
+
+#define GRID_STRIDE(i, n)                                                              \
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+// velocity_handling::apply (velocity_handling.cpp:15-31): infer_velocity.comp:25-26,
+// position backup, apply_acceleration.comp:23-24, apply_velocity.comp:24-25, fused.
+__global__ __launch_bounds__(BLOCK) void k_integrate(int4* __restrict__ pos, int4* __restrict__ backup,
+                                                     float4* __restrict__ vel,
+                                                     const uint32_t* __restrict__ counts, float last_dt,
+                                                     float dt, float ax, float ay, float az) {
+    const uint32_t n = counts[C_N];
+    const float denom = last_dt * POS_RES;
+    const float s = dt * POS_RES;
+    GRID_STRIDE(i, n) {
+        int4 p = pos[i];
+        int4 b = backup[i];
+        float4 v = vel[i];
+        v.x = (float)isub(p.x, b.x) / denom;
+        v.y = (float)isub(p.y, b.y) / denom;
+        v.z = (float)isub(p.z, b.z) / denom;
+        backup[i] = p;
+        v.x += ax;
+        v.y += ay;
+        v.z += az;
+        vel[i] = v;
+        p.x = iadd(p.x, to_i32(v.x * s));
+        p.y = iadd(p.y, to_i32(v.y * s));
+        p.z = iadd(p.z, to_i32(v.z * s));
+        pos[i] = p;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_copy_int4(int4* __restrict__ dst, const int4* __restrict__ src,
+                                                     const uint32_t* __restrict__ counts) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) dst[i] = src[i];
+}
+
+// calculate_position_hash.comp:39-46 over every hidden particle, with the
+// index sequence of neighborhood_green.cpp:75.
+__global__ __launch_bounds__(BLOCK) void k_hash_iota(const int4* __restrict__ pos,
+                                                     const uint32_t* __restrict__ d_len, GridDesc g,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+    const uint32_t n = *d_len;
+    GRID_STRIDE(i, n) {
+        v3f p = pos_world(pos[i]);
+        uint32_t gx = grid_cell(p.x, g.mn[0], g.mx[0], g.res);
+        uint32_t gy = grid_cell(p.y, g.mn[1], g.mx[1], g.res);
+        uint32_t gz = grid_cell(p.z, g.mn[2], g.mx[2], g.res);
+        keys[i] = zhash(gx, gy, gz, g.res);
+        if (vals) vals[i] = i;
+    }
+}
+
+// calculate_position_code.comp:23-62: bit b of the 96-bit code is bit b/3 of
+// coordinate b%3 (positions reinterpreted as uint).
+__global__ __launch_bounds__(BLOCK) void k_position_code(const int4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ d_len,
+                                                         uint32_t section, uint32_t* __restrict__ out) {
+    const uint32_t n = *d_len;
+    GRID_STRIDE(i, n) {
+        int4 p = pos[i];
+        uint32_t u[3] = {(uint32_t)p.x, (uint32_t)p.y, (uint32_t)p.z};
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 32u; ++k) {
+            uint32_t b = section * 32u + k;
+            r |= ((u[b % 3u] >> (b / 3u)) & 1u) << k;
+        }
+        out[i] = r;
+    }
+}
+
+// Gather of every particle/fluid array by the sort permutation (the
+// hidden_list().apply_edit of neighborhood_green.cpp:78 plus the owner
+// rebuild of indexed_list.h:289-308, which with fluid id == hidden slot is the
+// same permutation for the fluid attributes).
+__global__ __launch_bounds__(BLOCK) void k_permute(const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ counts, ParticleArrays src,
+                                                   ParticleArrays dst) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        uint32_t o = perm[i];
+        dst.pos[i] = src.pos[o];
+        dst.vel[i] = src.vel[o];
+        dst.inv_mass[i] = src.inv_mass[o];
+        dst.radius[i] = src.radius[o];
+        dst.backup[i] = src.backup[o];
+        dst.transferring[i] = src.transferring[o];
+        dst.target_radius[i] = src.target_radius[o];
+        dst.kernel_width[i] = src.kernel_width[o];
+        dst.boundariness[i] = src.boundariness[o];
+        dst.boundary_distance[i] = src.boundary_distance[o];
+        if (src.prev_old) dst.prev_old[i] = o;
+    }
+}
+
+// find_value_ranges.comp:16-31; particle 0 has no predecessor.
+__global__ __launch_bounds__(BLOCK) void k_ranges(const uint32_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ d_len,
+                                                  uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const uint32_t n = *d_len;
+    GRID_STRIDE(i, n) {
+        uint32_t cur = keys[i];
+        if (i == 0) {
+            start[cur] = 0;
+        } else {
+            uint32_t prev = keys[i - 1];
+            if (cur != prev) {
+                start[cur] = i;
+                end[prev] = i;
+            }
+        }
+        if (i == n - 1) end[cur] = i + 1;
+    }
+}
+
+// neighborhood_green.comp:57-86: walk the cells of [pos - range, pos + range]
+// (x fastest, then y, then z) and take every particle j != i with
+// distance <= range. COUNT_ONLY counts; otherwise neighbours are written from
+// row_offset[i], dropping those at or beyond pair_cap (neighbor_add.glsl:22-23).
+template <bool COUNT_ONLY>
+__global__ __launch_bounds__(BLOCK) void k_green(const int4* __restrict__ pos,
+                                                 const float* __restrict__ range_in,
+                                                 const uint32_t* __restrict__ counts, float range_scale,
+                                                 GridDesc g, const uint32_t* __restrict__ cs,
+                                                 const uint32_t* __restrict__ ce, uint32_t* __restrict__ cnt,
+                                                 const uint32_t* __restrict__ row, uint32_t* __restrict__ col,
+                                                 uint32_t pair_cap) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        float range = range_in[i] * range_scale;
+        v3f p = pos_world(pos[i]);
+        uint32_t lo[3], hi[3];
+        lo[0] = grid_cell(p.x - range, g.mn[0], g.mx[0], g.res);
+        lo[1] = grid_cell(p.y - range, g.mn[1], g.mx[1], g.res);
+        lo[2] = grid_cell(p.z - range, g.mn[2], g.mx[2], g.res);
+        hi[0] = grid_cell(p.x + range, g.mn[0], g.mx[0], g.res);
+        hi[1] = grid_cell(p.y + range, g.mn[1], g.mx[1], g.res);
+        hi[2] = grid_cell(p.z + range, g.mn[2], g.mx[2], g.res);
+        // Bound the walk to 4 * 2^res cells per axis: wider walks only revisit
+        // aliased cells, and an unbounded span (infinite range) would not end.
+        const uint32_t span = 4u << g.res;
+        for (int d = 0; d < 3; ++d)
+            if (hi[d] - lo[d] >= span && hi[d] >= lo[d]) hi[d] = lo[d] + span - 1u;
+        uint32_t c = 0;
+        uint32_t base = COUNT_ONLY ? 0u : row[i];
+        for (uint32_t z = lo[2]; z <= hi[2]; ++z) {
+            for (uint32_t y = lo[1]; y <= hi[1]; ++y) {
+                for (uint32_t x = lo[0]; x <= hi[0]; ++x) {
+                    uint32_t h = zhash(x, y, z, g.res);
+                    uint32_t s = cs[h], e = ce[h];
+                    for (uint32_t j = s; j < e; ++j) {
+                        if (j == i) continue;
+                        v3f q = pos_world(pos[j]);
+                        if (length(p - q) > range) continue;
+                        if (!COUNT_ONLY) {
+                            uint32_t slot = base + c;
+                            if (slot < pair_cap) col[slot] = j;
+                        }
+                        ++c;
+                    }
+                    if (x == 0xFFFFFFFFu) break;
+                }
+                if (y == 0xFFFFFFFFu) break;
+            }
+            if (z == 0xFFFFFFFFu) break;
+        }
+        if (COUNT_ONLY) cnt[i] = c;
+    }
+    if (COUNT_ONLY && blockIdx.x == 0 && threadIdx.x == 0) cnt[n] = 0;
+}
+
+// Pair totals after the row-offset scan: clamp to the capacity and flag overflow.
+__global__ void k_pairs_total(uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
+                              uint32_t pair_cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t n = counts[C_N];
+    uint32_t total = row[n];
+    counts[C_PAIRS_RAW] = total;
+    if (total > pair_cap) {
+        counts[C_PAIRS] = pair_cap;
+        counts[C_OVERFLOW] |= 1u;
+    } else {
+        counts[C_PAIRS] = total;
+    }
+}
+
+__global__ void k_set_n_plus_1(uint32_t* __restrict__ counts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counts[C_N_PLUS_1] = counts[C_N] + 1u;
+}
+
+// Reverse adjacency: rcnt[j] = number of rows that name j.
+__global__ __launch_bounds__(BLOCK) void k_rev_count(const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ row,
+                                                     const uint32_t* __restrict__ col, uint32_t pair_cap,
+                                                     uint32_t* __restrict__ rcnt) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        uint32_t b = row[i], e = min(row[i + 1], pair_cap);
+        for (uint32_t k = b; k < e; ++k) atomicAdd(&rcnt[col[k]], 1u);
+    }
+}
+__global__ __launch_bounds__(BLOCK) void k_rev_fill(const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ row,
+                                                    const uint32_t* __restrict__ col, uint32_t pair_cap,
+                                                    uint32_t* __restrict__ rcur, uint32_t* __restrict__ rcol) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        uint32_t b = row[i], e = min(row[i + 1], pair_cap);
+        for (uint32_t k = b; k < e; ++k) rcol[atomicAdd(&rcur[col[k]], 1u)] = i;
+    }
+}
+
+// Kernel constants per particle (depend on the kernel width only).
+__global__ __launch_bounds__(BLOCK) void k_kernel_consts(const float* __restrict__ kw,
+                                                         const uint32_t* __restrict__ counts,
+                                                         float2* __restrict__ gc) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        GaussConst g = gauss_const(kw[i]);
+        gc[i] = make_float2(g.ivd, g.norm);
+    }
+}
+
+// box_collision.comp:33-55 over every particle.
+__global__ __launch_bounds__(BLOCK) void k_box(int4* __restrict__ pos, const float* __restrict__ radius,
+                                               const uint32_t* __restrict__ counts, BoxSet boxes) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) pos[i] = box_collide(i, pos[i], radius[i], boxes);
+}
+
+// incompressibility_0.comp + incompressibility_1.comp (sums over row i) +
+// incompressibility_2.comp, and the boundariness filter of
+// incompressibility_3.comp:41-46,60 for row i. Writes lambda_i, the particle's
+// own position shift (applied by k_solve_b) and boundariness_i.
+__global__ __launch_bounds__(BLOCK) void k_solve_a(const int4* __restrict__ pos,
+                                                   const float* __restrict__ inv_mass,
+                                                   const float* __restrict__ radius,
+                                                   const float* __restrict__ kw,
+                                                   const float2* __restrict__ gc,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ row,
+                                                   const uint32_t* __restrict__ col, uint32_t pair_cap,
+                                                   SolverArgs a, float* __restrict__ boundariness,
+                                                   float* __restrict__ lambda, int4* __restrict__ shift) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        const int4 pi = pos[i];
+        const float im = inv_mass[i];
+        const float kwi = kw[i];
+        const float2 g2 = gc[i];
+        const GaussConst g{g2.x, g2.y};
+        float w0;
+        v3f dummy;
+        kernel_eval(a.hid, a.gid, mk(0, 0, 0), kwi, g, true, false, &w0, &dummy);
+        uint32_t density = to_u32(w0 / im * INCOMP_RES);
+        int32_t gs0 = 0, gs1 = 0, gs2 = 0;
+        uint32_t sq = 0;
+        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        const uint32_t rb = row[i], re = min(row[i + 1], pair_cap);
+        for (uint32_t k = rb; k < re; ++k) {
+            const uint32_t j = col[k];
+            const int4 pj = pos[j];
+            const float imN = inv_mass[j];
+            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            float w;
+            v3f grad;
+            kernel_eval(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+            v3f wg = grad / imN;
+            density += to_u32(w / imN * INCOMP_RES);
+            gs0 = iadd(gs0, to_i32(wg.x * INCOMP_RES));
+            gs1 = iadd(gs1, to_i32(wg.y * INCOMP_RES));
+            gs2 = iadd(gs2, to_i32(wg.z * INCOMP_RES));
+            sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
+            if (a.method == 1) {
+                float sc = imN * kwi;
+                v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / sc;
+                c0 = iadd(c0, to_i32(wd.x));
+                c1 = iadd(c1, to_i32(wd.y));
+                c2 = iadd(c2, to_i32(wd.z));
+                c3 = iadd(c3, to_i32(INCOMP_RES / imN));
+            }
+        }
+        // incompressibility_2.comp:77-109
+        const float r = radius[i];
+        const float ird = pow_f(2.0f * r, DIM) * im;
+        const float dens = (float)density / INCOMP_RES;
+        const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
+        float sgs = (float)sq / INCOMP_RES;
+        const v3f selfGrad = (-wgs) * ird;
+        const float wgss = dot(wgs, wgs);
+        const float sgl = __builtin_sqrtf(wgss) * ird;
+        sgs += wgss * im;
+        const float under = 1.0f - dens * ird;
+        float b = boundariness[i];
+        if (a.update_b) {
+            float bb;
+            if (a.method == 1) {
+                float totalMass = (float)c3 / INCOMP_RES + 1.0f / im;
+                float dev = length(vec_i(c0, c1, c2)) / (POS_RES * totalMass);
+                bb = dev * a.b_sgl_factor;
+            } else {
+                float s1 = sgl * kwi * a.b_sgl_factor;
+                float up = fmax_g(0.0f, under) * a.b_under_factor;
+                bb = s1 + up;
+            }
+            bb = bb >= 1.0f ? 1.0f : 0.0f;
+            bb = move_towards_add(b, bb, a.b_speed);
+            b = fmin_g(1.0f, bb);
+        }
+        float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
+        lam /= a.lambda_scale_base / ird * im;
+        lambda[i] = lam;
+        int4 sh = make_int4(0, 0, 0, 0);
+        if (!(lam >= 0.0f)) {
+            float s = lam * im * POS_RES;
+            v3f d = selfGrad * s;
+            sh = make_int4(to_i32(d.x), to_i32(d.y), to_i32(d.z), 0);
+        }
+        shift[i] = sh;
+        // boundariness filter (incompressibility_3.comp:41-46), any neighbour of row i
+        if (a.method == 2 && __builtin_bit_cast(uint32_t, b) != 0u) {
+            v3f empty = -normalize(vec_i(gs0, gs1, gs2));
+            for (uint32_t k = rb; k < re; ++k) {
+                const uint32_t j = col[k];
+                const int4 pj = pos[j];
+                v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+                float w;
+                v3f grad;
+                kernel_eval(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+                if (dot(empty, normalize(grad)) > 0.6f) {
+                    b = 0.0f;
+                    break;
+                }
+            }
+        }
+        boundariness[i] = b;
+    }
+}
+
+// incompressibility_3.comp:62-66 gathered per target particle j over the rows
+// that name j, plus j's own shift from incompressibility_2.comp:106-109. When
+// `boxes` has walls, box_collision of the next solver iteration follows.
+__global__ __launch_bounds__(BLOCK) void k_solve_b(const int4* __restrict__ pos,
+                                                   const float* __restrict__ kw,
+                                                   const float2* __restrict__ gc,
+                                                   const float* __restrict__ radius,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ rrow,
+                                                   const uint32_t* __restrict__ rcol, int hid, int gid,
+                                                   const float* __restrict__ lambda,
+                                                   const int4* __restrict__ shift, BoxSet boxes,
+                                                   int4* __restrict__ pos_out) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(j, n) {
+        const int4 pj = pos[j];
+        const int4 sj = shift[j];
+        int32_t ax = iadd(pj.x, sj.x), ay = iadd(pj.y, sj.y), az = iadd(pj.z, sj.z);
+        const uint32_t rb = rrow[j], re = rrow[j + 1];
+        for (uint32_t k = rb; k < re; ++k) {
+            const uint32_t i = rcol[k];
+            const float lam = lambda[i];
+            if (!(lam < 0.0f)) continue;  // lambda >= 0: no shift; NaN shifts by ivec3(NaN) = 0
+            const int4 pi = pos[i];
+            const float2 g2 = gc[i];
+            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            float w;
+            v3f grad;
+            kernel_eval(hid, gid, diff, kw[i], GaussConst{g2.x, g2.y}, false, true, &w, &grad);
+            v3f d = grad * (lam * POS_RES);
+            ax = iadd(ax, to_i32(d.x));
+            ay = iadd(ay, to_i32(d.y));
+            az = iadd(az, to_i32(d.z));
+        }
+        int4 out = make_int4(ax, ay, az, pj.w);
+        if (boxes.n) out = box_collide(j, out, radius[j], boxes);
+        pos_out[j] = out;
+    }
+}
+
+// Density error per particle: max(0, rho_i / rho0_i - 1) from the sums of
+// incompressibility_0/1, reduced to per-block (sum, max) partials.
+__global__ __launch_bounds__(BLOCK) void k_density_error(const int4* __restrict__ pos,
+                                                         const float* __restrict__ inv_mass,
+                                                         const float* __restrict__ radius,
+                                                         const float* __restrict__ kw,
+                                                         const float2* __restrict__ gc,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ row,
+                                                         const uint32_t* __restrict__ col,
+                                                         uint32_t pair_cap, int hid, int gid,
+                                                         double* __restrict__ part_sum,
+                                                         float* __restrict__ part_max) {
+    __shared__ double ssum[BLOCK];
+    __shared__ float smax[BLOCK];
+    const uint32_t n = counts[C_N];
+    double acc = 0.0;
+    float mx = 0.0f;
+    GRID_STRIDE(i, n) {
+        const int4 pi = pos[i];
+        const float im = inv_mass[i];
+        const float kwi = kw[i];
+        const float2 g2 = gc[i];
+        const GaussConst g{g2.x, g2.y};
+        float w0;
+        v3f dummy;
+        kernel_eval(hid, gid, mk(0, 0, 0), kwi, g, true, false, &w0, &dummy);
+        uint32_t density = to_u32(w0 / im * INCOMP_RES);
+        const uint32_t rb = row[i], re = min(row[i + 1], pair_cap);
+        for (uint32_t k = rb; k < re; ++k) {
+            const uint32_t j = col[k];
+            const int4 pj = pos[j];
+            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            float w;
+            kernel_eval(hid, gid, diff, kwi, g, true, false, &w, &dummy);
+            density += to_u32(w / inv_mass[j] * INCOMP_RES);
+        }
+        float ird = pow_f(2.0f * radius[i], DIM) * im;
+        float e = fmax_g(0.0f, (float)density / INCOMP_RES * ird - 1.0f);
+        acc += (double)e;
+        mx = fmax_g(mx, e);
+    }
+    ssum[threadIdx.x] = acc;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            smax[threadIdx.x] = fmax_g(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_sum[blockIdx.x] = ssum[0];
+        part_max[blockIdx.x] = smax[0];
+    }
+}
+
+// ------------------------------------------------------------ launch wrappers
+#define LAUNCH(kern, grid, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, __VA_ARGS__)
+
+void launch_integrate(int4* pos, int4* backup, float4* vel, const uint32_t* counts, uint32_t cap,
+                      float last_dt, float dt, const float acc[3], hipStream_t s) {
+    LAUNCH(k_integrate, stride_grid(cap), pos, backup, vel, counts, last_dt, dt, acc[0], acc[1], acc[2]);
+}
+void launch_copy_int4(int4* dst, const int4* src, const uint32_t* counts, uint32_t cap, hipStream_t s) {
+    LAUNCH(k_copy_int4, stride_grid(cap), dst, src, counts);
+}
+void launch_hash_iota(const int4* pos, const uint32_t* d_len, uint32_t cap, const GridDesc& g,
+                      uint32_t* keys, uint32_t* vals, hipStream_t s) {
+    LAUNCH(k_hash_iota, stride_grid(cap), pos, d_len, g, keys, vals);
+}
+void launch_position_code(const int4* pos, const uint32_t* d_len, uint32_t cap, uint32_t section,
+                          uint32_t* out, hipStream_t s) {
+    LAUNCH(k_position_code, stride_grid(cap), pos, d_len, section, out);
+}
+void launch_permute(const uint32_t* perm, const uint32_t* counts, uint32_t cap,
+                    const ParticleArrays& src, const ParticleArrays& dst, hipStream_t s) {
+    LAUNCH(k_permute, stride_grid(cap), perm, counts, src, dst);
+}
+void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, uint32_t* start,
+                   uint32_t* end, hipStream_t s) {
+    LAUNCH(k_ranges, stride_grid(cap), keys, d_len, start, end);
+}
+void launch_green_count(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
+                        float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
+                        uint32_t* cnt, hipStream_t s) {
+    LAUNCH(k_green<true>, stride_grid(cap, 8192), pos, range, counts, scale, g, cs, ce, cnt,
+           (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u);
+}
+void launch_green_fill(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
+                       float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
+                       const uint32_t* row, uint32_t* col, uint32_t pair_cap, hipStream_t s) {
+    LAUNCH(k_green<false>, stride_grid(cap, 8192), pos, range, counts, scale, g, cs, ce,
+           (uint32_t*)nullptr, row, col, pair_cap);
+}
+void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_pairs_total, dim3(1), dim3(64), 0, s, counts, row, pair_cap);
+}
+void launch_set_n_plus_1(uint32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_n_plus_1, dim3(1), dim3(64), 0, s, counts);
+}
+void launch_rev_count(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                      uint32_t pair_cap, uint32_t* rcnt, hipStream_t s) {
+    LAUNCH(k_rev_count, stride_grid(cap, 8192), counts, row, col, pair_cap, rcnt);
+}
+void launch_rev_fill(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                     uint32_t pair_cap, uint32_t* rcur, uint32_t* rcol, hipStream_t s) {
+    LAUNCH(k_rev_fill, stride_grid(cap, 8192), counts, row, col, pair_cap, rcur, rcol);
+}
+void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
+                          hipStream_t s) {
+    LAUNCH(k_kernel_consts, stride_grid(cap), kw, counts, gc);
+}
+void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
+                const BoxSet& boxes, hipStream_t s) {
+    LAUNCH(k_box, stride_grid(cap), pos, radius, counts, boxes);
+}
+void launch_solve_a(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                    const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                    const uint32_t* col, uint32_t pair_cap, const SolverArgs& a, float* boundariness,
+                    float* lambda, int4* shift, hipStream_t s) {
+    LAUNCH(k_solve_a, stride_grid(cap, 8192), pos, inv_mass, radius, kw, gc, counts, row, col, pair_cap,
+           a, boundariness, lambda, shift);
+}
+void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const float* radius,
+                    const uint32_t* counts, uint32_t cap, const uint32_t* rrow, const uint32_t* rcol,
+                    int hid, int gid, const float* lambda, const int4* shift, const BoxSet& boxes,
+                    int4* pos_out, hipStream_t s) {
+    LAUNCH(k_solve_b, stride_grid(cap, 8192), pos, kw, gc, radius, counts, rrow, rcol, hid, gid, lambda,
+           shift, boxes, pos_out);
+}
+uint32_t density_error_blocks(uint32_t cap) { return stride_grid(cap, 1024); }
+void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                          const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                          const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,
+                          float* part_max, hipStream_t s) {
+    LAUNCH(k_density_error, density_error_blocks(cap), pos, inv_mass, radius, kw, gc, counts, row, col,
+           pair_cap, hid, gid, part_sum, part_max);
+}
+
+}  // namespace apbf

@@ -1,0 +1,318 @@
+// Device-wide scan and radix sort for gfx950 (64-lane wavefronts).
+//
+// Replaces algorithms::prefix_sum / algorithms::sort (algorithms.cpp:59-118)
+// and their shaders (prefix_sum_apply_on_block_level.comp,
+// prefix_sum_spread_from_block_level.comp, radix_sort_apply_on_block_level.comp,
+// radix_sort_scattered_write.comp). Lengths are read from device memory so the
+// host never synchronises; grids are sized for the capacity and tiles past the
+// length exit or contribute zeros.
+//
+// Scan: reduce-then-scan over tiles of 2048 values (256 threads x 8 values).
+// Wave-level inclusive scans use __shfl_up over the 64 lanes; the four waves of
+// a block combine through LDS.
+//
+// Radix sort: LSD on 8-bit digits (half the passes of the reference's 4-bit
+// digits). Per pass: (1) per-tile digit histograms written bucket-major
+// [digit][tile], (2) one exclusive scan of that table gives every (digit, tile)
+// its global output offset, (3) a stable scatter: each tile walks its values in
+// eight 256-wide rounds in index order; inside a round a value's rank among
+// equal digits comes from a 64-lane match built from eight __ballot masks, and
+// per-wave digit counts in LDS give the cross-wave and cross-round offsets.
+#include "apbf_internal.h"
+
+namespace apbf {
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan of one value per thread; returns the exclusive prefix
+// and the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds4, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    uint32_t w0 = lds4[0], w1 = lds4[1], w2 = lds4[2], w3 = lds4[3];
+    uint32_t woff = (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + (wave > 2 ? w2 : 0u);
+    total = w0 + w1 + w2 + w3;
+    __syncthreads();
+    return woff + inc - v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                       const uint32_t* __restrict__ d_len,
+                                                       uint32_t n_host, uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t lds[4];
+    const uint32_t n = d_len ? *d_len : n_host;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * SCAN_TILE;
+    uint32_t sum = 0;
+    if (base < n) {
+        uint64_t i0 = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            if (i0 + k < n) sum += in[i0 + k];
+    }
+    uint32_t total;
+    block_excl_scan(sum, lds, total);
+    if (threadIdx.x == 0) tile_sums[tile] = (base < n) ? total : 0u;
+}
+
+// Exclusive scan of the tile sums by one block, looping over chunks.
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t* __restrict__ tile_sums,
+                                                      uint32_t n_tiles,
+                                                      uint32_t* __restrict__ total_out) {
+    __shared__ uint32_t lds[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n_tiles; base += BLOCK) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < n_tiles ? tile_sums[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_excl_scan(v, lds, total);
+        if (i < n_tiles) tile_sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                      uint32_t* __restrict__ out,
+                                                      const uint32_t* __restrict__ d_len, uint32_t n_host,
+                                                      const uint32_t* __restrict__ tile_offsets,
+                                                      int inclusive) {
+    __shared__ uint32_t lds[4];
+    const uint32_t n = d_len ? *d_len : n_host;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * SCAN_TILE;
+    if (base >= n) return;
+    uint64_t i0 = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(sum, lds, total) + tile_offsets[tile];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint32_t x = v[k];
+        if (i0 + k < n) out[i0 + k] = inclusive ? run + x : run;
+        run += x;
+    }
+}
+
+size_t scan_workspace_bytes(uint32_t capacity) {
+    return (size_t)(div_up(capacity ? capacity : 1, SCAN_TILE) + 1) * 4 + 256;
+}
+
+static void scan_impl(const uint32_t* in, uint32_t* out, const uint32_t* d_len, uint32_t n_host,
+                      uint32_t capacity, bool inclusive, uint32_t* total, void* ws, hipStream_t s) {
+    uint32_t tiles = div_up(capacity ? capacity : 1, SCAN_TILE);
+    uint32_t* tile_sums = static_cast<uint32_t*>(ws);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(BLOCK), 0, s, in, d_len, n_host, tile_sums);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(BLOCK), 0, s, tile_sums, tiles, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(BLOCK), 0, s, in, out, d_len, n_host,
+                       tile_sums, inclusive ? 1 : 0);
+}
+
+void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* d_len, uint32_t capacity,
+              bool inclusive, uint32_t* total, void* ws, hipStream_t s) {
+    scan_impl(in, out, d_len, 0, capacity, inclusive, total, ws, s);
+}
+
+void scan_u32_n(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint32_t* total,
+                void* ws, hipStream_t s) {
+    scan_impl(in, out, nullptr, n, n, inclusive, total, ws, s);
+}
+
+// ------------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(BLOCK) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ d_len,
+                                                     uint32_t shift, uint32_t tiles_cap,
+                                                     uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[RADIX];
+    const uint32_t n = *d_len;
+    const uint32_t tile = blockIdx.x;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)tile * SORT_TILE;
+    if (base < n) {
+#pragma unroll
+        for (int r = 0; r < SORT_ITEMS; ++r) {
+            uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
+            if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * tiles_cap + tile] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sort_scatter(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const uint32_t* __restrict__ d_len,
+                                                        uint32_t shift, uint32_t tiles_cap,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        uint32_t* __restrict__ out_keys,
+                                                        uint32_t* __restrict__ out_vals) {
+    __shared__ uint32_t running[RADIX];
+    __shared__ uint32_t wcnt[4][RADIX];
+    const uint32_t n = *d_len;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * SORT_TILE;
+    if (base >= n) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    running[threadIdx.x] = offsets[(uint64_t)threadIdx.x * tiles_cap + tile];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        wcnt[0][threadIdx.x] = 0;
+        wcnt[1][threadIdx.x] = 0;
+        wcnt[2][threadIdx.x] = 0;
+        wcnt[3][threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? keys[i] : 0u;
+        uint32_t val = valid ? vals[i] : 0u;
+        uint32_t d = (key >> shift) & (RADIX - 1);
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            match &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(match & lt_mask);
+        uint32_t cnt = (uint32_t)__popcll(match);
+        if (valid && rank == 0) wcnt[wave][d] = cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t off = running[d] + rank;
+            for (int w = 0; w < wave; ++w) off += wcnt[w][d];
+            out_keys[off] = key;
+            out_vals[off] = val;
+        }
+        __syncthreads();
+        running[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
+                                wcnt[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void k_copy_pairs(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                             uint32_t* __restrict__ oa, uint32_t* __restrict__ ob,
+                             const uint32_t* __restrict__ d_len) {
+    const uint32_t n = *d_len;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        oa[i] = a[i];
+        ob[i] = b[i];
+    }
+}
+
+size_t sort_workspace_bytes(uint32_t capacity) {
+    uint32_t tiles = div_up(capacity ? capacity : 1, SORT_TILE);
+    size_t hist = (size_t)tiles * RADIX * 4;
+    size_t tmp = (size_t)(capacity ? capacity : 1) * 8;
+    return hist + scan_workspace_bytes(tiles * RADIX) + tmp + 1024;
+}
+
+void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, const uint32_t* d_len,
+                      uint32_t capacity, uint32_t upper_bound, uint32_t* out_keys,
+                      uint32_t* out_vals, void* ws, hipStream_t s) {
+    uint32_t cap = capacity ? capacity : 1;
+    uint32_t tiles = div_up(cap, SORT_TILE);
+    char* w = static_cast<char*>(ws);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w);
+    w += (size_t)tiles * RADIX * 4;
+    void* scan_ws = w;
+    w += scan_workspace_bytes(tiles * RADIX);
+    uint32_t* tk = reinterpret_cast<uint32_t*>(w);
+    uint32_t* tv = tk + cap;
+    int passes = 0;
+    for (uint32_t off = 0; off < 32u && (upper_bound >> off) != 0u; off += RADIX_BITS) ++passes;
+    if (passes == 0) {
+        hipLaunchKernelGGL(k_copy_pairs, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, keys, vals,
+                           out_keys, out_vals, d_len);
+        return;
+    }
+    // Ping-pong so that the last pass lands in out_*.
+    const uint32_t* src_k = keys;
+    const uint32_t* src_v = vals;
+    for (int p = 0; p < passes; ++p) {
+        bool last = (p == passes - 1);
+        // pass p writes to out_* when (passes - 1 - p) is even
+        bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* dk = to_out ? out_keys : tk;
+        uint32_t* dv = to_out ? out_vals : tv;
+        uint32_t shift = (uint32_t)p * RADIX_BITS;
+        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, s, src_k, d_len, shift, tiles,
+                           hist);
+        scan_u32_n(hist, hist, tiles * RADIX, false, nullptr, scan_ws, s);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(BLOCK), 0, s, src_k, src_v, d_len,
+                           shift, tiles, hist, dk, dv);
+        src_k = dk;
+        src_v = dv;
+        (void)last;
+    }
+}
+
+// ------------------------------------------------------------------ helpers
+__global__ void k_fill(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_dev(uint32_t* __restrict__ p, uint32_t v, const uint32_t* __restrict__ d_len) {
+    const uint32_t n = *d_len;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_iota(uint32_t* __restrict__ p, const uint32_t* __restrict__ d_len) {
+    const uint32_t n = *d_len;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+__global__ void k_gather_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                               const uint32_t* __restrict__ edit, const uint32_t* __restrict__ d_len,
+                               uint32_t words) {
+    const uint64_t n = (uint64_t)(*d_len) * words;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t e = i / words, o = i - e * words;
+        dst[i] = src[(uint64_t)edit[e] * words + o];
+    }
+}
+__global__ void k_copy1(uint32_t* dst, const uint32_t* src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(stride_grid(n)), dim3(BLOCK), 0, s, p, value, n);
+}
+void fill_u32_dev(uint32_t* p, uint32_t value, const uint32_t* d_len, uint32_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_dev, dim3(stride_grid(capacity)), dim3(BLOCK), 0, s, p, value, d_len);
+}
+void iota_u32(uint32_t* p, const uint32_t* d_len, uint32_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL(k_iota, dim3(stride_grid(capacity)), dim3(BLOCK), 0, s, p, d_len);
+}
+void gather_words(const uint32_t* src, uint32_t* dst, const uint32_t* edit, const uint32_t* d_len,
+                  uint32_t capacity, uint32_t words, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_words, dim3(stride_grid((uint64_t)capacity * words)), dim3(BLOCK), 0, s,
+                       src, dst, edit, d_len, words);
+}
+void copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, s, dst, src);
+}
+
+}  // namespace apbf

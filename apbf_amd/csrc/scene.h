@@ -1,0 +1,99 @@
+// Scene: device buffers and subsystem methods of one simulation.
+#pragma once
+#include "plain_kernels.h"
+
+namespace apbf {
+
+struct Scene {
+    apbf_scene_params P{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    uint32_t cap = 0, tcap = 0, pcap = 0, n_cells = 0;
+    GridDesc grid{};
+    BoxSet boxes{}, no_boxes{};
+    float last_dt = 1.0f;  // velocity_handling.h:17
+    int cur = 0;           // current particle buffer set
+    int tcur = 0;          // current transfer buffer set
+    bool reverse_valid = false;
+
+    // particle SoA, two buffer sets (gathers are out of place)
+    int4* pos[2] = {};
+    float4* vel[2] = {};
+    float* inv_mass[2] = {};
+    float* radius[2] = {};
+    int4* backup[2] = {};
+    uint32_t* transferring[2] = {};
+    float* target_radius[2] = {};
+    float* kernel_width[2] = {};
+    float* boundariness[2] = {};
+    uint32_t* boundary_distance[2] = {};
+    // hidden transfer list (list_definitions.h:15-16), two buffer sets
+    uint32_t* t_source[2] = {};
+    uint32_t* t_target[2] = {};
+    float* t_time_left[2] = {};
+
+    uint32_t* counts = nullptr;  // CountSlot block
+    // neighbourhood
+    uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *perm = nullptr;
+    uint32_t *cell_start = nullptr, *cell_end = nullptr;
+    uint32_t *cnt = nullptr, *row = nullptr, *col = nullptr;
+    uint32_t *rcnt = nullptr, *rrow = nullptr, *rcol = nullptr, *col_tmp = nullptr;
+    // solver
+    float2* gc = nullptr;
+    float* lambda = nullptr;
+    int4* shift = nullptr;
+    // adaptive scratch
+    uint32_t* ukw = nullptr;
+    uint32_t *u0 = nullptr, *u1 = nullptr, *u2 = nullptr, *u3 = nullptr, *u4 = nullptr;
+    uint32_t* small = nullptr;
+    // split/merge claim resolution (per particle) and transfer compaction scratch
+    uint32_t *a_type = nullptr, *a_s = nullptr, *a_t = nullptr, *a_state = nullptr, *a_owner = nullptr;
+    uint32_t *a_flag = nullptr, *a_scan = nullptr, *a_list = nullptr, *a_t0 = nullptr;
+    uint32_t *t_flag = nullptr, *t_scan = nullptr;
+    int claim_rounds = 6;
+    // density error partials
+    uint32_t err_blocks = 0;
+    double* part_sum = nullptr;
+    float* part_max = nullptr;
+    void* ws_mem = nullptr;
+    size_t ws_bytes = 0;
+
+    Scene() = default;
+    ~Scene();
+    apbf_status init(const apbf_scene_params& p);
+    ParticleArrays arrays(int set);
+
+    apbf_status set_state(uint32_t n, const int32_t* h_pos, const float* h_vel, const float* h_im,
+                          const float* h_r, const int32_t* h_backup, const uint32_t* h_tr,
+                          const uint32_t* h_fidx, const float* h_trad, const float* h_kw,
+                          const float* h_b, const uint32_t* h_bd, uint32_t nt, const uint32_t* h_ts,
+                          const uint32_t* h_tt, const float* h_ttl, float last_delta_time);
+    apbf_status get_state(int32_t* h_pos, float* h_vel, float* h_im, float* h_r, int32_t* h_backup,
+                          uint32_t* h_tr, float* h_trad, float* h_kw, float* h_b, uint32_t* h_bd,
+                          uint32_t* h_ts, uint32_t* h_tt, float* h_ttl, uint32_t* h_pairs,
+                          float* h_ldt);
+
+    // subsystems (reference L4 apply() calls)
+    void velocity_handling(float dt);
+    void particle_transfer(float dt);
+    void kernel_width_from_bd();
+    void neighborhood_green(float range_scale);
+    void spread_kernel_width();
+    void box_collision();
+    void incompressibility();
+    void update_transfers();
+    void update(float dt);
+    apbf_status stage(int st, float dt);
+    apbf_status density_error(double* mean_err, double* max_err);
+
+    // helpers
+    void build_pairs(float range_scale);
+    void build_reverse();
+    void prepare_solver();
+    SolverArgs solver_args() const;
+    void incompressibility_iteration(const BoxSet& next_boxes);
+    void solver_loop();
+    void remap_transfers_after_permute();
+};
+
+}  // namespace apbf

@@ -1,0 +1,437 @@
+// Host side of a scene: device buffers, the subsystem apply() sequence of one
+// substep, and the C ABI of include/apbf.h.
+//
+// The subsystem methods mirror the reference's L4 classes and keep their
+// order and flags (source/pool.cpp:67-106):
+//   velocity_handling::apply      -> Scene::velocity_handling
+//   particle_transfer::apply      -> Scene::particle_transfer      (adaptive.hip)
+//   kernel width from boundary d. -> Scene::kernel_width_from_bd   (adaptive.hip)
+//   neighborhood_green::apply     -> Scene::neighborhood_green
+//   spread_kernel_width::apply    -> Scene::spread_kernel_width    (adaptive.hip)
+//   box_collision::apply          -> Scene::box_collision
+//   incompressibility::apply      -> Scene::incompressibility
+//   update_transfers::apply       -> Scene::update_transfers       (adaptive.hip)
+// No device memory is allocated after construction and nothing is read back
+// to the host inside a substep; all list lengths stay on the device.
+#include "scene.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace apbf {
+
+#define HIP_OK(x)                                  \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return APBF_ERR_HIP; \
+    } while (0)
+
+template <class T>
+static apbf_status dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+ParticleArrays Scene::arrays(int set) {
+    ParticleArrays a;
+    a.pos = pos[set];
+    a.vel = vel[set];
+    a.inv_mass = inv_mass[set];
+    a.radius = radius[set];
+    a.backup = backup[set];
+    a.transferring = transferring[set];
+    a.target_radius = target_radius[set];
+    a.kernel_width = kernel_width[set];
+    a.boundariness = boundariness[set];
+    a.boundary_distance = boundary_distance[set];
+    a.prev_old = nullptr;
+    return a;
+}
+
+apbf_status Scene::init(const apbf_scene_params& p) {
+    P = p;
+    if (p.cap_particles == 0 || p.hash_res_log2 == 0 || p.hash_res_log2 > 10 || p.n_boxes > APBF_MAX_BOXES)
+        return APBF_ERR_INVALID_ARGUMENT;
+    cap = p.cap_particles;
+    tcap = std::max<uint32_t>(p.cap_transfers, 1);
+    pcap = std::max<uint32_t>(p.cap_pairs, 1);
+    n_cells = 1u << (3u * p.hash_res_log2);
+    grid.res = p.hash_res_log2;
+    for (int d = 0; d < 3; ++d) {
+        grid.mn[d] = p.hash_min[d];
+        grid.mx[d] = p.hash_max[d];
+    }
+    boxes.n = p.n_boxes;
+    for (uint32_t i = 0; i < p.n_boxes; ++i)
+        for (int d = 0; d < 3; ++d) {
+            boxes.mn[i][d] = p.box_min[i][d];
+            boxes.mx[i][d] = p.box_max[i][d];
+        }
+    no_boxes.n = 0;
+    HIP_OK(hipGetDevice(&device));
+    apbf_status st;
+#define A(ptr, n)                               \
+    if ((st = dalloc(&(ptr), (n))) != APBF_OK) \
+        return st;
+    for (int s = 0; s < 2; ++s) {
+        A(pos[s], cap);
+        A(vel[s], cap);
+        A(inv_mass[s], cap);
+        A(radius[s], cap);
+        A(backup[s], cap);
+        A(transferring[s], cap);
+        A(target_radius[s], cap);
+        A(kernel_width[s], cap);
+        A(boundariness[s], cap);
+        A(boundary_distance[s], cap);
+        A(t_source[s], tcap);
+        A(t_target[s], tcap);
+        A(t_time_left[s], tcap);
+    }
+    A(counts, C_COUNT);
+    A(keys, cap);
+    A(vals, cap);
+    A(skeys, cap);
+    A(perm, cap);
+    A(cell_start, n_cells);
+    A(cell_end, n_cells);
+    A(cnt, cap + 1);
+    A(row, cap + 1);
+    A(col, pcap);
+    A(rcnt, cap + 1);
+    A(rrow, cap + 1);
+    A(rcol, pcap);
+    A(col_tmp, pcap);
+    A(gc, cap);
+    A(lambda, cap);
+    A(shift, cap);
+    A(ukw, cap);
+    A(u0, cap + 1);
+    A(u1, cap + 1);
+    A(u2, cap + 1);
+    A(u3, cap + 1);
+    A(u4, cap + 1);
+    A(small, 64);
+    A(a_type, cap + 1);
+    A(a_s, cap + 1);
+    A(a_t, cap + 1);
+    A(a_state, cap + 1);
+    A(a_owner, cap + 1);
+    A(a_flag, cap + 1);
+    A(a_scan, cap + 1);
+    A(a_list, cap + 1);
+    A(a_t0, cap + 1);
+    A(t_flag, tcap + 1);
+    A(t_scan, tcap + 1);
+    err_blocks = density_error_blocks(cap);
+    A(part_sum, err_blocks);
+    A(part_max, err_blocks);
+#undef A
+    size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
+    ws = std::max(ws, scan_workspace_bytes(pcap));
+    ws = std::max(ws, scan_workspace_bytes(tcap + 1));
+    ws_bytes = ws;
+    HIP_OK(hipMalloc(&ws_mem, ws_bytes));
+    HIP_OK(hipMemset(counts, 0, C_COUNT * 4));
+    HIP_OK(hipDeviceSynchronize());
+    return APBF_OK;
+}
+
+Scene::~Scene() {
+    for (int s = 0; s < 2; ++s) {
+        hipFree(pos[s]); hipFree(vel[s]); hipFree(inv_mass[s]); hipFree(radius[s]);
+        hipFree(backup[s]); hipFree(transferring[s]); hipFree(target_radius[s]);
+        hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]);
+        hipFree(t_source[s]); hipFree(t_target[s]); hipFree(t_time_left[s]);
+    }
+    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, rcnt, rrow,
+                  rcol, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
+                  part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_flag, a_scan,
+                  a_list, a_t0, t_flag, t_scan};
+    for (void* q : ps) hipFree(q);
+}
+
+// ------------------------------------------------------ velocity_handling
+// velocity_handling.cpp:15-31
+void Scene::velocity_handling(float dt) {
+    if (dt == 0.0f) {
+        launch_copy_int4(backup[cur], pos[cur], counts, cap, stream);
+        return;
+    }
+    float acc[3] = {P.gravity[0] * dt, P.gravity[1] * dt, P.gravity[2] * dt};
+    launch_integrate(pos[cur], backup[cur], vel[cur], counts, cap, last_dt, dt, acc, stream);
+    last_dt = dt;
+}
+
+// ----------------------------------------------------- neighborhood_green
+// neighborhood_green.cpp:51-101: hash every particle, stable sort by hash,
+// permute the particle SoA, cell tables, pair search.
+void Scene::neighborhood_green(float range_scale) {
+    hipStream_t s = stream;
+    uint32_t upper = 1u << (grid.res * 3u);
+    launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s);
+    radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s);
+    ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
+    launch_permute(perm, counts, cap, src, dst, s);
+    cur = 1 - cur;
+    // The transfer list follows its particles (indexed_list owner rebuild).
+    remap_transfers_after_permute();
+    fill_u32(cell_start, 0u, n_cells, s);
+    fill_u32(cell_end, 0u, n_cells, s);
+    launch_ranges(skeys, counts + C_N, cap, cell_start, cell_end, s);
+    build_pairs(range_scale);
+}
+
+void Scene::build_pairs(float range_scale) {
+    hipStream_t s = stream;
+    launch_set_n_plus_1(counts, s);
+    launch_green_count(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
+                       cnt, s);
+    scan_u32(cnt, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+    launch_pairs_total(counts, row, pcap, s);
+    launch_green_fill(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
+                      row, col, pcap, s);
+    reverse_valid = false;
+}
+
+void Scene::build_reverse() {
+    hipStream_t s = stream;
+    fill_u32_dev(rcnt, 0u, counts + C_N_PLUS_1, cap + 1, s);
+    launch_rev_count(counts, cap, row, col, pcap, rcnt, s);
+    scan_u32(rcnt, rrow, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+    hipMemcpyAsync(u0, rrow, (size_t)(cap + 1) * 4, hipMemcpyDeviceToDevice, s);
+    launch_rev_fill(counts, cap, row, col, pcap, u0, rcol, s);
+    reverse_valid = true;
+}
+
+// --------------------------------------------------------- box_collision
+void Scene::box_collision() { launch_box(pos[cur], radius[cur], counts, cap, boxes, stream); }
+
+// ----------------------------------------------------- incompressibility
+// incompressibility.cpp:12-45 as two gather kernels: k_solve_a (per particle
+// i over its row: _0, _1, _2 and the boundariness filter of _3) and k_solve_b
+// (per particle j over the rows naming j: the neighbour shifts of _3 plus j's
+// own shift). `next_boxes` fuses the box collision that opens the next solver
+// iteration into k_solve_b.
+SolverArgs Scene::solver_args() const {
+    SolverArgs a;
+    a.hid = P.s.mHeightKernelId;
+    a.gid = P.s.mGradientKernelId;
+    a.method = P.s.mBoundarinessCalculationMethod;
+    a.update_b = P.s.mUpdateBoundariness;
+    a.b_speed = P.s.mBoundarinessAdaptionSpeed;
+    a.b_sgl_factor = P.s.mBoundarinessSelfGradLengthFactor;
+    a.b_under_factor = P.s.mBoundarinessUnderpressureFactor;
+    a.lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, DIM);
+    return a;
+}
+
+void Scene::prepare_solver() {
+    if (!reverse_valid) build_reverse();
+    launch_kernel_consts(kernel_width[cur], counts, cap, gc, stream);
+}
+
+void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
+    hipStream_t s = stream;
+    SolverArgs a = solver_args();
+    launch_solve_a(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col,
+                   pcap, a, boundariness[cur], lambda, shift, s);
+    // The other buffer set's position array is free between permutations.
+    int4* out = pos[1 - cur];
+    launch_solve_b(pos[cur], kernel_width[cur], gc, radius[cur], counts, cap, rrow, rcol, a.hid, a.gid,
+                   lambda, shift, next_boxes, out, s);
+    std::swap(pos[0], pos[1]);
+}
+
+void Scene::incompressibility() {
+    prepare_solver();
+    incompressibility_iteration(no_boxes);
+}
+
+void Scene::solver_loop() {
+    if (P.solver_iterations <= 0) return;
+    prepare_solver();
+    box_collision();
+    for (int it = 0; it < P.solver_iterations; ++it) {
+        bool last = it == P.solver_iterations - 1;
+        incompressibility_iteration(last ? no_boxes : boxes);
+    }
+}
+
+// ----------------------------------------------------------------- update
+void Scene::update(float dt) {
+    const apbf_settings& s = P.s;
+    velocity_handling(dt);
+    if (!P.basic_pbf && (s.mMerge || s.mSplit)) particle_transfer(dt);
+    if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_bd();
+    neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+    if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
+    solver_loop();
+    if (!P.basic_pbf) update_transfers();
+}
+
+apbf_status Scene::stage(int st, float dt) {
+    const apbf_settings& s = P.s;
+    switch (st) {
+        case APBF_STAGE_VELOCITY: velocity_handling(dt); break;
+        case APBF_STAGE_PARTICLE_TRANSFER: particle_transfer(dt); break;
+        case APBF_STAGE_KERNEL_WIDTH_BD: kernel_width_from_bd(); break;
+        case APBF_STAGE_NEIGHBORHOOD:
+            neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            break;
+        case APBF_STAGE_SPREAD: spread_kernel_width(); break;
+        case APBF_STAGE_BOX: box_collision(); break;
+        case APBF_STAGE_INCOMPRESSIBILITY: incompressibility(); break;
+        case APBF_STAGE_UPDATE_TRANSFERS: update_transfers(); break;
+        default: return APBF_ERR_INVALID_ARGUMENT;
+    }
+    return hipGetLastError() == hipSuccess ? APBF_OK : APBF_ERR_HIP;
+}
+
+apbf_status Scene::density_error(double* mean_err, double* max_err) {
+    hipStream_t s = stream;
+    launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
+    launch_density_error(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row,
+                         col, pcap, P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+    std::vector<double> hs(err_blocks);
+    std::vector<float> hm(err_blocks);
+    uint32_t n = 0;
+    HIP_OK(hipMemcpyAsync(hs.data(), part_sum, err_blocks * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hm.data(), part_max, err_blocks * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&n, counts + C_N, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double sum = 0.0;
+    float mx = 0.0f;
+    for (uint32_t b = 0; b < err_blocks; ++b) {
+        sum += hs[b];
+        mx = fmax_g(mx, hm[b]);
+    }
+    *mean_err = n ? sum / (double)n : 0.0;
+    *max_err = (double)mx;
+    return APBF_OK;
+}
+
+}  // namespace apbf
+
+// ======================================================================= C ABI
+using apbf::Scene;
+
+extern "C" {
+
+apbf_status apbf_scene_create(const apbf_scene_params* params, void** out_scene) {
+    if (!params || !out_scene) return APBF_ERR_INVALID_ARGUMENT;
+    *out_scene = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return APBF_ERR_NO_DEVICE;
+    Scene* s = new (std::nothrow) Scene();
+    if (!s) return APBF_ERR_INVALID_ARGUMENT;
+    apbf_status st = s->init(*params);
+    if (st != APBF_OK) {
+        delete s;
+        return st;
+    }
+    *out_scene = s;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_destroy(void* scene) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    hipDeviceSynchronize();
+    delete static_cast<Scene*>(scene);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_stream(void* scene, apbf_stream stream) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    static_cast<Scene*>(scene)->stream = static_cast<hipStream_t>(stream);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_state(void* scene, uint32_t n, const int32_t* h_pos, const float* h_vel,
+                                 const float* h_im, const float* h_r, const int32_t* h_backup,
+                                 const uint32_t* h_tr, const uint32_t* h_fidx, const float* h_trad,
+                                 const float* h_kw, const float* h_b, const uint32_t* h_bd,
+                                 uint32_t nt, const uint32_t* h_ts, const uint32_t* h_tt,
+                                 const float* h_ttl, float last_delta_time) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    return s->set_state(n, h_pos, h_vel, h_im, h_r, h_backup, h_tr, h_fidx, h_trad, h_kw, h_b, h_bd, nt,
+                        h_ts, h_tt, h_ttl, last_delta_time);
+}
+
+apbf_status apbf_scene_get_counts(void* scene, uint32_t* n, uint32_t* nt, uint32_t* np) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    uint32_t c[apbf::C_COUNT];
+    if (hipMemcpyAsync(c, s->counts, sizeof(c), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (n) *n = c[apbf::C_N];
+    if (nt) *nt = c[apbf::C_TRANSFERS];
+    if (np) *np = c[apbf::C_PAIRS];
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_get_state(void* scene, int32_t* h_pos, float* h_vel, float* h_im, float* h_r,
+                                 int32_t* h_backup, uint32_t* h_tr, float* h_trad, float* h_kw,
+                                 float* h_b, uint32_t* h_bd, uint32_t* h_ts, uint32_t* h_tt,
+                                 float* h_ttl, uint32_t* h_pairs, float* h_ldt) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    return static_cast<Scene*>(scene)->get_state(h_pos, h_vel, h_im, h_r, h_backup, h_tr, h_trad, h_kw,
+                                                 h_b, h_bd, h_ts, h_tt, h_ttl, h_pairs, h_ldt);
+}
+
+apbf_status apbf_scene_get_lambda(void* scene, float* h_lambda) {
+    if (!scene || !h_lambda) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    uint32_t n = 0;
+    if (hipMemcpyAsync(&n, s->counts + apbf::C_N, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (hipMemcpy(h_lambda, s->lambda, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_stage(void* scene, int32_t stage, float dt) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    return static_cast<Scene*>(scene)->stage(stage, dt);
+}
+
+apbf_status apbf_scene_update(void* scene, float dt) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->update(dt);
+    return hipGetLastError() == hipSuccess ? APBF_OK : APBF_ERR_HIP;
+}
+
+apbf_status apbf_scene_density_error(void* scene, double* mean_err, double* max_err) {
+    if (!scene || !mean_err || !max_err) return APBF_ERR_INVALID_ARGUMENT;
+    return static_cast<Scene*>(scene)->density_error(mean_err, max_err);
+}
+
+apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags) {
+    if (!scene || !flags) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (hipMemcpyAsync(flags, s->counts + apbf::C_OVERFLOW, 4, hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(s->counts + apbf::C_OVERFLOW, 0, 4, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+const char* apbf_version(void) { return "apbf-mi355x 0.1 (gfx950)"; }
+
+int32_t apbf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

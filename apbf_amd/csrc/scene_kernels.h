@@ -1,0 +1,107 @@
+// Device state of one scene and the kernels of a substep.
+#pragma once
+#include "apbf_internal.h"
+#include "apbf_sph.h"
+
+namespace apbf {
+
+// Device-resident list lengths and flags (the reference's per-list 4-byte
+// length buffers, gathered into one block).
+enum CountSlot : uint32_t {
+    C_N = 0,           // particles (hidden list length == fluid list length)
+    C_N_PLUS_1 = 1,    // n + 1, length of the CSR offset arrays
+    C_PAIRS = 2,       // forward neighbour pairs (after the capacity clamp)
+    C_TRANSFERS = 3,   // hidden transfer list length
+    C_SPLITS = 4,      // split list length
+    C_PROPOSALS = 5,   // merge/split proposals of find_split_and_merge_3
+    C_OVERFLOW = 6,    // bit 0 pairs, bit 1 transfers, bit 2 particles
+    C_PAIRS_RAW = 7,   // neighbour pairs before the clamp
+    C_KEEP = 8,        // particles surviving a compaction
+    C_TKEEP = 9,       // transfers surviving a compaction
+    C_UNDECIDED = 10,  // claims left after the parallel rounds
+    C_MERGES = 11,     // merges won in find_split_and_merge_3
+    C_TMP0 = 12,
+    C_TMP1 = 13,
+    C_COUNT = 16
+};
+
+struct GridDesc {
+    float mn[3];
+    float mx[3];
+    uint32_t res;
+};
+
+struct BoxSet {
+    uint32_t n;
+    float mn[APBF_MAX_BOXES][3];
+    float mx[APBF_MAX_BOXES][3];
+};
+
+struct SolverArgs {
+    int hid, gid, method, update_b;
+    float b_speed, b_sgl_factor, b_under_factor, lambda_scale_base;
+};
+
+// -------------------------------------------------------- shared device code
+APBF_HD uint32_t grid_cell(float p, float lo, float hi, uint32_t res) {
+    return to_u32((p - lo) / (hi - lo) * (float)(1u << res));
+}
+APBF_HD uint32_t spread_bits3(uint32_t v) {  // bit i -> bit 3i, for the low 10 bits
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+// z-curve of calculate_position_hash.comp:29-36: bit i of axis d -> bit 3i+d, i < res.
+APBF_HD uint32_t zhash(uint32_t gx, uint32_t gy, uint32_t gz, uint32_t res) {
+    uint32_t m = (res >= 32u) ? 0xFFFFFFFFu : ((1u << res) - 1u);
+    return spread_bits3(gx & m) | (spread_bits3(gy & m) << 1) | (spread_bits3(gz & m) << 2);
+}
+APBF_HD v3f pos_world(int4 p) { return vec_i(p.x, p.y, p.z) / POS_RES; }
+
+// box_collision.comp:15-20 ("Hash without Sine", MIT licence)
+APBF_HD v3f hash31(float p) {
+    v3f p3 = mk(fract(p * 0.1031f), fract(p * 0.1030f), fract(p * 0.0973f));
+    float d = dot(p3, mk(p3.y + 33.33f, p3.z + 33.33f, p3.x + 33.33f));
+    p3 = mk(p3.x + d, p3.y + d, p3.z + d);
+    return mk(fract((p3.x + p3.y) * p3.z), fract((p3.x + p3.z) * p3.y), fract((p3.y + p3.z) * p3.x));
+}
+APBF_HD v3f smallest_mask(v3f v) {  // box_collision.comp:22-28
+    bool b1 = v.x <= v.y, b2 = v.x <= v.z, b3 = v.y <= v.z;
+    return mk((b1 && b2) ? 1.0f : 0.0f, (!b1 && b3) ? 1.0f : 0.0f, (!b2 && !b3) ? 1.0f : 0.0f);
+}
+// box_collision.comp:36-54 for the particle with list position `id`.
+APBF_HD int4 box_collide(uint32_t id, int4 p, float r, const BoxSet& boxes) {
+    v3f pos = pos_world(p);
+    for (uint32_t i = 0; i < boxes.n; ++i) {
+        v3f jmin = hash31((float)id * pos.x - pos.y - pos.z) * 0.05f;
+        v3f bmin = (mk(boxes.mn[i][0], boxes.mn[i][1], boxes.mn[i][2]) - mk(r, r, r)) - jmin;
+        v3f jmax = hash31((float)id * pos.y + pos.x + pos.z) * 0.05f;
+        v3f bmax = (mk(boxes.mx[i][0], boxes.mx[i][1], boxes.mx[i][2]) + mk(r, r, r)) + jmax;
+        v3f toMin = bmin - pos, toMax = bmax - pos;
+        toMin = toMin * smallest_mask(-toMin);
+        toMax = toMax * smallest_mask(toMax);
+        float dMin = -dot(toMin, mk(1, 1, 1));
+        float dMax = dot(toMax, mk(1, 1, 1));
+        if (dMin <= 0.0f || dMax <= 0.0f) continue;
+        pos = pos + (dMin < dMax ? toMin : toMax);
+    }
+    return make_int4(to_i32(pos.x * POS_RES), to_i32(pos.y * POS_RES), to_i32(pos.z * POS_RES), p.w);
+}
+
+// incompressibility_2.comp:37-41
+APBF_HD float move_towards_add(float oldv, float newv, float max_step) {
+    float step = newv - oldv;
+    return oldv + fmin_g(max_step, fmax_g(-max_step, step));
+}
+// uint_to_float_but_gradual.comp:18-24 / uint_to_float_with_indexed_lower_bound.comp:20-26
+APBF_HD float move_towards_mult(float oldv, float newv, float step) {
+    float dir = sign_f(newv - oldv);
+    float result = oldv * (1.0f + step * dir);
+    bool reached = (oldv < newv) != (result < newv);
+    return reached ? newv : result;
+}
+
+}  // namespace apbf

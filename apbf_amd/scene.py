@@ -1,0 +1,126 @@
+"""Python front end of a GPU scene (apbf_scene_* in include/apbf.h).
+
+A :class:`Scene` holds one simulation in HBM; ``update(dt)`` runs one substep
+(the reference's ``pool::update`` / ``waterdrop::update``) entirely on the
+device. State upload/download is for setup, tests and inspection only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+STAGES = {
+    "velocity": 0, "particle_transfer": 1, "kernel_width_bd": 2, "neighborhood": 3,
+    "spread": 4, "box": 5, "incompressibility": 6, "update_transfers": 7,
+}
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class Scene:
+    def __init__(self, params: dict, stream: int | None = None):
+        self._L = _lib.lib()
+        self.params = params
+        self._p = _lib.make_params(params)
+        h = C.c_void_p()
+        check(self._L.apbf_scene_create(C.byref(self._p), C.byref(h)), "apbf_scene_create")
+        self._h = h
+        if stream is not None:
+            check(self._L.apbf_scene_set_stream(self._h, C.c_void_p(stream)), "apbf_scene_set_stream")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.apbf_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream: int) -> None:
+        check(self._L.apbf_scene_set_stream(self._h, C.c_void_p(stream)), "apbf_scene_set_stream")
+
+    def set_state(self, st: dict) -> None:
+        n = len(st["radius"])
+        nt = len(st["t_source"])
+        a = dict(
+            position=np.ascontiguousarray(st["position"], np.int32).reshape(n, 4),
+            velocity=np.ascontiguousarray(st["velocity"], np.float32).reshape(n, 4),
+            inverse_mass=np.ascontiguousarray(st["inverse_mass"], np.float32),
+            radius=np.ascontiguousarray(st["radius"], np.float32),
+            pos_backup=np.ascontiguousarray(st["pos_backup"], np.int32).reshape(n, 4),
+            transferring=np.ascontiguousarray(st["transferring"], np.uint32),
+            fluid_index=np.ascontiguousarray(st["fluid_index"], np.uint32),
+            target_radius=np.ascontiguousarray(st["target_radius"], np.float32),
+            kernel_width=np.ascontiguousarray(st["kernel_width"], np.float32),
+            boundariness=np.ascontiguousarray(st["boundariness"], np.float32),
+            boundary_distance=np.ascontiguousarray(st["boundary_distance"], np.uint32),
+            t_source=np.ascontiguousarray(st["t_source"], np.uint32),
+            t_target=np.ascontiguousarray(st["t_target"], np.uint32),
+            t_time_left=np.ascontiguousarray(st["t_time_left"], np.float32))
+        if len(a["fluid_index"]) != n:
+            raise ValueError("every particle must be a fluid particle exactly once")
+        check(self._L.apbf_scene_set_state(
+            self._h, n, _ptr(a["position"]), _ptr(a["velocity"]), _ptr(a["inverse_mass"]),
+            _ptr(a["radius"]), _ptr(a["pos_backup"]), _ptr(a["transferring"]), _ptr(a["fluid_index"]),
+            _ptr(a["target_radius"]), _ptr(a["kernel_width"]), _ptr(a["boundariness"]),
+            _ptr(a["boundary_distance"]), nt, _ptr(a["t_source"]), _ptr(a["t_target"]),
+            _ptr(a["t_time_left"]), float(st.get("last_delta_time", 1.0))), "apbf_scene_set_state")
+
+    def counts(self):
+        n, nt, npairs = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(self._L.apbf_scene_get_counts(self._h, C.byref(n), C.byref(nt), C.byref(npairs)),
+              "apbf_scene_get_counts")
+        return n.value, nt.value, npairs.value
+
+    def get_state(self, pairs: bool = True) -> dict:
+        n, nt, npairs = self.counts()
+        st = dict(
+            position=np.zeros((n, 4), np.int32), velocity=np.zeros((n, 4), np.float32),
+            inverse_mass=np.zeros(n, np.float32), radius=np.zeros(n, np.float32),
+            pos_backup=np.zeros((n, 4), np.int32), transferring=np.zeros(n, np.uint32),
+            fluid_index=np.arange(n, dtype=np.uint32), target_radius=np.zeros(n, np.float32),
+            kernel_width=np.zeros(n, np.float32), boundariness=np.zeros(n, np.float32),
+            boundary_distance=np.zeros(n, np.uint32), t_source=np.zeros(nt, np.uint32),
+            t_target=np.zeros(nt, np.uint32), t_time_left=np.zeros(nt, np.float32),
+            pairs=np.zeros((npairs if pairs else 0, 2), np.uint32))
+        ldt = C.c_float()
+        check(self._L.apbf_scene_get_state(
+            self._h, _ptr(st["position"]), _ptr(st["velocity"]), _ptr(st["inverse_mass"]),
+            _ptr(st["radius"]), _ptr(st["pos_backup"]), _ptr(st["transferring"]),
+            _ptr(st["target_radius"]), _ptr(st["kernel_width"]), _ptr(st["boundariness"]),
+            _ptr(st["boundary_distance"]), _ptr(st["t_source"]), _ptr(st["t_target"]),
+            _ptr(st["t_time_left"]), _ptr(st["pairs"]) if pairs else None, C.byref(ldt)),
+            "apbf_scene_get_state")
+        st["last_delta_time"] = ldt.value
+        return st
+
+    def lambdas(self) -> np.ndarray:
+        n = self.counts()[0]
+        out = np.zeros(max(n, 1), np.float32)
+        check(self._L.apbf_scene_get_lambda(self._h, _ptr(out)), "apbf_scene_get_lambda")
+        return out[:n]
+
+    def stage(self, name: str, dt: float = 0.0) -> None:
+        check(self._L.apbf_scene_stage(self._h, STAGES[name], float(dt)), f"stage {name}")
+
+    def update(self, dt: float) -> None:
+        check(self._L.apbf_scene_update(self._h, float(dt)), "apbf_scene_update")
+
+    def density_error(self):
+        m, x = C.c_double(), C.c_double()
+        check(self._L.apbf_scene_density_error(self._h, C.byref(m), C.byref(x)), "density_error")
+        return m.value, x.value
+
+    def overflow_flags(self) -> int:
+        f = C.c_uint32()
+        check(self._L.apbf_scene_overflow_flags(self._h, C.byref(f)), "overflow_flags")
+        return f.value

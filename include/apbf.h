@@ -1,0 +1,244 @@
+/* apbf.h — C ABI of the MI355X (gfx950) implementation of APBF's per-substep
+ * particle pipeline.
+ *
+ * The reference (cg-tuwien/APBF) drives its GLSL compute shaders through the
+ * static `shader_provider` class (source/shader_provider.h:9-76), which takes
+ * device buffer handles plus separate 4-byte device buffers for every list
+ * length, and through the list containers and subsystem `apply()` calls above
+ * it (source/gpu_list.h, indexed_list.h, algorithms.h, neighborhood_green.h,
+ * incompressibility.h, spread_kernel_width.h, update_transfers.h,
+ * particle_transfer.h, velocity_handling.h, box_collision.h, pool.h).
+ * Every entry point below replaces one of those interfaces (cited per
+ * function). Conventions, following the reference:
+ *  - pointers are device pointers unless the name says `host`;
+ *  - list lengths live in device memory (`d_len`, u32) so nothing reads back
+ *    to the host inside a substep; `capacity` is the allocated element count;
+ *  - positions are int32 fixed point, 2^18 units per world unit, 4 components
+ *    per particle (x, y, z, w; w is carried, never interpreted);
+ *  - every call is enqueued on `stream` (0 = the null stream) and returns
+ *    immediately; no call allocates device memory unless documented;
+ *  - every call returns an apbf_status and never throws.
+ */
+#ifndef APBF_H
+#define APBF_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t apbf_status;
+#define APBF_OK 0
+#define APBF_ERR_INVALID_ARGUMENT -1
+#define APBF_ERR_HIP -2
+#define APBF_ERR_CAPACITY -3
+#define APBF_ERR_NO_DEVICE -4
+#define APBF_ERR_WORKSPACE -5
+
+typedef void* apbf_stream; /* a hipStream_t */
+
+/* Uniform-block settings, field order and meaning of
+ * shaders/cpu_gpu_shared_config.h:74-94 (defaults in source/settings.cpp:5-31). */
+typedef struct apbf_settings {
+    int32_t mHeightKernelId;
+    int32_t mGradientKernelId;
+    int32_t mMerge;
+    int32_t mSplit;
+    int32_t mBaseKernelWidthOnTargetRadius;
+    int32_t mBaseKernelWidthOnBoundaryDistance;
+    int32_t mUpdateTargetRadius;
+    int32_t mUpdateBoundariness;
+    int32_t mNeighborListSorted;
+    int32_t mBoundarinessCalculationMethod;
+    float mBoundarinessAdaptionSpeed;
+    float mKernelWidthAdaptionSpeed;
+    float mBoundarinessSelfGradLengthFactor;
+    float mBoundarinessUnderpressureFactor;
+    float mMergeDuration;
+    float mSmallestTargetRadius;
+    float mTargetRadiusOffset;
+    float mTargetRadiusScaleFactor;
+} apbf_settings;
+
+#define APBF_MAX_BOXES 16
+
+/* Scene wiring of source/pool.cpp:6-56 and source/waterdrop.cpp:6-58 plus the
+ * host-only knobs of source/settings.cpp (basicPbf, solverIterations,
+ * splitDuration). Capacities are the reference's requested lengths. */
+typedef struct apbf_scene_params {
+    apbf_settings s;
+    int32_t basic_pbf;
+    int32_t solver_iterations;
+    float split_duration;
+    float gravity[3];
+    float hash_min[3];
+    float hash_max[3];
+    uint32_t hash_res_log2;
+    uint32_t n_boxes;
+    float box_min[APBF_MAX_BOXES][4];
+    float box_max[APBF_MAX_BOXES][4];
+    uint32_t cap_particles;
+    uint32_t cap_transfers;
+    uint32_t cap_pairs;
+} apbf_scene_params;
+
+/* ------------------------------------------------------------------ scene */
+/* A scene owns the particle state of one simulation on the current device:
+ * the hidden particle SoA, the fluid attributes, the transfer list and the
+ * neighbour lists (list_definitions.h:9-20), plus all temporaries of a substep
+ * (allocated here, once; the substep itself allocates nothing).
+ * Replaces the scene classes pool / waterdrop (pool.h:14-56, waterdrop.h). */
+apbf_status apbf_scene_create(const apbf_scene_params* params, void** out_scene);
+apbf_status apbf_scene_destroy(void* scene);
+apbf_status apbf_scene_set_stream(void* scene, apbf_stream stream);
+
+/* Upload a state from host arrays (layout as the reference lists: position,
+ * velocity and pos_backup 4 x 32 bit per particle). The fluid list must cover
+ * every hidden particle once (fluid_index a permutation); it is stored with the
+ * particles reordered so that fluid id == hidden slot. */
+apbf_status apbf_scene_set_state(void* scene, uint32_t n, const int32_t* host_position,
+                                 const float* host_velocity, const float* host_inverse_mass,
+                                 const float* host_radius, const int32_t* host_pos_backup,
+                                 const uint32_t* host_transferring, const uint32_t* host_fluid_index,
+                                 const float* host_target_radius, const float* host_kernel_width,
+                                 const float* host_boundariness,
+                                 const uint32_t* host_boundary_distance, uint32_t n_transfers,
+                                 const uint32_t* host_t_source, const uint32_t* host_t_target,
+                                 const float* host_t_time_left, float last_delta_time);
+/* Device-resident counts, copied to the host (synchronises the stream). */
+apbf_status apbf_scene_get_counts(void* scene, uint32_t* n_particles, uint32_t* n_transfers,
+                                  uint32_t* n_pairs);
+/* Download the state (synchronises). `host_pairs` receives the neighbour pairs
+ * (fluid id, neighbour fluid id) ordered by first id, then by the order of the
+ * reference's cell walk (neighborhood_green.comp:69-86). Any pointer may be NULL. */
+apbf_status apbf_scene_get_state(void* scene, int32_t* host_position, float* host_velocity,
+                                 float* host_inverse_mass, float* host_radius,
+                                 int32_t* host_pos_backup, uint32_t* host_transferring,
+                                 float* host_target_radius, float* host_kernel_width,
+                                 float* host_boundariness, uint32_t* host_boundary_distance,
+                                 uint32_t* host_t_source, uint32_t* host_t_target,
+                                 float* host_t_time_left, uint32_t* host_pairs,
+                                 float* host_last_delta_time);
+apbf_status apbf_scene_get_lambda(void* scene, float* host_lambda);
+
+/* Stages of one substep in the order of pool.cpp:67-106 (same numbering as the
+ * subsystem apply() calls they replace). */
+#define APBF_STAGE_VELOCITY 0          /* velocity_handling::apply (velocity_handling.cpp:15-31) */
+#define APBF_STAGE_PARTICLE_TRANSFER 1 /* particle_transfer::apply (particle_transfer.cpp:10-28) */
+#define APBF_STAGE_KERNEL_WIDTH_BD 2   /* uint_to_float_with_indexed_lower_bound (pool.cpp:77-80) */
+#define APBF_STAGE_NEIGHBORHOOD 3      /* neighborhood_green::apply (neighborhood_green.cpp:51-101) */
+#define APBF_STAGE_SPREAD 4            /* spread_kernel_width::apply (spread_kernel_width.cpp:12-26) */
+#define APBF_STAGE_BOX 5               /* box_collision::apply (box_collision.cpp:12-24) */
+#define APBF_STAGE_INCOMPRESSIBILITY 6 /* incompressibility::apply (incompressibility.cpp:12-45) */
+#define APBF_STAGE_UPDATE_TRANSFERS 7  /* update_transfers::apply (update_transfers.cpp:14-71) */
+apbf_status apbf_scene_stage(void* scene, int32_t stage, float dt);
+/* One substep (pool::update / waterdrop::update, pool.cpp:67-106). The solver
+ * loop runs box collision + incompressibility solverIterations times. */
+apbf_status apbf_scene_update(void* scene, float dt);
+/* Mean and maximum of max(0, rho_i/rho0_i - 1) over the current positions (a
+ * density pass like incompressibility_0/1 that leaves the state untouched).
+ * Synchronises. */
+apbf_status apbf_scene_density_error(void* scene, double* mean_err, double* max_err);
+/* Overflow flags raised by capacity clamps since the last call (bit 0 pairs,
+ * bit 1 transfers, bit 2 particles); cleared by the call. Synchronises. */
+apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags);
+
+/* ------------------------------------------------------ list algorithms */
+/* Workspace sizes (bytes) for the primitives below. */
+size_t apbf_prefix_sum_workspace_size(uint32_t capacity);
+size_t apbf_radix_sort_workspace_size(uint32_t capacity);
+
+/* Inclusive prefix sum in place over d_len[0] values (algorithms::prefix_sum,
+ * algorithms.cpp:93-118). */
+apbf_status apbf_prefix_sum(uint32_t* values, const uint32_t* d_len, uint32_t capacity,
+                            void* workspace, size_t workspace_bytes, apbf_stream stream);
+
+/* Stable LSD radix sort of (key, value) pairs by key (algorithms::sort,
+ * algorithms.cpp:59-91). Digits are processed while upper_bound >> offset != 0,
+ * as in the reference; keys must be < upper_bound for a full sort. Inputs are
+ * left unchanged; outputs must not alias inputs. */
+apbf_status apbf_radix_sort(const uint32_t* keys, const uint32_t* values, const uint32_t* d_len,
+                            uint32_t capacity, uint32_t upper_bound, uint32_t* out_keys,
+                            uint32_t* out_values, void* workspace, size_t workspace_bytes,
+                            apbf_stream stream);
+
+/* z-order hash of each position in the grid [min, max] with 2^res cells per
+ * axis (shader_provider::calculate_position_hash, shader_provider.cpp:805;
+ * calculate_position_hash.comp:23-46). */
+apbf_status apbf_calculate_position_hash(const int32_t* position, const uint32_t* d_len,
+                                         uint32_t capacity, const float* host_min3,
+                                         const float* host_max3, uint32_t res_log2,
+                                         uint32_t* out_hash, apbf_stream stream);
+
+/* One 32-bit section (0..2) of the 96-bit Morton code of the raw positions
+ * (shader_provider::calculate_position_code, shader_provider.cpp:782;
+ * calculate_position_code.comp:23-72). */
+apbf_status apbf_calculate_position_code(const int32_t* position, const uint32_t* d_len,
+                                         uint32_t capacity, uint32_t section, uint32_t* out_code,
+                                         apbf_stream stream);
+
+/* Cell start/end tables over sorted keys (clears both tables of n_cells entries,
+ * then shader_provider::find_value_ranges, shader_provider.cpp:221;
+ * find_value_ranges.comp:16-31). */
+apbf_status apbf_find_value_ranges(const uint32_t* sorted_keys, const uint32_t* d_len,
+                                   uint32_t capacity, uint32_t n_cells, uint32_t* cell_start,
+                                   uint32_t* cell_end, apbf_stream stream);
+
+/* Green hash-grid neighbour search over particles sorted by cell
+ * (shader_provider::neighborhood_green, shader_provider.cpp:659;
+ * neighborhood_green.comp:49-87). Output is a CSR list: row_offset[i] ..
+ * row_offset[i+1] holds the neighbours of particle i in the reference's cell-
+ * walk order; pairs beyond pair_capacity are dropped (the reference's clamp,
+ * neighbor_add.glsl:22-23) and *d_n_pairs receives the clamped total.
+ * `workspace` >= apbf_prefix_sum_workspace_size(capacity + 1). */
+apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
+                                    const uint32_t* d_len, uint32_t capacity, float range_scale,
+                                    const float* host_min3, const float* host_max3,
+                                    uint32_t res_log2, const uint32_t* cell_start,
+                                    const uint32_t* cell_end, uint32_t* row_offset,
+                                    uint32_t* neighbors, uint32_t pair_capacity,
+                                    uint32_t* d_n_pairs, void* workspace, size_t workspace_bytes,
+                                    apbf_stream stream);
+
+/* Gather by edit list: dst[i] = src[edit[i]] for i < d_edit_len, element size
+ * `stride` bytes (a multiple of 4) (gpu_list::apply_edit, gpu_list.h:126-135;
+ * copy_scattered_read.comp:17-27). */
+apbf_status apbf_copy_scattered_read(const void* src, void* dst, const uint32_t* edit,
+                                     const uint32_t* d_edit_len, uint32_t capacity, uint32_t stride,
+                                     apbf_stream stream);
+
+/* Index-list rebuild after a hidden-list edit (indexed_list::apply_hidden_edit,
+ * indexed_list.h:289-308; atomic_swap.comp, generate_new_index_and_edit_list.comp).
+ * For each new hidden slot h (ascending) and each index-list position p with
+ * index[p] == edit[h] (descending p), appends h to new_index and p to new_edit,
+ * up to max_new entries; *d_new_len receives the length. Deterministic.
+ * `workspace` >= apbf_hidden_edit_workspace_size(index_capacity, hidden_capacity, edit_capacity). */
+size_t apbf_hidden_edit_workspace_size(uint32_t index_capacity, uint32_t hidden_capacity,
+                                       uint32_t edit_capacity);
+apbf_status apbf_apply_hidden_edit(const uint32_t* index_list, const uint32_t* d_index_len,
+                                   uint32_t index_capacity, uint32_t hidden_capacity,
+                                   const uint32_t* edit, const uint32_t* d_edit_len,
+                                   uint32_t edit_capacity, uint32_t max_new, uint32_t* new_index,
+                                   uint32_t* new_edit, uint32_t* d_new_len, void* workspace,
+                                   size_t workspace_bytes, apbf_stream stream);
+
+/* Edit list of indexed_list::delete_these (indexed_list.h:126-138): the hidden
+ * slots [0, d_hidden_len) not named in `deleted`, ascending; *d_edit_len receives
+ * the count. `workspace` >= apbf_prefix_sum_workspace_size(hidden_capacity) +
+ * 4 * hidden_capacity. */
+apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_deleted_len,
+                                  uint32_t deleted_capacity, const uint32_t* d_hidden_len,
+                                  uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len,
+                                  void* workspace, size_t workspace_bytes, apbf_stream stream);
+
+/* ------------------------------------------------------------ utilities */
+/* Library version string and the device it was built for ("gfx950"). */
+const char* apbf_version(void);
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int32_t apbf_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
