@@ -615,6 +615,7 @@ void Scene::particle_transfer(float dt) {
     cur = 1 - cur;
     LAUNCH1(k_commit_compaction, counts);
     reverse_valid = false;
+    pairs_valid = false;
 }
 
 // spread_kernel_width.cpp:12-26 on the CSR list built with range scale 1.5.
@@ -623,9 +624,11 @@ void Scene::spread_kernel_width() {
     int on_t = P.s.mBaseKernelWidthOnTargetRadius;
     LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, on_t, ukw);
     launch_set_n_plus_1(counts, s);
+    bool tk = tbegin(K_SPREAD);
     LAUNCH(k_spread<false>, stride_grid(cap, 8192), pos[cur], radius[cur], target_radius[cur],
            kernel_width[cur], counts, row, col, pcap, on_t, ukw, cnt, (const uint32_t*)nullptr,
            (uint32_t*)nullptr);
+    tend(tk);
     scan_u32(cnt, u0, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     LAUNCH(k_spread<true>, stride_grid(cap, 8192), pos[cur], radius[cur], target_radius[cur],
            kernel_width[cur], counts, row, col, pcap, on_t, ukw, (uint32_t*)nullptr, u0, col_tmp);
@@ -642,8 +645,10 @@ void Scene::update_transfers() {
     uint32_t* old_bd = u0;
     uint32_t* nearest = u1;
     LAUNCH(k_copy_u32_n, stride_grid(cap), old_bd, boundary_distance[cur], counts);
+    bool tk = tbegin(K_FSM12);
     LAUNCH(k_fsm12, stride_grid(cap, 8192), pos[cur], counts, row, col, pcap, old_bd,
            boundary_distance[cur], nearest);
+    tend(tk);
     Fsm3Args fa;
     fa.update_tr = P.s.mUpdateTargetRadius;
     fa.base_bd = P.s.mBaseKernelWidthOnBoundaryDistance;
@@ -681,6 +686,7 @@ void Scene::update_transfers() {
            t_target[tcur], t_time_left[tcur], -P.split_duration);
     LAUNCH1(k_split_counts, counts, cap, tcap);
     reverse_valid = false;
+    pairs_valid = false;
 }
 
 // ------------------------------------------------------------ state I/O
@@ -751,6 +757,7 @@ apbf_status Scene::set_state(uint32_t n, const int32_t* h_pos, const float* h_ve
     HIP_OK(hipMemset(row, 0, (size_t)(cap + 1) * 4));
     last_dt = last_delta_time;
     reverse_valid = false;
+    pairs_valid = false;
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
 }
@@ -779,15 +786,18 @@ apbf_status Scene::get_state(int32_t* h_pos, float* h_vel, float* h_im, float* h
         if (h_ttl) HIP_OK(hipMemcpy(h_ttl, t_time_left[tcur], (size_t)nt * 4, hipMemcpyDeviceToHost));
     }
     if (h_pairs) {
-        std::vector<uint32_t> hr(n + 1);
-        HIP_OK(hipMemcpy(hr.data(), row, (size_t)(n + 1) * 4, hipMemcpyDeviceToHost));
+        // rows cover the particles present when the list was built (later splits
+        // append particles without neighbours, as the reference's pair list)
+        uint32_t rows = std::min(c[C_ROWS], cap);
+        std::vector<uint32_t> hr(rows + 1);
+        HIP_OK(hipMemcpy(hr.data(), row, (size_t)(rows + 1) * 4, hipMemcpyDeviceToHost));
         uint32_t np = c[C_PAIRS];
         std::vector<uint32_t> hc(np);
         if (np) HIP_OK(hipMemcpy(hc.data(), col, (size_t)np * 4, hipMemcpyDeviceToHost));
         uint64_t w = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            uint32_t e = std::min(hr[i + 1], np);
-            for (uint32_t k = hr[i]; k < e; ++k) {
+        for (uint32_t i = 0; i < rows; ++i) {
+            uint32_t b = std::min(hr[i], np), e = std::min(hr[i + 1], np);
+            for (uint32_t k = b; k < e && w < np; ++k) {
                 h_pairs[2 * w] = i;
                 h_pairs[2 * w + 1] = hc[k];
                 ++w;

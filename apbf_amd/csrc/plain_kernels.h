@@ -56,6 +56,9 @@ void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const fl
                     int hid, int gid, const float* lambda, const int4* shift, const BoxSet& boxes,
                     int4* pos_out, hipStream_t s);
 uint32_t density_error_blocks(uint32_t cap);
+void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s);
+void launch_density_error_finish(const double* part_sum, const float* part_max, uint32_t blocks,
+                                 uint32_t* counts, double* stats, hipStream_t s);
 void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                           const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                           const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,

@@ -193,6 +193,7 @@ __global__ void k_pairs_total(uint32_t* __restrict__ counts, const uint32_t* __r
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t n = counts[C_N];
     uint32_t total = row[n];
+    counts[C_ROWS] = n;
     counts[C_PAIRS_RAW] = total;
     if (total > pair_cap) {
         counts[C_PAIRS] = pair_cap;
@@ -423,7 +424,8 @@ __global__ __launch_bounds__(BLOCK) void k_density_error(const int4* __restrict_
         v3f dummy;
         kernel_eval(hid, gid, mk(0, 0, 0), kwi, g, true, false, &w0, &dummy);
         uint32_t density = to_u32(w0 / im * INCOMP_RES);
-        const uint32_t rb = row[i], re = min(row[i + 1], pair_cap);
+        const uint32_t rows = counts[C_ROWS];
+        const uint32_t rb = i < rows ? row[i] : 0u, re = i < rows ? min(row[i + 1], pair_cap) : 0u;
         for (uint32_t k = rb; k < re; ++k) {
             const uint32_t j = col[k];
             const int4 pj = pos[j];
@@ -450,6 +452,48 @@ __global__ __launch_bounds__(BLOCK) void k_density_error(const int4* __restrict_
     if (threadIdx.x == 0) {
         part_sum[blockIdx.x] = ssum[0];
         part_max[blockIdx.x] = smax[0];
+    }
+}
+
+// Adds one substep's mean and max compression error to the running statistics
+// (stats[0] += mean, stats[1] = max(stats[1], max)); stats[2] keeps the last mean.
+__global__ __launch_bounds__(BLOCK) void k_density_error_finish(const double* __restrict__ part_sum,
+                                                                const float* __restrict__ part_max,
+                                                                uint32_t blocks, uint32_t* __restrict__ counts,
+                                                                double* __restrict__ stats) {
+    __shared__ double ssum[BLOCK];
+    __shared__ double smax[BLOCK];
+    double a = 0.0, m = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += BLOCK) {
+        a += part_sum[b];
+        m = m > (double)part_max[b] ? m : (double)part_max[b];
+    }
+    ssum[threadIdx.x] = a;
+    smax[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            smax[threadIdx.x] = smax[threadIdx.x] > smax[threadIdx.x + s] ? smax[threadIdx.x] : smax[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t n = counts[C_N];
+        double mean = n ? ssum[0] / (double)n : 0.0;
+        stats[0] += mean;
+        stats[1] = stats[1] > smax[0] ? stats[1] : smax[0];
+        stats[2] = mean;
+        stats[3] = smax[0];
+        counts[C_ERR_SUBSTEPS] += 1u;
+    }
+}
+
+__global__ void k_count_particle_substeps(const uint32_t* __restrict__ counts,
+                                          unsigned long long* __restrict__ acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        acc[0] += (unsigned long long)counts[C_ROWS];
+        acc[1] += (unsigned long long)counts[C_PAIRS];
     }
 }
 
@@ -528,6 +572,14 @@ void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const fl
            shift, boxes, pos_out);
 }
 uint32_t density_error_blocks(uint32_t cap) { return stride_grid(cap, 1024); }
+void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_particle_substeps, dim3(1), dim3(64), 0, s, counts, acc);
+}
+void launch_density_error_finish(const double* part_sum, const float* part_max, uint32_t blocks,
+                                 uint32_t* counts, double* stats, hipStream_t s) {
+    hipLaunchKernelGGL(k_density_error_finish, dim3(1), dim3(BLOCK), 0, s, part_sum, part_max, blocks, counts,
+                       stats);
+}
 void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                           const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                           const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,

@@ -1,8 +1,22 @@
 // Scene: device buffers and subsystem methods of one simulation.
 #pragma once
+#include <vector>
+
 #include "plain_kernels.h"
 
 namespace apbf {
+
+// Kernels that can be timed with HIP events (apbf_scene_time_kernel).
+enum KernelId : int {
+    K_SOLVE_A = 0,      // k_solve_a: density, lambda, boundariness, own shift (row gather)
+    K_SOLVE_B = 1,      // k_solve_b: neighbour shifts (reverse-row gather) + next box collision
+    K_GREEN_COUNT = 2,  // k_green<count>: neighbour counting walk
+    K_GREEN_FILL = 3,   // k_green<fill>: neighbour writing walk
+    K_SPREAD = 4,       // k_spread<count>: kernel-width spreading
+    K_FSM12 = 5,        // k_fsm12: boundary-distance flood fill + nearest neighbour
+    K_PERMUTE = 6,      // k_permute: particle SoA gather by the sort permutation
+    K_COUNT_IDS = 7
+};
 
 struct Scene {
     apbf_scene_params P{};
@@ -15,6 +29,23 @@ struct Scene {
     int cur = 0;           // current particle buffer set
     int tcur = 0;          // current transfer buffer set
     bool reverse_valid = false;
+    bool pairs_valid = false;    // CSR rows refer to the current particle slots
+    bool measure_error = false;  // accumulate the density error after every solver loop
+    double* err_stats = nullptr; // [sum of per-substep means, max, last mean, last max]
+    unsigned long long* particle_substeps = nullptr;  // sum over substeps of the solver's particle count
+
+    // Optional HIP-event timing of one kernel (KernelId) on the scene stream.
+    int timed_kernel = -1;
+    std::vector<hipEvent_t> events;
+    size_t events_used = 0;
+    bool tbegin(int id) {
+        if (id != timed_kernel || events_used + 2 > events.size()) return false;
+        (void)hipEventRecord(events[events_used++], stream);
+        return true;
+    }
+    void tend(bool on) {
+        if (on) (void)hipEventRecord(events[events_used++], stream);
+    }
 
     // particle SoA, two buffer sets (gathers are out of place)
     int4* pos[2] = {};
@@ -94,6 +125,7 @@ struct Scene {
     void incompressibility_iteration(const BoxSet& next_boxes);
     void solver_loop();
     void remap_transfers_after_permute();
+    void accumulate_density_error();
 };
 
 }  // namespace apbf

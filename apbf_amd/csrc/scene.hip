@@ -131,6 +131,8 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     err_blocks = density_error_blocks(cap);
     A(part_sum, err_blocks);
     A(part_max, err_blocks);
+    A(err_stats, 4);
+    A(particle_substeps, 2);
 #undef A
     size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
     ws = std::max(ws, scan_workspace_bytes(pcap));
@@ -138,6 +140,8 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     ws_bytes = ws;
     HIP_OK(hipMalloc(&ws_mem, ws_bytes));
     HIP_OK(hipMemset(counts, 0, C_COUNT * 4));
+    HIP_OK(hipMemset(err_stats, 0, 4 * sizeof(double)));
+    HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
 }
@@ -152,7 +156,8 @@ Scene::~Scene() {
     void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, rcnt, rrow,
                   rcol, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_flag, a_scan,
-                  a_list, a_t0, t_flag, t_scan};
+                  a_list, a_t0, t_flag, t_scan, err_stats, particle_substeps};
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
 }
 
@@ -177,7 +182,9 @@ void Scene::neighborhood_green(float range_scale) {
     launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s);
     radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s);
     ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
+    bool tk = tbegin(K_PERMUTE);
     launch_permute(perm, counts, cap, src, dst, s);
+    tend(tk);
     cur = 1 - cur;
     // The transfer list follows its particles (indexed_list owner rebuild).
     remap_transfers_after_permute();
@@ -190,13 +197,18 @@ void Scene::neighborhood_green(float range_scale) {
 void Scene::build_pairs(float range_scale) {
     hipStream_t s = stream;
     launch_set_n_plus_1(counts, s);
+    bool tk = tbegin(K_GREEN_COUNT);
     launch_green_count(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
                        cnt, s);
+    tend(tk);
     scan_u32(cnt, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     launch_pairs_total(counts, row, pcap, s);
+    tk = tbegin(K_GREEN_FILL);
     launch_green_fill(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
                       row, col, pcap, s);
+    tend(tk);
     reverse_valid = false;
+    pairs_valid = true;
 }
 
 void Scene::build_reverse() {
@@ -239,12 +251,16 @@ void Scene::prepare_solver() {
 void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
     hipStream_t s = stream;
     SolverArgs a = solver_args();
+    bool tk = tbegin(K_SOLVE_A);
     launch_solve_a(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col,
                    pcap, a, boundariness[cur], lambda, shift, s);
+    tend(tk);
     // The other buffer set's position array is free between permutations.
     int4* out = pos[1 - cur];
+    tk = tbegin(K_SOLVE_B);
     launch_solve_b(pos[cur], kernel_width[cur], gc, radius[cur], counts, cap, rrow, rcol, a.hid, a.gid,
                    lambda, shift, next_boxes, out, s);
+    tend(tk);
     std::swap(pos[0], pos[1]);
 }
 
@@ -272,7 +288,17 @@ void Scene::update(float dt) {
     neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
     if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
     solver_loop();
+    launch_count_particle_substeps(counts, particle_substeps, stream);
+    if (measure_error) accumulate_density_error();
     if (!P.basic_pbf) update_transfers();
+}
+
+void Scene::accumulate_density_error() {
+    hipStream_t s = stream;
+    launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
+    launch_density_error(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col,
+                         pcap, P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+    launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
 }
 
 apbf_status Scene::stage(int st, float dt) {
@@ -294,6 +320,7 @@ apbf_status Scene::stage(int st, float dt) {
 }
 
 apbf_status Scene::density_error(double* mean_err, double* max_err) {
+    if (!pairs_valid) return APBF_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream;
     launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
     launch_density_error(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row,
@@ -423,6 +450,81 @@ apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags) {
         hipMemsetAsync(s->counts + apbf::C_OVERFLOW, 0, 4, s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess)
         return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_error_measurement(void* scene, int32_t enable) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    static_cast<Scene*>(scene)->measure_error = enable != 0;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* max_err, double* last_mean,
+                                   uint32_t* substeps, int32_t reset) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    double st[4];
+    uint32_t n = 0;
+    if (hipMemcpyAsync(st, s->err_stats, sizeof(st), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipMemcpyAsync(&n, s->counts + apbf::C_ERR_SUBSTEPS, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (mean_of_means) *mean_of_means = n ? st[0] / (double)n : 0.0;
+    if (max_err) *max_err = st[1];
+    if (last_mean) *last_mean = st[2];
+    if (substeps) *substeps = n;
+    if (reset) {
+        if (hipMemsetAsync(s->err_stats, 0, sizeof(st), s->stream) != hipSuccess ||
+            hipMemsetAsync(s->counts + apbf::C_ERR_SUBSTEPS, 0, 4, s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess)
+            return APBF_ERR_HIP;
+    }
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t* pairs_total, int32_t reset) {
+    if (!scene || !total) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    unsigned long long v[2] = {0, 0};
+    if (hipMemcpyAsync(v, s->particle_substeps, 16, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    *total = v[0];
+    if (pairs_total) *pairs_total = v[1];
+    if (reset && (hipMemsetAsync(s->particle_substeps, 0, 16, s->stream) != hipSuccess ||
+                  hipStreamSynchronize(s->stream) != hipSuccess))
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches) {
+    if (!scene || kernel_id >= apbf::K_COUNT_IDS) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
+    for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+    s->events.clear();
+    s->events_used = 0;
+    s->timed_kernel = kernel_id;
+    if (kernel_id < 0) return APBF_OK;
+    s->events.resize((size_t)max_launches * 2);
+    for (auto& e : s->events)
+        if (hipEventCreate(&e) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches) {
+    if (!scene || !total_ms || !launches) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
+    double t = 0.0;
+    for (size_t i = 0; i + 1 < s->events_used; i += 2) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s->events[i], s->events[i + 1]) != hipSuccess) return APBF_ERR_HIP;
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = (uint32_t)(s->events_used / 2);
+    s->events_used = 0;
     return APBF_OK;
 }
 

@@ -22,6 +22,8 @@ enum CountSlot : uint32_t {
     C_MERGES = 11,     // merges won in find_split_and_merge_3
     C_TMP0 = 12,
     C_TMP1 = 13,
+    C_ROWS = 14,       // particles covered by the CSR rows (n when the list was built)
+    C_ERR_SUBSTEPS = 15,  // substeps accumulated in the density-error statistics
     C_COUNT = 16
 };
 

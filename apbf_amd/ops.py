@@ -25,8 +25,20 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_keep: list = []
+
+
 def _len(n: int, device) -> torch.Tensor:
-    return torch.tensor([n], dtype=torch.int32, device=device)
+    """Device-resident u32 length. The tensor is kept alive until the next
+    synchronisation (the library reads it asynchronously)."""
+    t = torch.tensor([n], dtype=torch.int32, device=device)
+    _keep.append(t)
+    return t
+
+
+def _sync():
+    torch.cuda.current_stream().synchronize()
+    _keep.clear()
 
 
 def _ws(nbytes: int, device) -> torch.Tensor:
@@ -47,6 +59,7 @@ def prefix_sum(values: torch.Tensor, n: int | None = None) -> torch.Tensor:
     ws = _ws(L.apbf_prefix_sum_workspace_size(cap), values.device)
     check(L.apbf_prefix_sum(_p(values), _p(_len(n, values.device)), cap, _p(ws), ws.numel(), _stream()),
           "apbf_prefix_sum")
+    _sync()
     return values
 
 
@@ -63,6 +76,7 @@ def radix_sort(keys: torch.Tensor, values: torch.Tensor, upper_bound: int = 0xFF
     ws = _ws(L.apbf_radix_sort_workspace_size(cap), keys.device)
     check(L.apbf_radix_sort(_p(keys), _p(values), _p(_len(n, keys.device)), cap, upper_bound & 0xFFFFFFFF,
                             _p(ok), _p(ov), _p(ws), ws.numel(), _stream()), "apbf_radix_sort")
+    _sync()
     return ok, ov
 
 
@@ -74,6 +88,7 @@ def position_hash(position: torch.Tensor, mn, mx, res: int) -> torch.Tensor:
     fmx = (C.c_float * 3)(*mx)
     check(L.apbf_calculate_position_hash(_p(position), _p(_len(n, position.device)), n, fmn, fmx, res,
                                          _p(out), _stream()), "apbf_calculate_position_hash")
+    _sync()
     return out
 
 
@@ -83,6 +98,7 @@ def position_code(position: torch.Tensor, section: int) -> torch.Tensor:
     out = torch.empty(n, dtype=torch.int32, device=position.device)
     check(L.apbf_calculate_position_code(_p(position), _p(_len(n, position.device)), n, section, _p(out),
                                          _stream()), "apbf_calculate_position_code")
+    _sync()
     return out
 
 
@@ -93,6 +109,7 @@ def find_value_ranges(sorted_keys: torch.Tensor, n_cells: int):
     ce = torch.empty(n_cells, dtype=torch.int32, device=sorted_keys.device)
     check(L.apbf_find_value_ranges(_p(sorted_keys), _p(_len(n, sorted_keys.device)), n, n_cells, _p(cs),
                                    _p(ce), _stream()), "apbf_find_value_ranges")
+    _sync()
     return cs, ce
 
 
@@ -110,6 +127,7 @@ def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx,
     check(L.apbf_neighborhood_green(_p(position), _p(rng), _p(_len(n, dev)), n, float(range_scale), fmn, fmx,
                                     res, _p(cell_start), _p(cell_end), _p(row), _p(col), pair_cap, _p(npairs),
                                     _p(ws), ws.numel(), _stream()), "apbf_neighborhood_green")
+    _sync()
     return row, col, npairs
 
 
@@ -119,6 +137,7 @@ def copy_scattered_read(src: torch.Tensor, edit: torch.Tensor, stride: int, out_
     dst = torch.empty(max(n * stride // 4, 1), dtype=torch.int32, device=src.device)
     check(L.apbf_copy_scattered_read(_p(src), _p(dst), _p(edit), _p(_len(n, src.device)), edit.numel(),
                                      stride, _stream()), "apbf_copy_scattered_read")
+    _sync()
     return dst[: n * stride // 4]
 
 
@@ -137,6 +156,7 @@ def apply_hidden_edit(index_list: torch.Tensor, n_hidden: int, edit: torch.Tenso
                                    cap_e, max_new, _p(ni), _p(ne), _p(nl), _p(ws), ws.numel(), _stream()),
           "apbf_apply_hidden_edit")
     k = int(nl.item())
+    _sync()
     return ni[:k], ne[:k]
 
 
@@ -151,4 +171,5 @@ def delete_edit_list(deleted: torch.Tensor, n_hidden: int):
     check(L.apbf_delete_edit_list(_p(d), _p(_len(deleted.numel(), dev)), max(d.numel(), 1),
                                   _p(_len(n_hidden, dev)), cap, _p(edit), _p(el), _p(ws), ws.numel(), _stream()),
           "apbf_delete_edit_list")
+    _sync()
     return edit[: int(el.item())]

@@ -120,6 +120,38 @@ class Scene:
         check(self._L.apbf_scene_density_error(self._h, C.byref(m), C.byref(x)), "density_error")
         return m.value, x.value
 
+    def set_error_measurement(self, enable: bool) -> None:
+        check(self._L.apbf_scene_set_error_measurement(self._h, int(enable)), "set_error_measurement")
+
+    def error_stats(self, reset: bool = False):
+        """(mean over substeps of the mean density error, max error, last mean, substeps)."""
+        m, x, l = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_uint32()
+        check(self._L.apbf_scene_error_stats(self._h, C.byref(m), C.byref(x), C.byref(l), C.byref(n),
+                                             int(reset)), "error_stats")
+        return m.value, x.value, l.value, n.value
+
+    def particle_substeps(self, reset: bool = False):
+        """(sum of particle counts, sum of pair counts) over substeps since the last reset."""
+        v, p = C.c_uint64(), C.c_uint64()
+        check(self._L.apbf_scene_particle_substeps(self._h, C.byref(v), C.byref(p), int(reset)),
+              "particle_substeps")
+        return v.value, p.value
+
+    KERNELS = {"solve_a": 0, "solve_b": 1, "green_count": 2, "green_fill": 3, "spread": 4, "fsm12": 5,
+               "permute": 6}
+
+    def time_kernel(self, name: str | None, max_launches: int = 4096) -> None:
+        kid = -1 if name is None else self.KERNELS[name]
+        check(self._L.apbf_scene_time_kernel(self._h, kid, max_launches), "time_kernel")
+
+    def kernel_time(self):
+        """(total milliseconds, launches) of the timed kernel since the last call."""
+        t = C.c_double()
+        n = C.c_uint32()
+        check(self._L.apbf_scene_kernel_time(self._h, C.byref(t), C.byref(n)), "kernel_time")
+        return t.value, n.value
+
     def overflow_flags(self) -> int:
         f = C.c_uint32()
         check(self._L.apbf_scene_overflow_flags(self._h, C.byref(f)), "overflow_flags")

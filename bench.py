@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-substeps/s of the adaptive water drop (1M particles:
+900k coarse pool particles r = 1h and a 100k fine-particle drop r = 0.5h, split,
+merge and kernel-width spreading on, 4 solver iterations, dt = 1/120) on one
+MI355X, with the mean density error of the timed substeps.
+
+One step = one substep (the reference's waterdrop::update) over the whole
+scene, inputs resident in HBM. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of each timeable kernel, per particle and per neighbour
+# pair, from the fixed byte model of the substep (unpadded elements, inputs read
+# and outputs written once, neighbour gathers assumed cache-served).
+BYTE_MODEL = {
+    "solve_a": (36.0, 8.0),      # density / lambda pass
+    "solve_b": (28.0, 8.0),      # position-delta pass
+    "green_fill": (16.0, 8.0),   # pair build
+    "green_count": (16.0, 0.0),  # counting walk of the pair build (reads the particle, writes a count)
+    "spread": (24.0, 16.0),      # kernel-width spreading
+    "fsm12": (38.0, 8.0),        # split/merge find, pair part
+    "permute": (100.0, 0.0),     # permute 48 B of state: index 4 + gather 48 + write 48
+}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def workload(name: str):
+    from apbf_amd import config as cfg
+    if name == "water_drop":
+        return cfg.water_drop()
+    if name == "dam_break":
+        return cfg.dam_break(n_side=32)
+    if name == "small_water_drop":
+        return cfg.small_water_drop()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(params, state, seconds: float):
+    """The scalar oracle (single thread) on the same workload state, for a bounded
+    number of substeps (at least one, then until `seconds` have passed)."""
+    from oracle import oracle as O
+    O.build()
+    sc = O.OracleScene(params)
+    sc.set_state(state)
+    t0 = time.perf_counter()
+    ps = 0
+    steps = 0
+    while True:
+        n = sc.counts()[1]
+        sc.update(params["dt"])
+        ps += n
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": ps / dt, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} substep(s) of the same scene from the post-warmup state, "
+                      f"{ps} particle-substeps in {dt:.1f} s on 1 thread of '{cpu_model()}' "
+                      f"(nproc {os.cpu_count()}), g++ -O2 -ffp-contract=off"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="water_drop")
+    ap.add_argument("--timed-kernel", default="solve_a", choices=sorted(BYTE_MODEL))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build is single-GPU (the multi-GPU design is "
+              "domain decomposition, not implemented); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from apbf_amd.scene import Scene
+
+    params, state = workload(args.workload)
+    dt = params["dt"]
+    torch.cuda.init()
+    scene = Scene(params)
+    scene.set_state(state)
+    scene.set_error_measurement(True)
+
+    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    log(f"scene {params.get('workload')} with {len(state['radius'])} particles ready")
+    for _ in range(args.warmup):
+        scene.update(dt)
+    torch.cuda.synchronize()
+    log("warmup done")
+    scene.error_stats(reset=True)
+    scene.particle_substeps(reset=True)
+    scene.time_kernel(args.timed_kernel, max_launches=args.steps * (params["solver_iterations"] + 2))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        scene.update(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    elapsed = t1 - t0
+    log(f"timed {args.steps} substeps in {elapsed:.3f} s")
+    ps, pairs = scene.particle_substeps()
+    k_ms, k_launches = scene.kernel_time()
+    scene.time_kernel(None)
+    mean_err, max_err, last_err, err_steps = scene.error_stats()
+    n_now, nt_now, np_now = scene.counts()
+    value = ps / elapsed
+
+    # roofline of the timed kernel: algorithmic bytes per launch / mean launch time
+    per_p, per_pair = BYTE_MODEL[args.timed_kernel]
+    launches_per_step = k_launches / args.steps if args.steps else 0
+    mean_n = ps / args.steps
+    mean_pairs = pairs / args.steps
+    bytes_per_launch = per_p * mean_n + per_pair * mean_pairs
+    avg_launch_s = (k_ms / 1e3) / k_launches if k_launches else float("nan")
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        st = scene.get_state(pairs=False)
+        st["fluid_index"] = st["fluid_index"][: len(st["radius"])]
+        cpu = cpu_baseline(params, st, args.cpu_seconds)
+
+    out = {
+        "metric": "particle-substeps/s, 1M adaptive particles at <=0.1% density error, plain vs learned",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+i32",
+        "data": "synthetic",
+        "config": {
+            "workload": params.get("workload", args.workload),
+            "particles_start": int(len(state["radius"])),
+            "particles_end": int(n_now),
+            "mean_particles": mean_n,
+            "mean_pairs": mean_pairs,
+            "transfers_end": int(nt_now),
+            "solver_iterations": params["solver_iterations"],
+            "dt": dt,
+            "hash_res_log2": params["hash_res_log2"],
+            "solver": "plain (cold-start lambda)",
+        },
+        "density_error": {"mean_of_substep_means": mean_err, "max": max_err, "substeps": err_steps},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": args.timed_kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "launches_per_step": launches_per_step,
+            "avg_launch_ms": avg_launch_s * 1e3 if k_launches else None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -138,8 +138,32 @@ apbf_status apbf_scene_stage(void* scene, int32_t stage, float dt);
 apbf_status apbf_scene_update(void* scene, float dt);
 /* Mean and maximum of max(0, rho_i/rho0_i - 1) over the current positions (a
  * density pass like incompressibility_0/1 that leaves the state untouched).
- * Synchronises. */
+ * Needs a neighbour list that refers to the current particle slots (after the
+ * neighbourhood stage and before particle_transfer / update_transfers edit the
+ * particle list); returns APBF_ERR_INVALID_ARGUMENT otherwise. Synchronises. */
 apbf_status apbf_scene_density_error(void* scene, double* mean_err, double* max_err);
+/* Density-error statistics accumulated inside apbf_scene_update: when enabled,
+ * every substep ends its solver loop with a density pass (as above) whose mean
+ * and max are added to device-resident accumulators, without synchronising.
+ * apbf_scene_error_stats returns the mean over substeps of the per-substep mean
+ * error, the maximum error, the last substep's mean and the substep count, and
+ * clears them when `reset` is non-zero (synchronises). */
+apbf_status apbf_scene_set_error_measurement(void* scene, int32_t enable);
+apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* max_err,
+                                   double* last_mean, uint32_t* substeps, int32_t reset);
+/* Sums over substeps of the number of particles the solver ran on (the unit of
+ * the particle-substeps/s metric) and of the neighbour pairs, kept on the
+ * device; cleared when `reset` is non-zero. Synchronises. */
+apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t* pairs_total,
+                                         int32_t reset);
+/* Time every launch of one kernel with HIP events on the scene stream:
+ * kernel_id 0 k_solve_a, 1 k_solve_b, 2 neighbour count walk, 3 neighbour fill
+ * walk, 4 kernel-width spreading, 5 boundary-distance flood fill, 6 particle
+ * permutation; -1 stops timing. At most max_launches launches are recorded.
+ * apbf_scene_kernel_time returns the summed duration and the launch count of
+ * the recorded launches and restarts the recording. Both synchronise. */
+apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches);
+apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches);
 /* Overflow flags raised by capacity clamps since the last call (bit 0 pairs,
  * bit 1 transfers, bit 2 particles); cleared by the call. Synchronises. */
 apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags);
