@@ -150,52 +150,62 @@ def _sphere_points(center, radius_world, spacing, jitter, rng):
 
 
 def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coarse: float = 1.0,
-               r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6,
-               cap_factor: float = 3.0, pairs_per_particle: int = 160):
-    """Adaptive water drop: a sphere of fine particles (radius r_fine) above a pool
-    of coarse particles (radius r_coarse) filling the box floor. Split, merge and
-    kernel-width spreading on (baseKernelWidthOnBoundaryDistance off, neighbour
-    range scale 1.5), released at t = 0. Target radius grows as
-    r_fine + 0.05 * boundary distance (targetRadiusOffset 0), which keeps radii
-    within [r_fine, 4 r_fine] for a pool 80h deep."""
+               r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6, cap_factor: float = 1.5,
+               pairs_per_particle: int = 160, target_scale: float | None = None):
+    """Adaptive water drop: a sphere of `drop_particles` fine particles (radius
+    r_fine = 0.5h) released at t = 0 above a pool of coarse particles (radius
+    r_coarse = 1h, a pool_counts lattice at spacing 2h: 150 x 40 x 150 = 900 000
+    particles in a 300 x 80 x 300 h box). Split, merge and kernel-width spreading
+    on (baseKernelWidthOnBoundaryDistance off, neighbour range scale 1.5),
+    4 solver iterations. smallestTargetRadius = r_coarse (the reference scenes use
+    their base radius, pool.cpp:255 / waterdrop.cpp:54); target radius
+    r_coarse + s * boundary distance with targetRadiusOffset 0 and s chosen so it
+    reaches 2 r_coarse at the pool's centre depth, so radii stay in [0.5h, 2h].
+    Initial boundary distances are the geometric distances to the pool faces /
+    the drop surface, boundariness 1 within one particle diameter of them."""
     rng = np.random.default_rng(seed)
     sc = 2 * r_coarse
-    pool_max = (pool_counts[0] * sc, pool_counts[1] * sc, pool_counts[2] * sc)
+    pool_max = tuple(c * sc for c in pool_counts)
     pool = lattice((0, 0, 0), pool_counts, sc, 0.05, rng)
     sf = 2 * r_fine
-    drop_r = (drop_particles * sf ** 3 * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
-    drop_r += sf * 0.5
+    drop_r = (drop_particles * sf ** 3 * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) + sf * 0.5
     center = (pool_max[0] * 0.5, pool_max[1] + 8.0 + drop_r, pool_max[2] * 0.5)
     drop = _sphere_points(center, drop_r, sf, 0.05, rng)
     pos = np.concatenate([pool, drop], 0)
     radius = np.concatenate([np.full(len(pool), r_coarse), np.full(len(drop), r_fine)]).astype(np.float32)
-    # boundary distance: distance to the pool's free surface or walls, resp. to the drop surface
-    d_pool = np.minimum.reduce([pool[:, 0], pool_max[0] - pool[:, 0], pool[:, 1],
-                                pool_max[1] - pool[:, 1], pool[:, 2], pool_max[2] - pool[:, 2]])
+    d_pool = np.minimum.reduce([pool[:, 0], pool_max[0] - pool[:, 0], pool[:, 1], pool_max[1] - pool[:, 1],
+                                pool[:, 2], pool_max[2] - pool[:, 2]])
     d_drop = drop_r - np.linalg.norm(drop - np.asarray(center), axis=1)
     bd = np.maximum(np.concatenate([d_pool, d_drop]), radius)
+    if target_scale is None:
+        target_scale = r_coarse / (0.5 * min(pool_max))
     s = dict(REFERENCE_SETTINGS)
-    s.update(mBaseKernelWidthOnBoundaryDistance=0, mSmallestTargetRadius=r_fine,
-             mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=0.05)
-    tr = np.maximum(r_fine + np.maximum(0.0, bd * 0.05), r_fine).astype(np.float32)
+    s.update(mBaseKernelWidthOnBoundaryDistance=0, mSmallestTargetRadius=r_coarse,
+             mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=target_scale)
+    tr = (r_coarse + bd * target_scale).astype(np.float32)
     bness = (bd < 2.0 * radius).astype(np.float32)
     st = make_state(pos, radius, bd=bd, boundariness=bness, target_radius=tr)
     n = len(pos)
     top = center[1] + drop_r + 4
-    hmin = (-4.0, -4.0, -4.0)
-    hmax = (pool_max[0] + 4.0, top, pool_max[2] + 4.0)
     cap = int(n * cap_factor)
     p = base_params(
         settings=s, basic_pbf=0, solver_iterations=iterations,
-        hash_min=hmin, hash_max=hmax, hash_res_log2=res,
+        hash_min=(-4.0, -4.0, -4.0), hash_max=(pool_max[0] + 4.0, top, pool_max[2] + 4.0), hash_res_log2=res,
         boxes=walls((0, 0, 0), pool_max, r_coarse, ceiling=False),
-        cap_particles=cap, cap_transfers=max(1000, n // 5), cap_pairs=cap * pairs_per_particle,
+        cap_particles=cap, cap_transfers=max(1000, n // 4), cap_pairs=cap * pairs_per_particle,
         workload=f"water_drop_{n}",
     )
     return p, st
 
 
-def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: int = 2000, res: int = 4):
-    """A few-thousand-particle version of :func:`water_drop` for parity tests."""
-    return water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, res=res,
-                      pairs_per_particle=400)
+def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: int = 2000, res: int = 4,
+                     active: bool = False):
+    """A ~8k-particle version of :func:`water_drop` for parity tests. With
+    active=True the smallest target radius is the fine radius, so most pool
+    particles split (a stress case for split, transfer and claim handling)."""
+    p, st = water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, res=res,
+                       pairs_per_particle=400 if active else 200, cap_factor=3.0 if active else 2.0)
+    if active:
+        p["settings"]["mSmallestTargetRadius"] = 0.5
+        p["settings"]["mTargetRadiusScaleFactor"] = 0.05
+    return p, st

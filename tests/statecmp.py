@@ -18,6 +18,8 @@ def first_diff(a, b):
     a, b = bits(a), bits(b)
     if a.shape != b.shape:
         return f"shape {a.shape} vs {b.shape}"
+    if a.size == 0:
+        return None
     d = np.nonzero((a != b).reshape(len(a), -1).any(axis=1))[0] if a.ndim > 1 else np.nonzero(a != b)[0]
     if len(d) == 0:
         return None

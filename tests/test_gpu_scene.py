@@ -73,8 +73,9 @@ def test_dam_break_substeps(device):
     assert abs(m_g - m_r) <= 1e-12 + 1e-9 * abs(m_r)
 
 
-def test_water_drop_stages(device):
-    p, st = cfg.small_water_drop()
+@pytest.mark.parametrize("active", [False, True])
+def test_water_drop_stages(device, active):
+    p, st = cfg.small_water_drop(active=active)
     g, r = _pair(p, st)
     dt = p["dt"]
     for step in range(4):
@@ -84,20 +85,22 @@ def test_water_drop_stages(device):
             _check(g, r, f"substep {step} after {name}")
 
 
-def test_water_drop_substeps(device):
-    p, st = cfg.small_water_drop(seed=3)
+@pytest.mark.parametrize("active,steps", [(False, 30), (True, 10)])
+def test_water_drop_substeps(device, active, steps):
+    p, st = cfg.small_water_drop(seed=3, active=active)
     g, r = _pair(p, st)
-    for step in range(10):
+    transfers = 0
+    for step in range(steps):
         g.update(p["dt"])
         r.update(p["dt"])
-        _check(g, r, f"substep {step}")
-    gs = g.get_state()
-    assert len(gs["radius"]) > len(st["radius"])  # splits happened
+        gs, _ = _check(g, r, f"substep {step}")
+        transfers = max(transfers, len(gs["t_source"]))
+    assert transfers > 0  # merges or splits were in flight
 
 
 def test_water_drop_boundary_distance_mode(device):
     """Reference default mode: kernel width from boundary distance, range scale 1."""
-    p, st = cfg.small_water_drop(seed=5)
+    p, st = cfg.small_water_drop(seed=5, active=True)
     p["settings"]["mBaseKernelWidthOnBoundaryDistance"] = 1
     p["settings"]["mTargetRadiusScaleFactor"] = 0.3
     g, r = _pair(p, st)
@@ -110,7 +113,7 @@ def test_water_drop_boundary_distance_mode(device):
 def test_transfer_capacity_clamp(device):
     """A tiny transfer capacity forces the clamps of find_split_and_merge_3 and
     remove_impossible_splits (and the sequential claim replay)."""
-    p, st = cfg.small_water_drop(seed=2)
+    p, st = cfg.small_water_drop(seed=2, active=True)
     p["cap_transfers"] = 40
     g, r = _pair(p, st)
     for step in range(5):
