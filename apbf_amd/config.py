@@ -151,7 +151,7 @@ def _sphere_points(center, radius_world, spacing, jitter, rng):
 
 def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coarse: float = 1.0,
                r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6, cap_factor: float = 1.5,
-               pairs_per_particle: int = 160, target_scale: float | None = None):
+               pairs_per_particle: int = 400, target_scale: float | None = None):
     """Adaptive water drop: a sphere of `drop_particles` fine particles (radius
     r_fine = 0.5h) released at t = 0 above a pool of coarse particles (radius
     r_coarse = 1h, a pool_counts lattice at spacing 2h: 150 x 40 x 150 = 900 000

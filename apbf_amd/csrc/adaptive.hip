@@ -176,50 +176,6 @@ __global__ __launch_bounds__(BLOCK) void k_kw_init(const float* __restrict__ rad
     }
 }
 
-// kernel_width.comp:22-56 over row i: spread the original kernel width to each
-// neighbour (atomicMax, order-free) and keep the pairs within the cutoff. FILL
-// writes the kept neighbours from row2[i]; otherwise counts them.
-template <bool FILL>
-__global__ __launch_bounds__(BLOCK) void k_spread(const int4* __restrict__ pos,
-                                                  const float* __restrict__ radius,
-                                                  const float* __restrict__ trad,
-                                                  const float* __restrict__ kw_old,
-                                                  const uint32_t* __restrict__ counts,
-                                                  const uint32_t* __restrict__ row,
-                                                  const uint32_t* __restrict__ col, uint32_t pcap,
-                                                  int on_target, uint32_t* __restrict__ ukw,
-                                                  uint32_t* __restrict__ cnt2,
-                                                  const uint32_t* __restrict__ row2,
-                                                  uint32_t* __restrict__ col2) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        const int4 pi = pos[i];
-        const float tr = on_target ? trad[i] : 0.0f;
-        const float okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
-        const float cutoff = fmax_g(okw, kw_old[i]);
-        const uint32_t rb = row[i], re = min(row[i + 1], pcap);
-        uint32_t c = 0;
-        const uint32_t base = FILL ? row2[i] : 0u;
-        for (uint32_t k = rb; k < re; ++k) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float dist = length(diff);
-            if (!FILL) {
-                float dfk = dist - okw;
-                float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
-                atomicMax(&ukw[j], to_u32(okw * infl * KW_RES));
-            }
-            if (dist <= cutoff) {
-                if (FILL) col2[base + c] = j;
-                ++c;
-            }
-        }
-        if (!FILL) cnt2[i] = c;
-    }
-    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cnt2[n] = 0;
-}
-
 // uint_to_float_but_gradual.comp:27-36 with the default lower bound -inf.
 __global__ __launch_bounds__(BLOCK) void k_kw_gradual(const uint32_t* __restrict__ ukw,
                                                       float* __restrict__ kw,
@@ -234,39 +190,6 @@ __global__ __launch_bounds__(BLOCK) void k_kw_gradual(const uint32_t* __restrict
 
 __global__ void k_set_pairs_from_row(uint32_t* __restrict__ counts, const uint32_t* __restrict__ row) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counts[C_PAIRS] = row[counts[C_N]];
-}
-
-// find_split_and_merge_1.comp:20-31 and find_split_and_merge_2.comp:20-32 over
-// row i: boundary-distance flood fill, shortest neighbour distance and the last
-// neighbour in row order at that distance. A particle without neighbours keeps
-// UINT_MAX and names itself as nearest neighbour.
-__global__ __launch_bounds__(BLOCK) void k_fsm12(const int4* __restrict__ pos,
-                                                 const uint32_t* __restrict__ counts,
-                                                 const uint32_t* __restrict__ row,
-                                                 const uint32_t* __restrict__ col, uint32_t pcap,
-                                                 const uint32_t* __restrict__ old_bd,
-                                                 uint32_t* __restrict__ bd, uint32_t* __restrict__ nearest) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        const int4 pi = pos[i];
-        uint32_t b = 0xFFFFFFFFu, best = 0xFFFFFFFFu, nn = i;
-        const uint32_t rb = row[i], re = min(row[i + 1], pcap);
-        for (uint32_t k = rb; k < re; ++k) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            uint32_t dist = to_u32(length(vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z))));
-            uint32_t cand = old_bd[j] + dist;
-            b = cand < b ? cand : b;
-            if (dist < best) {
-                best = dist;
-                nn = j;
-            } else if (dist == best) {
-                nn = j;
-            }
-        }
-        bd[i] = b;
-        nearest[i] = nn;
-    }
 }
 
 struct Fsm3Args {
@@ -614,7 +537,6 @@ void Scene::particle_transfer(float dt) {
     LAUNCH(k_compact_particles, stride_grid(cap), del_p, u3, counts, arrays(cur), arrays(1 - cur));
     cur = 1 - cur;
     LAUNCH1(k_commit_compaction, counts);
-    reverse_valid = false;
     pairs_valid = false;
 }
 
@@ -625,18 +547,16 @@ void Scene::spread_kernel_width() {
     LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, on_t, ukw);
     launch_set_n_plus_1(counts, s);
     bool tk = tbegin(K_SPREAD);
-    LAUNCH(k_spread<false>, stride_grid(cap, 8192), pos[cur], radius[cur], target_radius[cur],
-           kernel_width[cur], counts, row, col, pcap, on_t, ukw, cnt, (const uint32_t*)nullptr,
-           (uint32_t*)nullptr);
+    launch_spread_rows(false, pos[cur], radius[cur], target_radius[cur], kernel_width[cur], counts, cap, row, col, pcap,
+                       on_t, ukw, cnt, nullptr, nullptr, s);
     tend(tk);
     scan_u32(cnt, u0, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    LAUNCH(k_spread<true>, stride_grid(cap, 8192), pos[cur], radius[cur], target_radius[cur],
-           kernel_width[cur], counts, row, col, pcap, on_t, ukw, (uint32_t*)nullptr, u0, col_tmp);
+    launch_spread_rows(true, pos[cur], radius[cur], target_radius[cur], kernel_width[cur], counts, cap, row, col, pcap,
+                       on_t, ukw, nullptr, u0, col_tmp, s);
     std::swap(row, u0);
     std::swap(col, col_tmp);
     LAUNCH1(k_set_pairs_from_row, counts, row);
     LAUNCH(k_kw_gradual, stride_grid(cap), ukw, kernel_width[cur], counts, P.s.mKernelWidthAdaptionSpeed);
-    reverse_valid = false;
 }
 
 // update_transfers.cpp:14-71
@@ -646,8 +566,7 @@ void Scene::update_transfers() {
     uint32_t* nearest = u1;
     LAUNCH(k_copy_u32_n, stride_grid(cap), old_bd, boundary_distance[cur], counts);
     bool tk = tbegin(K_FSM12);
-    LAUNCH(k_fsm12, stride_grid(cap, 8192), pos[cur], counts, row, col, pcap, old_bd,
-           boundary_distance[cur], nearest);
+    launch_fsm12_rows(pos[cur], counts, cap, row, col, pcap, old_bd, boundary_distance[cur], nearest, s);
     tend(tk);
     Fsm3Args fa;
     fa.update_tr = P.s.mUpdateTargetRadius;
@@ -685,7 +604,6 @@ void Scene::update_transfers() {
     LAUNCH(k_split_finish, stride_grid(cap), u4, counts, cap, tcap, arrays(cur), t_source[tcur],
            t_target[tcur], t_time_left[tcur], -P.split_duration);
     LAUNCH1(k_split_counts, counts, cap, tcap);
-    reverse_valid = false;
     pairs_valid = false;
 }
 
@@ -751,12 +669,12 @@ apbf_status Scene::set_state(uint32_t n, const int32_t* h_pos, const float* h_ve
     uint32_t c[C_COUNT] = {0};
     c[C_N] = n;
     c[C_N_PLUS_1] = n + 1;
+    c[C_N4] = 4 * n;
     c[C_TRANSFERS] = nt;
     HIP_OK(hipMemcpy(counts, c, sizeof(c), hipMemcpyHostToDevice));
     // no neighbour list yet: empty rows
     HIP_OK(hipMemset(row, 0, (size_t)(cap + 1) * 4));
     last_dt = last_delta_time;
-    reverse_valid = false;
     pairs_valid = false;
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;

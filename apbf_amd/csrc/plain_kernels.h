@@ -39,29 +39,32 @@ void launch_green_fill(const int4* pos, const float* range, const uint32_t* coun
                        const uint32_t* row, uint32_t* col, uint32_t pair_cap, hipStream_t s);
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
-void launch_rev_count(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
-                      uint32_t pair_cap, uint32_t* rcnt, hipStream_t s);
-void launch_rev_fill(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
-                     uint32_t pair_cap, uint32_t* rcur, uint32_t* rcol, hipStream_t s);
 void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
                           hipStream_t s);
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s);
-void launch_solve_a(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                    const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                    const uint32_t* col, uint32_t pair_cap, const SolverArgs& a, float* boundariness,
-                    float* lambda, int4* shift, hipStream_t s);
-void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const float* radius,
-                    const uint32_t* counts, uint32_t cap, const uint32_t* rrow, const uint32_t* rcol,
-                    int hid, int gid, const float* lambda, const int4* shift, const BoxSet& boxes,
-                    int4* pos_out, hipStream_t s);
-uint32_t density_error_blocks(uint32_t cap);
 void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s);
 void launch_density_error_finish(const double* part_sum, const float* part_max, uint32_t blocks,
                                  uint32_t* counts, double* stats, hipStream_t s);
-void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                          const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                          const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,
-                          float* part_max, hipStream_t s);
+
+// rows.hip: kernels over CSR rows with a group of lanes per row
+void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                       const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
+                       float* lambda, int4* acc, hipStream_t s);
+void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
+                        const BoxSet& boxes, hipStream_t s);
+uint32_t density_rows_blocks(uint32_t cap);
+void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                         const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                         const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
+                         float* part_max, hipStream_t s);
+void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,
+                       hipStream_t s);
+void launch_spread_rows(bool fill, const int4* pos, const float* radius, const float* trad, const float* kw_old,
+                        const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col, uint32_t pcap,
+                        int on_target, uint32_t* ukw, uint32_t* cnt2, const uint32_t* row2, uint32_t* col2,
+                        hipStream_t s);
 
 }  // namespace apbf

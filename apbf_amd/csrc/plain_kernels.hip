@@ -195,6 +195,7 @@ __global__ void k_pairs_total(uint32_t* __restrict__ counts, const uint32_t* __r
     uint32_t total = row[n];
     counts[C_ROWS] = n;
     counts[C_PAIRS_RAW] = total;
+    if (total > counts[C_PAIRS_PEAK]) counts[C_PAIRS_PEAK] = total;
     if (total > pair_cap) {
         counts[C_PAIRS] = pair_cap;
         counts[C_OVERFLOW] |= 1u;
@@ -204,28 +205,9 @@ __global__ void k_pairs_total(uint32_t* __restrict__ counts, const uint32_t* __r
 }
 
 __global__ void k_set_n_plus_1(uint32_t* __restrict__ counts) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) counts[C_N_PLUS_1] = counts[C_N] + 1u;
-}
-
-// Reverse adjacency: rcnt[j] = number of rows that name j.
-__global__ __launch_bounds__(BLOCK) void k_rev_count(const uint32_t* __restrict__ counts,
-                                                     const uint32_t* __restrict__ row,
-                                                     const uint32_t* __restrict__ col, uint32_t pair_cap,
-                                                     uint32_t* __restrict__ rcnt) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        uint32_t b = row[i], e = min(row[i + 1], pair_cap);
-        for (uint32_t k = b; k < e; ++k) atomicAdd(&rcnt[col[k]], 1u);
-    }
-}
-__global__ __launch_bounds__(BLOCK) void k_rev_fill(const uint32_t* __restrict__ counts,
-                                                    const uint32_t* __restrict__ row,
-                                                    const uint32_t* __restrict__ col, uint32_t pair_cap,
-                                                    uint32_t* __restrict__ rcur, uint32_t* __restrict__ rcol) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        uint32_t b = row[i], e = min(row[i + 1], pair_cap);
-        for (uint32_t k = b; k < e; ++k) rcol[atomicAdd(&rcur[col[k]], 1u)] = i;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counts[C_N_PLUS_1] = counts[C_N] + 1u;
+        counts[C_N4] = 4u * counts[C_N];
     }
 }
 
@@ -245,214 +227,6 @@ __global__ __launch_bounds__(BLOCK) void k_box(int4* __restrict__ pos, const flo
                                                const uint32_t* __restrict__ counts, BoxSet boxes) {
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) pos[i] = box_collide(i, pos[i], radius[i], boxes);
-}
-
-// incompressibility_0.comp + incompressibility_1.comp (sums over row i) +
-// incompressibility_2.comp, and the boundariness filter of
-// incompressibility_3.comp:41-46,60 for row i. Writes lambda_i, the particle's
-// own position shift (applied by k_solve_b) and boundariness_i.
-__global__ __launch_bounds__(BLOCK) void k_solve_a(const int4* __restrict__ pos,
-                                                   const float* __restrict__ inv_mass,
-                                                   const float* __restrict__ radius,
-                                                   const float* __restrict__ kw,
-                                                   const float2* __restrict__ gc,
-                                                   const uint32_t* __restrict__ counts,
-                                                   const uint32_t* __restrict__ row,
-                                                   const uint32_t* __restrict__ col, uint32_t pair_cap,
-                                                   SolverArgs a, float* __restrict__ boundariness,
-                                                   float* __restrict__ lambda, int4* __restrict__ shift) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        const int4 pi = pos[i];
-        const float im = inv_mass[i];
-        const float kwi = kw[i];
-        const float2 g2 = gc[i];
-        const GaussConst g{g2.x, g2.y};
-        float w0;
-        v3f dummy;
-        kernel_eval(a.hid, a.gid, mk(0, 0, 0), kwi, g, true, false, &w0, &dummy);
-        uint32_t density = to_u32(w0 / im * INCOMP_RES);
-        int32_t gs0 = 0, gs1 = 0, gs2 = 0;
-        uint32_t sq = 0;
-        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-        const uint32_t rb = row[i], re = min(row[i + 1], pair_cap);
-        for (uint32_t k = rb; k < re; ++k) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            const float imN = inv_mass[j];
-            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float w;
-            v3f grad;
-            kernel_eval(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
-            v3f wg = grad / imN;
-            density += to_u32(w / imN * INCOMP_RES);
-            gs0 = iadd(gs0, to_i32(wg.x * INCOMP_RES));
-            gs1 = iadd(gs1, to_i32(wg.y * INCOMP_RES));
-            gs2 = iadd(gs2, to_i32(wg.z * INCOMP_RES));
-            sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
-            if (a.method == 1) {
-                float sc = imN * kwi;
-                v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / sc;
-                c0 = iadd(c0, to_i32(wd.x));
-                c1 = iadd(c1, to_i32(wd.y));
-                c2 = iadd(c2, to_i32(wd.z));
-                c3 = iadd(c3, to_i32(INCOMP_RES / imN));
-            }
-        }
-        // incompressibility_2.comp:77-109
-        const float r = radius[i];
-        const float ird = pow_f(2.0f * r, DIM) * im;
-        const float dens = (float)density / INCOMP_RES;
-        const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
-        float sgs = (float)sq / INCOMP_RES;
-        const v3f selfGrad = (-wgs) * ird;
-        const float wgss = dot(wgs, wgs);
-        const float sgl = __builtin_sqrtf(wgss) * ird;
-        sgs += wgss * im;
-        const float under = 1.0f - dens * ird;
-        float b = boundariness[i];
-        if (a.update_b) {
-            float bb;
-            if (a.method == 1) {
-                float totalMass = (float)c3 / INCOMP_RES + 1.0f / im;
-                float dev = length(vec_i(c0, c1, c2)) / (POS_RES * totalMass);
-                bb = dev * a.b_sgl_factor;
-            } else {
-                float s1 = sgl * kwi * a.b_sgl_factor;
-                float up = fmax_g(0.0f, under) * a.b_under_factor;
-                bb = s1 + up;
-            }
-            bb = bb >= 1.0f ? 1.0f : 0.0f;
-            bb = move_towards_add(b, bb, a.b_speed);
-            b = fmin_g(1.0f, bb);
-        }
-        float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
-        lam /= a.lambda_scale_base / ird * im;
-        lambda[i] = lam;
-        int4 sh = make_int4(0, 0, 0, 0);
-        if (!(lam >= 0.0f)) {
-            float s = lam * im * POS_RES;
-            v3f d = selfGrad * s;
-            sh = make_int4(to_i32(d.x), to_i32(d.y), to_i32(d.z), 0);
-        }
-        shift[i] = sh;
-        // boundariness filter (incompressibility_3.comp:41-46), any neighbour of row i
-        if (a.method == 2 && __builtin_bit_cast(uint32_t, b) != 0u) {
-            v3f empty = -normalize(vec_i(gs0, gs1, gs2));
-            for (uint32_t k = rb; k < re; ++k) {
-                const uint32_t j = col[k];
-                const int4 pj = pos[j];
-                v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-                float w;
-                v3f grad;
-                kernel_eval(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
-                if (dot(empty, normalize(grad)) > 0.6f) {
-                    b = 0.0f;
-                    break;
-                }
-            }
-        }
-        boundariness[i] = b;
-    }
-}
-
-// incompressibility_3.comp:62-66 gathered per target particle j over the rows
-// that name j, plus j's own shift from incompressibility_2.comp:106-109. When
-// `boxes` has walls, box_collision of the next solver iteration follows.
-__global__ __launch_bounds__(BLOCK) void k_solve_b(const int4* __restrict__ pos,
-                                                   const float* __restrict__ kw,
-                                                   const float2* __restrict__ gc,
-                                                   const float* __restrict__ radius,
-                                                   const uint32_t* __restrict__ counts,
-                                                   const uint32_t* __restrict__ rrow,
-                                                   const uint32_t* __restrict__ rcol, int hid, int gid,
-                                                   const float* __restrict__ lambda,
-                                                   const int4* __restrict__ shift, BoxSet boxes,
-                                                   int4* __restrict__ pos_out) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(j, n) {
-        const int4 pj = pos[j];
-        const int4 sj = shift[j];
-        int32_t ax = iadd(pj.x, sj.x), ay = iadd(pj.y, sj.y), az = iadd(pj.z, sj.z);
-        const uint32_t rb = rrow[j], re = rrow[j + 1];
-        for (uint32_t k = rb; k < re; ++k) {
-            const uint32_t i = rcol[k];
-            const float lam = lambda[i];
-            if (!(lam < 0.0f)) continue;  // lambda >= 0: no shift; NaN shifts by ivec3(NaN) = 0
-            const int4 pi = pos[i];
-            const float2 g2 = gc[i];
-            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float w;
-            v3f grad;
-            kernel_eval(hid, gid, diff, kw[i], GaussConst{g2.x, g2.y}, false, true, &w, &grad);
-            v3f d = grad * (lam * POS_RES);
-            ax = iadd(ax, to_i32(d.x));
-            ay = iadd(ay, to_i32(d.y));
-            az = iadd(az, to_i32(d.z));
-        }
-        int4 out = make_int4(ax, ay, az, pj.w);
-        if (boxes.n) out = box_collide(j, out, radius[j], boxes);
-        pos_out[j] = out;
-    }
-}
-
-// Density error per particle: max(0, rho_i / rho0_i - 1) from the sums of
-// incompressibility_0/1, reduced to per-block (sum, max) partials.
-__global__ __launch_bounds__(BLOCK) void k_density_error(const int4* __restrict__ pos,
-                                                         const float* __restrict__ inv_mass,
-                                                         const float* __restrict__ radius,
-                                                         const float* __restrict__ kw,
-                                                         const float2* __restrict__ gc,
-                                                         const uint32_t* __restrict__ counts,
-                                                         const uint32_t* __restrict__ row,
-                                                         const uint32_t* __restrict__ col,
-                                                         uint32_t pair_cap, int hid, int gid,
-                                                         double* __restrict__ part_sum,
-                                                         float* __restrict__ part_max) {
-    __shared__ double ssum[BLOCK];
-    __shared__ float smax[BLOCK];
-    const uint32_t n = counts[C_N];
-    double acc = 0.0;
-    float mx = 0.0f;
-    GRID_STRIDE(i, n) {
-        const int4 pi = pos[i];
-        const float im = inv_mass[i];
-        const float kwi = kw[i];
-        const float2 g2 = gc[i];
-        const GaussConst g{g2.x, g2.y};
-        float w0;
-        v3f dummy;
-        kernel_eval(hid, gid, mk(0, 0, 0), kwi, g, true, false, &w0, &dummy);
-        uint32_t density = to_u32(w0 / im * INCOMP_RES);
-        const uint32_t rows = counts[C_ROWS];
-        const uint32_t rb = i < rows ? row[i] : 0u, re = i < rows ? min(row[i + 1], pair_cap) : 0u;
-        for (uint32_t k = rb; k < re; ++k) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float w;
-            kernel_eval(hid, gid, diff, kwi, g, true, false, &w, &dummy);
-            density += to_u32(w / inv_mass[j] * INCOMP_RES);
-        }
-        float ird = pow_f(2.0f * radius[i], DIM) * im;
-        float e = fmax_g(0.0f, (float)density / INCOMP_RES * ird - 1.0f);
-        acc += (double)e;
-        mx = fmax_g(mx, e);
-    }
-    ssum[threadIdx.x] = acc;
-    smax[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            ssum[threadIdx.x] += ssum[threadIdx.x + s];
-            smax[threadIdx.x] = fmax_g(smax[threadIdx.x], smax[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part_sum[blockIdx.x] = ssum[0];
-        part_max[blockIdx.x] = smax[0];
-    }
 }
 
 // Adds one substep's mean and max compression error to the running statistics
@@ -541,14 +315,6 @@ void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s) {
     hipLaunchKernelGGL(k_set_n_plus_1, dim3(1), dim3(64), 0, s, counts);
 }
-void launch_rev_count(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
-                      uint32_t pair_cap, uint32_t* rcnt, hipStream_t s) {
-    LAUNCH(k_rev_count, stride_grid(cap, 8192), counts, row, col, pair_cap, rcnt);
-}
-void launch_rev_fill(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
-                     uint32_t pair_cap, uint32_t* rcur, uint32_t* rcol, hipStream_t s) {
-    LAUNCH(k_rev_fill, stride_grid(cap, 8192), counts, row, col, pair_cap, rcur, rcol);
-}
 void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
                           hipStream_t s) {
     LAUNCH(k_kernel_consts, stride_grid(cap), kw, counts, gc);
@@ -557,21 +323,6 @@ void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t
                 const BoxSet& boxes, hipStream_t s) {
     LAUNCH(k_box, stride_grid(cap), pos, radius, counts, boxes);
 }
-void launch_solve_a(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                    const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                    const uint32_t* col, uint32_t pair_cap, const SolverArgs& a, float* boundariness,
-                    float* lambda, int4* shift, hipStream_t s) {
-    LAUNCH(k_solve_a, stride_grid(cap, 8192), pos, inv_mass, radius, kw, gc, counts, row, col, pair_cap,
-           a, boundariness, lambda, shift);
-}
-void launch_solve_b(const int4* pos, const float* kw, const float2* gc, const float* radius,
-                    const uint32_t* counts, uint32_t cap, const uint32_t* rrow, const uint32_t* rcol,
-                    int hid, int gid, const float* lambda, const int4* shift, const BoxSet& boxes,
-                    int4* pos_out, hipStream_t s) {
-    LAUNCH(k_solve_b, stride_grid(cap, 8192), pos, kw, gc, radius, counts, rrow, rcol, hid, gid, lambda,
-           shift, boxes, pos_out);
-}
-uint32_t density_error_blocks(uint32_t cap) { return stride_grid(cap, 1024); }
 void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s) {
     hipLaunchKernelGGL(k_count_particle_substeps, dim3(1), dim3(64), 0, s, counts, acc);
 }
@@ -579,13 +330,6 @@ void launch_density_error_finish(const double* part_sum, const float* part_max, 
                                  uint32_t* counts, double* stats, hipStream_t s) {
     hipLaunchKernelGGL(k_density_error_finish, dim3(1), dim3(BLOCK), 0, s, part_sum, part_max, blocks, counts,
                        stats);
-}
-void launch_density_error(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                          const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                          const uint32_t* col, uint32_t pair_cap, int hid, int gid, double* part_sum,
-                          float* part_max, hipStream_t s) {
-    LAUNCH(k_density_error, density_error_blocks(cap), pos, inv_mass, radius, kw, gc, counts, row, col,
-           pair_cap, hid, gid, part_sum, part_max);
 }
 
 }  // namespace apbf

@@ -1,0 +1,446 @@
+// Per-particle kernels over CSR neighbour rows, with a group of L lanes of a
+// 64-lane wavefront cooperating on one row (L = 16 by default): the lanes read
+// consecutive entries of the row (coalesced) and gather the neighbours'
+// positions, and the row totals come from xor-shuffle reductions inside the
+// group. Every per-pair quantity the reference accumulates is a truncated
+// integer (incompressibility_1.comp:55-59, incompressibility_3.comp:66,
+// find_split_and_merge_1.comp:30-31), so the reduction order does not change
+// any bit of the result.
+//
+// The incompressibility iteration (incompressibility.cpp:12-45) is one kernel
+// per iteration plus a light apply pass:
+//   k_solve_rows: incompressibility_0/1 sums over row i, incompressibility_2
+//     (lambda, boundariness), then the position shifts of
+//     incompressibility_2.comp:106-109 (particle i) and _3.comp:62-66 (every
+//     neighbour j of row i, gradient recomputed from the same inputs) added
+//     with int32 atomics into a per-particle accumulator, and the boundariness
+//     filter of _3.comp:41-46;
+//   k_solve_apply: position += accumulator (and the box collision that opens
+//     the next solver iteration).
+// Positions are read-only during k_solve_rows, as in the reference, where
+// _1 and _3 use gradients of the positions _1 saw.
+#include "scene.h"
+
+namespace apbf {
+
+constexpr int ROW_LANES = 16;
+
+template <int L>
+__device__ __forceinline__ uint32_t gsum(uint32_t v) {
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+template <int L>
+__device__ __forceinline__ uint32_t gmin(uint32_t v) {
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) {
+        uint32_t o = __shfl_xor(v, m, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+template <int L>
+__device__ __forceinline__ float gmaxf(float v) {
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) v = fmax_g(v, __shfl_xor(v, m, 64));
+    return v;
+}
+// Mask of this lane's group within the wave's 64-bit ballot.
+template <int L>
+__device__ __forceinline__ uint64_t group_mask() {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = (L == 64) ? ~0ull : ((1ull << L) - 1ull);
+    return m << (lane & ~(L - 1));
+}
+
+#define ROW_LOOP_BEGIN(L)                                                                          \
+    const uint32_t n = counts[C_N];                                                                \
+    const int gl = threadIdx.x & (L - 1);                                                          \
+    const uint32_t groups = BLOCK / L;                                                             \
+    for (uint32_t base = blockIdx.x * groups; base < n; base += gridDim.x * groups) {              \
+        const uint32_t i = base + threadIdx.x / L;                                                 \
+        const bool active = i < n;
+
+#define ROW_LOOP_END }
+
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ pos,
+                                                      const float* __restrict__ inv_mass,
+                                                      const float* __restrict__ radius,
+                                                      const float* __restrict__ kw,
+                                                      const float2* __restrict__ gc,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ row,
+                                                      const uint32_t* __restrict__ col, uint32_t pcap,
+                                                      SolverArgs a, float* __restrict__ boundariness,
+                                                      float* __restrict__ lambda, int* __restrict__ acc) {
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    float im = 1.0f, kwi = 1.0f;
+    GaussConst g{1.0f, 1.0f};
+    uint32_t rb = 0, re = 0;
+    if (active) {
+        pi = pos[i];
+        im = inv_mass[i];
+        kwi = kw[i];
+        const float2 g2 = gc[i];
+        g = GaussConst{g2.x, g2.y};
+        rb = row[i];
+        re = min(row[i + 1], pcap);
+    }
+    // incompressibility_1.comp:43-68 over this lane's share of row i
+    uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    for (uint32_t k = rb + gl; k < re; k += L) {
+        const uint32_t j = col[k];
+        const int4 pj = pos[j];
+        const float imN = inv_mass[j];
+        const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+        float w;
+        v3f grad;
+        kernel_eval(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+        const v3f wg = grad / imN;
+        dens += to_u32(w / imN * INCOMP_RES);
+        s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
+        s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
+        s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
+        sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
+        if (a.method == 1) {
+            const v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / (imN * kwi);
+            m0 += (uint32_t)to_i32(wd.x);
+            m1 += (uint32_t)to_i32(wd.y);
+            m2 += (uint32_t)to_i32(wd.z);
+            m3 += (uint32_t)to_i32(INCOMP_RES / imN);
+        }
+    }
+    dens = gsum<L>(dens);
+    sq = gsum<L>(sq);
+    s0 = gsum<L>(s0);
+    s1 = gsum<L>(s1);
+    s2 = gsum<L>(s2);
+    if (a.method == 1) {
+        m0 = gsum<L>(m0);
+        m1 = gsum<L>(m1);
+        m2 = gsum<L>(m2);
+        m3 = gsum<L>(m3);
+    }
+    // incompressibility_0.comp:43: the particle's own contribution
+    float w0;
+    v3f unused;
+    kernel_eval(a.hid, a.gid, mk(0, 0, 0), kwi, g, true, false, &w0, &unused);
+    dens += to_u32(w0 / im * INCOMP_RES);
+    // incompressibility_2.comp:77-109 (every lane of the group computes the same values)
+    const int32_t gs0 = (int32_t)s0, gs1 = (int32_t)s1, gs2 = (int32_t)s2;
+    const float r = active ? radius[i] : 1.0f;
+    const float ird = pow_f(2.0f * r, DIM) * im;
+    const float densf = (float)dens / INCOMP_RES;
+    const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
+    float sgs = (float)sq / INCOMP_RES;
+    const v3f selfGrad = (-wgs) * ird;
+    const float wgss = dot(wgs, wgs);
+    const float sgl = __builtin_sqrtf(wgss) * ird;
+    sgs += wgss * im;
+    const float under = 1.0f - densf * ird;
+    float b = active ? boundariness[i] : 0.0f;
+    if (a.update_b) {
+        float bb;
+        if (a.method == 1) {
+            const float totalMass = (float)(int32_t)m3 / INCOMP_RES + 1.0f / im;
+            const float dev = length(vec_i((int32_t)m0, (int32_t)m1, (int32_t)m2)) / (POS_RES * totalMass);
+            bb = dev * a.b_sgl_factor;
+        } else {
+            bb = sgl * kwi * a.b_sgl_factor + fmax_g(0.0f, under) * a.b_under_factor;
+        }
+        bb = bb >= 1.0f ? 1.0f : 0.0f;
+        b = fmin_g(1.0f, move_towards_add(b, bb, a.b_speed));
+    }
+    float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
+    lam /= a.lambda_scale_base / ird * im;
+    if (active && gl == 0) {
+        lambda[i] = lam;
+        if (!(lam >= 0.0f)) {  // own shift; a NaN lambda shifts by ivec3(NaN) = 0
+            const v3f d = selfGrad * (lam * im * POS_RES);
+            const int32_t dx = to_i32(d.x), dy = to_i32(d.y), dz = to_i32(d.z);
+            if (dx) atomicAdd(&acc[4 * i + 0], dx);
+            if (dy) atomicAdd(&acc[4 * i + 1], dy);
+            if (dz) atomicAdd(&acc[4 * i + 2], dz);
+        }
+    }
+    // neighbour shifts (incompressibility_3.comp:62-66) and the boundariness
+    // filter (:41-46), over this lane's share of row i
+    const bool do_shift = active && lam < 0.0f;
+    const bool do_filter = active && a.method == 2 && __builtin_bit_cast(uint32_t, b) != 0u;
+    bool hit = false;
+    if (do_shift || do_filter) {
+        const float sc = lam * POS_RES;
+        const v3f empty = -normalize(vec_i(gs0, gs1, gs2));
+        for (uint32_t k = rb + gl; k < re; k += L) {
+            const uint32_t j = col[k];
+            const int4 pj = pos[j];
+            const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            float w;
+            v3f grad;
+            kernel_eval(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+            if (do_filter && !hit && dot(empty, normalize(grad)) > 0.6f) hit = true;
+            if (do_shift) {
+                const v3f d = grad * sc;
+                const int32_t dx = to_i32(d.x), dy = to_i32(d.y), dz = to_i32(d.z);
+                if (dx) atomicAdd(&acc[4 * j + 0], dx);
+                if (dy) atomicAdd(&acc[4 * j + 1], dy);
+                if (dz) atomicAdd(&acc[4 * j + 2], dz);
+            }
+        }
+    }
+    const uint64_t ballot = __ballot(hit);
+    if (ballot & group_mask<L>()) b = 0.0f;
+    if (active && gl == 0) boundariness[i] = b;
+    ROW_LOOP_END
+}
+
+// Applies the accumulated shifts, clears the accumulator, and (when `boxes`
+// holds walls) runs the box collision of the next solver iteration.
+__global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, int4* __restrict__ acc,
+                                                       const float* __restrict__ radius,
+                                                       const uint32_t* __restrict__ counts, BoxSet boxes) {
+    const uint32_t n = counts[C_N];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        int4 p = pos[j];
+        const int4 d = acc[j];
+        p.x = iadd(p.x, d.x);
+        p.y = iadd(p.y, d.y);
+        p.z = iadd(p.z, d.z);
+        acc[j] = make_int4(0, 0, 0, 0);
+        if (boxes.n) p = box_collide(j, p, radius[j], boxes);
+        pos[j] = p;
+    }
+}
+
+// Density error per particle (the sums of incompressibility_0/1), reduced to
+// per-block (sum, max) partials. Rows exist for particles below counts[C_ROWS].
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__ pos,
+                                                        const float* __restrict__ inv_mass,
+                                                        const float* __restrict__ radius,
+                                                        const float* __restrict__ kw,
+                                                        const float2* __restrict__ gc,
+                                                        const uint32_t* __restrict__ counts,
+                                                        const uint32_t* __restrict__ row,
+                                                        const uint32_t* __restrict__ col, uint32_t pcap,
+                                                        int hid, int gid, double* __restrict__ part_sum,
+                                                        float* __restrict__ part_max) {
+    __shared__ double ssum[BLOCK];
+    __shared__ float smax[BLOCK];
+    double acc_sum = 0.0;
+    float acc_max = 0.0f;
+    const uint32_t rows = counts[C_ROWS];
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    float im = 1.0f, kwi = 1.0f;
+    GaussConst g{1.0f, 1.0f};
+    uint32_t rb = 0, re = 0;
+    if (active) {
+        pi = pos[i];
+        im = inv_mass[i];
+        kwi = kw[i];
+        const float2 g2 = gc[i];
+        g = GaussConst{g2.x, g2.y};
+        if (i < rows) {
+            rb = row[i];
+            re = min(row[i + 1], pcap);
+        }
+    }
+    uint32_t dens = 0;
+    v3f unused;
+    for (uint32_t k = rb + gl; k < re; k += L) {
+        const uint32_t j = col[k];
+        const int4 pj = pos[j];
+        const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+        float w;
+        kernel_eval(hid, gid, diff, kwi, g, true, false, &w, &unused);
+        dens += to_u32(w / inv_mass[j] * INCOMP_RES);
+    }
+    dens = gsum<L>(dens);
+    if (active && gl == 0) {
+        float w0;
+        kernel_eval(hid, gid, mk(0, 0, 0), kwi, g, true, false, &w0, &unused);
+        dens += to_u32(w0 / im * INCOMP_RES);
+        const float ird = pow_f(2.0f * radius[i], DIM) * im;
+        const float e = fmax_g(0.0f, (float)dens / INCOMP_RES * ird - 1.0f);
+        acc_sum += (double)e;
+        acc_max = fmax_g(acc_max, e);
+    }
+    ROW_LOOP_END
+    ssum[threadIdx.x] = acc_sum;
+    smax[threadIdx.x] = acc_max;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            smax[threadIdx.x] = fmax_g(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_sum[blockIdx.x] = ssum[0];
+        part_max[blockIdx.x] = smax[0];
+    }
+}
+
+// find_split_and_merge_1.comp:20-31 and _2.comp:20-32 over row i: flood-fill
+// boundary distance, shortest neighbour distance, and the last neighbour in
+// row order at that distance (the last writer of the sequential schedule). A
+// particle without neighbours keeps UINT_MAX and names itself.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_fsm12_rows(const int4* __restrict__ pos,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ row,
+                                                      const uint32_t* __restrict__ col, uint32_t pcap,
+                                                      const uint32_t* __restrict__ old_bd,
+                                                      uint32_t* __restrict__ bd, uint32_t* __restrict__ nearest) {
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    uint32_t rb = 0, re = 0;
+    if (active) {
+        pi = pos[i];
+        rb = row[i];
+        re = min(row[i + 1], pcap);
+    }
+    uint32_t bmin = 0xFFFFFFFFu, best = 0xFFFFFFFFu, best_k = 0u;
+    bool found = false;
+    for (uint32_t k = rb + gl; k < re; k += L) {
+        const uint32_t j = col[k];
+        const int4 pj = pos[j];
+        const uint32_t dist = to_u32(length(vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z))));
+        const uint32_t cand = old_bd[j] + dist;
+        bmin = cand < bmin ? cand : bmin;
+        if (!found || dist <= best) {  // later entries of the lane win ties
+            best = dist;
+            best_k = k;
+            found = true;
+        }
+    }
+    bmin = gmin<L>(bmin);
+    // group arg-min of (distance, -position): smallest distance, then the last position
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) {
+        const uint32_t ob = __shfl_xor(best, m, 64);
+        const uint32_t ok = __shfl_xor(best_k, m, 64);
+        const bool of = __shfl_xor((int)found, m, 64) != 0;
+        if (of && (!found || ob < best || (ob == best && ok > best_k))) {
+            best = ob;
+            best_k = ok;
+            found = true;
+        }
+    }
+    if (active && gl == 0) {
+        bd[i] = bmin;
+        nearest[i] = found ? col[best_k] : i;
+    }
+    ROW_LOOP_END
+}
+
+// kernel_width.comp:22-56 over row i. Pass FILL = false: spread the original
+// kernel width to every neighbour (atomicMax, order-free) and count the pairs
+// within the cutoff; FILL = true: write those pairs from row2[i] in row order.
+template <int L, bool FILL>
+__global__ __launch_bounds__(BLOCK) void k_spread_rows(const int4* __restrict__ pos,
+                                                       const float* __restrict__ radius,
+                                                       const float* __restrict__ trad,
+                                                       const float* __restrict__ kw_old,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ row,
+                                                       const uint32_t* __restrict__ col, uint32_t pcap,
+                                                       int on_target, uint32_t* __restrict__ ukw,
+                                                       uint32_t* __restrict__ cnt2,
+                                                       const uint32_t* __restrict__ row2,
+                                                       uint32_t* __restrict__ col2) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    float okw = 0.0f, cutoff = 0.0f;
+    uint32_t rb = 0, re = 0, out = 0;
+    if (active) {
+        pi = pos[i];
+        const float tr = on_target ? trad[i] : 0.0f;
+        okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
+        cutoff = fmax_g(okw, kw_old[i]);
+        rb = row[i];
+        re = min(row[i + 1], pcap);
+        if (FILL) out = row2[i];
+    }
+    uint32_t c = 0;
+    // the group walks the row in chunks of L entries so kept pairs keep row order
+    const uint32_t len = re - rb;
+    const uint32_t maxlen = gmin<L>(0xFFFFFFFFu - len);  // max over the group, as a min of complements
+    const uint32_t chunks = ((0xFFFFFFFFu - maxlen) + L - 1) / L;
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+        const uint32_t k = rb + ch * L + gl;
+        bool keep = false;
+        uint32_t j = 0;
+        if (k < re) {
+            j = col[k];
+            const int4 pj = pos[j];
+            const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            const float dist = length(diff);
+            if (!FILL) {
+                const float dfk = dist - okw;
+                const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
+                atomicMax(&ukw[j], to_u32(okw * infl * KW_RES));
+            }
+            keep = dist <= cutoff;
+        }
+        const uint64_t bal = __ballot(keep) & group_mask<L>();
+        if (FILL && keep) col2[out + c + (uint32_t)__popcll(bal & below)] = j;
+        c += (uint32_t)__popcll(bal);
+    }
+    if (!FILL && active && gl == 0) cnt2[i] = c;
+    ROW_LOOP_END
+    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cnt2[counts[C_N]] = 0;
+}
+
+// ------------------------------------------------------------ launch wrappers
+static uint32_t row_grid(uint32_t cap) {
+    const uint32_t groups = BLOCK / ROW_LANES;
+    uint32_t g = div_up(cap ? cap : 1, groups);
+    return g < 8192u ? g : 8192u;
+}
+
+void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                       const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
+                       float* lambda, int4* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_rows<ROW_LANES>, dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, inv_mass, radius, kw,
+                       gc, counts, row, col, pcap, a, boundariness, lambda, reinterpret_cast<int*>(acc));
+}
+void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
+                        const BoxSet& boxes, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_apply, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, pos, acc, radius, counts, boxes);
+}
+uint32_t density_rows_blocks(uint32_t cap) { return row_grid(cap) < 2048u ? row_grid(cap) : 2048u; }
+void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
+                         const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                         const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
+                         float* part_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_density_rows<ROW_LANES>, dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos, inv_mass,
+                       radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+}
+void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_fsm12_rows<ROW_LANES>, dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, counts, row, col, pcap,
+                       old_bd, bd, nearest);
+}
+void launch_spread_rows(bool fill, const int4* pos, const float* radius, const float* trad, const float* kw_old,
+                        const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col, uint32_t pcap,
+                        int on_target, uint32_t* ukw, uint32_t* cnt2, const uint32_t* row2, uint32_t* col2,
+                        hipStream_t s) {
+    if (fill)
+        hipLaunchKernelGGL((k_spread_rows<ROW_LANES, true>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, radius, trad,
+                           kw_old, counts, row, col, pcap, on_target, ukw, cnt2, row2, col2);
+    else
+        hipLaunchKernelGGL((k_spread_rows<ROW_LANES, false>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, radius,
+                           trad, kw_old, counts, row, col, pcap, on_target, ukw, cnt2, row2, col2);
+}
+
+}  // namespace apbf

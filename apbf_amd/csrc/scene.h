@@ -8,12 +8,12 @@ namespace apbf {
 
 // Kernels that can be timed with HIP events (apbf_scene_time_kernel).
 enum KernelId : int {
-    K_SOLVE_A = 0,      // k_solve_a: density, lambda, boundariness, own shift (row gather)
-    K_SOLVE_B = 1,      // k_solve_b: neighbour shifts (reverse-row gather) + next box collision
+    K_SOLVE_A = 0,      // k_solve_rows: density, lambda, boundariness, shifts (one iteration)
+    K_SOLVE_B = 1,      // k_solve_apply: apply the shifts + next box collision
     K_GREEN_COUNT = 2,  // k_green<count>: neighbour counting walk
     K_GREEN_FILL = 3,   // k_green<fill>: neighbour writing walk
-    K_SPREAD = 4,       // k_spread<count>: kernel-width spreading
-    K_FSM12 = 5,        // k_fsm12: boundary-distance flood fill + nearest neighbour
+    K_SPREAD = 4,       // k_spread_rows<count>: kernel-width spreading (counting pass)
+    K_FSM12 = 5,        // k_fsm12_rows: boundary-distance flood fill + nearest neighbour
     K_PERMUTE = 6,      // k_permute: particle SoA gather by the sort permutation
     K_COUNT_IDS = 7
 };
@@ -28,7 +28,6 @@ struct Scene {
     float last_dt = 1.0f;  // velocity_handling.h:17
     int cur = 0;           // current particle buffer set
     int tcur = 0;          // current transfer buffer set
-    bool reverse_valid = false;
     bool pairs_valid = false;    // CSR rows refer to the current particle slots
     bool measure_error = false;  // accumulate the density error after every solver loop
     double* err_stats = nullptr; // [sum of per-substep means, max, last mean, last max]
@@ -68,11 +67,11 @@ struct Scene {
     uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *perm = nullptr;
     uint32_t *cell_start = nullptr, *cell_end = nullptr;
     uint32_t *cnt = nullptr, *row = nullptr, *col = nullptr;
-    uint32_t *rcnt = nullptr, *rrow = nullptr, *rcol = nullptr, *col_tmp = nullptr;
+    uint32_t* col_tmp = nullptr;
     // solver
     float2* gc = nullptr;
     float* lambda = nullptr;
-    int4* shift = nullptr;
+    int4* shift = nullptr;  // per-particle shift accumulator of the solver (x, y, z, unused)
     // adaptive scratch
     uint32_t* ukw = nullptr;
     uint32_t *u0 = nullptr, *u1 = nullptr, *u2 = nullptr, *u3 = nullptr, *u4 = nullptr;
@@ -119,7 +118,6 @@ struct Scene {
 
     // helpers
     void build_pairs(float range_scale);
-    void build_reverse();
     void prepare_solver();
     SolverArgs solver_args() const;
     void incompressibility_iteration(const BoxSet& next_boxes);

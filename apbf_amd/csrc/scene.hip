@@ -103,9 +103,6 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(cnt, cap + 1);
     A(row, cap + 1);
     A(col, pcap);
-    A(rcnt, cap + 1);
-    A(rrow, cap + 1);
-    A(rcol, pcap);
     A(col_tmp, pcap);
     A(gc, cap);
     A(lambda, cap);
@@ -128,7 +125,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(a_t0, cap + 1);
     A(t_flag, tcap + 1);
     A(t_scan, tcap + 1);
-    err_blocks = density_error_blocks(cap);
+    err_blocks = density_rows_blocks(cap);
     A(part_sum, err_blocks);
     A(part_max, err_blocks);
     A(err_stats, 4);
@@ -153,8 +150,7 @@ Scene::~Scene() {
         hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]);
         hipFree(t_source[s]); hipFree(t_target[s]); hipFree(t_time_left[s]);
     }
-    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, rcnt, rrow,
-                  rcol, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
+    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_flag, a_scan,
                   a_list, a_t0, t_flag, t_scan, err_stats, particle_substeps};
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -207,29 +203,17 @@ void Scene::build_pairs(float range_scale) {
     launch_green_fill(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
                       row, col, pcap, s);
     tend(tk);
-    reverse_valid = false;
     pairs_valid = true;
-}
-
-void Scene::build_reverse() {
-    hipStream_t s = stream;
-    fill_u32_dev(rcnt, 0u, counts + C_N_PLUS_1, cap + 1, s);
-    launch_rev_count(counts, cap, row, col, pcap, rcnt, s);
-    scan_u32(rcnt, rrow, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    hipMemcpyAsync(u0, rrow, (size_t)(cap + 1) * 4, hipMemcpyDeviceToDevice, s);
-    launch_rev_fill(counts, cap, row, col, pcap, u0, rcol, s);
-    reverse_valid = true;
 }
 
 // --------------------------------------------------------- box_collision
 void Scene::box_collision() { launch_box(pos[cur], radius[cur], counts, cap, boxes, stream); }
 
 // ----------------------------------------------------- incompressibility
-// incompressibility.cpp:12-45 as two gather kernels: k_solve_a (per particle
-// i over its row: _0, _1, _2 and the boundariness filter of _3) and k_solve_b
-// (per particle j over the rows naming j: the neighbour shifts of _3 plus j's
-// own shift). `next_boxes` fuses the box collision that opens the next solver
-// iteration into k_solve_b.
+// incompressibility.cpp:12-45: one k_solve_rows pass (sums, lambda,
+// boundariness, shifts into the accumulator) and one k_solve_apply pass per
+// iteration. `next_boxes` fuses the box collision that opens the next solver
+// iteration into the apply pass.
 SolverArgs Scene::solver_args() const {
     SolverArgs a;
     a.hid = P.s.mHeightKernelId;
@@ -244,24 +228,20 @@ SolverArgs Scene::solver_args() const {
 }
 
 void Scene::prepare_solver() {
-    if (!reverse_valid) build_reverse();
     launch_kernel_consts(kernel_width[cur], counts, cap, gc, stream);
+    fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
 }
 
 void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
     hipStream_t s = stream;
     SolverArgs a = solver_args();
     bool tk = tbegin(K_SOLVE_A);
-    launch_solve_a(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col,
-                   pcap, a, boundariness[cur], lambda, shift, s);
+    launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
+                      boundariness[cur], lambda, shift, s);
     tend(tk);
-    // The other buffer set's position array is free between permutations.
-    int4* out = pos[1 - cur];
     tk = tbegin(K_SOLVE_B);
-    launch_solve_b(pos[cur], kernel_width[cur], gc, radius[cur], counts, cap, rrow, rcol, a.hid, a.gid,
-                   lambda, shift, next_boxes, out, s);
+    launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s);
     tend(tk);
-    std::swap(pos[0], pos[1]);
 }
 
 void Scene::incompressibility() {
@@ -296,8 +276,8 @@ void Scene::update(float dt) {
 void Scene::accumulate_density_error() {
     hipStream_t s = stream;
     launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
-    launch_density_error(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col,
-                         pcap, P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+    launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
+                        P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
     launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
 }
 
@@ -323,8 +303,8 @@ apbf_status Scene::density_error(double* mean_err, double* max_err) {
     if (!pairs_valid) return APBF_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream;
     launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
-    launch_density_error(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row,
-                         col, pcap, P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+    launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
+                        P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
     std::vector<double> hs(err_blocks);
     std::vector<float> hm(err_blocks);
     uint32_t n = 0;
@@ -525,6 +505,17 @@ apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* laun
     *total_ms = t;
     *launches = (uint32_t)(s->events_used / 2);
     s->events_used = 0;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_diagnostics(void* scene, uint32_t* host_counts, uint32_t n_words) {
+    if (!scene || !host_counts) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    uint32_t c[apbf::C_COUNT];
+    if (hipMemcpyAsync(c, s->counts, sizeof(c), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    for (uint32_t k = 0; k < n_words && k < apbf::C_COUNT; ++k) host_counts[k] = c[k];
     return APBF_OK;
 }
 

@@ -24,7 +24,9 @@ enum CountSlot : uint32_t {
     C_TMP1 = 13,
     C_ROWS = 14,       // particles covered by the CSR rows (n when the list was built)
     C_ERR_SUBSTEPS = 15,  // substeps accumulated in the density-error statistics
-    C_COUNT = 16
+    C_N4 = 16,         // 4 n (words of the solver's shift accumulator)
+    C_PAIRS_PEAK = 17, // largest neighbour-walk pair count (before the clamp) since the scene was created
+    C_COUNT = 32
 };
 
 struct GridDesc {

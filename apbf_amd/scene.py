@@ -152,6 +152,13 @@ class Scene:
         check(self._L.apbf_scene_kernel_time(self._h, C.byref(t), C.byref(n)), "kernel_time")
         return t.value, n.value
 
+    def diagnostics(self) -> dict:
+        c = np.zeros(32, np.uint32)
+        check(self._L.apbf_scene_diagnostics(self._h, _ptr(c), 32), "diagnostics")
+        return {"particles": int(c[0]), "pairs": int(c[2]), "transfers": int(c[3]), "splits": int(c[4]),
+                "overflow_flags": int(c[6]), "pairs_before_clamp": int(c[7]), "rows": int(c[14]),
+                "peak_pairs_before_clamp": int(c[17])}
+
     def overflow_flags(self) -> int:
         f = C.c_uint32()
         check(self._L.apbf_scene_overflow_flags(self._h, C.byref(f)), "overflow_flags")

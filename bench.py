@@ -130,6 +130,10 @@ def main():
     scene.time_kernel(None)
     mean_err, max_err, last_err, err_steps = scene.error_stats()
     n_now, nt_now, np_now = scene.counts()
+    diag = scene.diagnostics()
+    if diag["overflow_flags"]:
+        log(f"WARNING: capacity clamp hit (flags {diag['overflow_flags']}, peak pairs "
+            f"{diag['peak_pairs_before_clamp']} of {params['cap_pairs']})")
     value = ps / elapsed
 
     # roofline of the timed kernel: algorithmic bytes per launch / mean launch time
@@ -171,6 +175,9 @@ def main():
             "dt": dt,
             "hash_res_log2": params["hash_res_log2"],
             "solver": "plain (cold-start lambda)",
+            "peak_walk_pairs": diag["peak_pairs_before_clamp"],
+            "pair_capacity": params["cap_pairs"],
+            "capacity_clamped": bool(diag["overflow_flags"]),
         },
         "density_error": {"mean_of_substep_means": mean_err, "max": max_err, "substeps": err_steps},
         "roofline": {

@@ -164,6 +164,11 @@ apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t*
  * the recorded launches and restarts the recording. Both synchronise. */
 apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches);
 apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches);
+/* Copies the first n_words words of the scene's device-resident count block:
+ * 0 particles, 2 neighbour pairs, 3 transfers, 4 splits, 6 overflow flags,
+ * 7 neighbour-walk pairs before the capacity clamp, 14 particles covered by
+ * the neighbour rows, 17 largest pre-clamp pair count seen. Synchronises. */
+apbf_status apbf_scene_diagnostics(void* scene, uint32_t* host_counts, uint32_t n_words);
 /* Overflow flags raised by capacity clamps since the last call (bit 0 pairs,
  * bit 1 transfers, bit 2 particles); cleared by the call. Synchronises. */
 apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags);
