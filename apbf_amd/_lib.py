@@ -111,6 +111,7 @@ PROTOTYPES = {
     "apbf_scene_kernel_time": (_i32, [_vp, _pd, _pu]),
     "apbf_prefix_sum_workspace_size": (_sz, [_u32]),
     "apbf_radix_sort_workspace_size": (_sz, [_u32]),
+    "apbf_neighborhood_green_workspace_size": (_sz, [_u32]),
     "apbf_prefix_sum": (_i32, [_vp, _vp, _u32, _vp, _sz, _vp]),
     "apbf_radix_sort": (_i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
     "apbf_calculate_position_hash": (_i32, [_vp, _vp, _u32, _pf, _pf, _u32, _vp, _vp]),

@@ -559,6 +559,20 @@ void Scene::spread_kernel_width() {
     LAUNCH(k_kw_gradual, stride_grid(cap), ukw, kernel_width[cur], counts, P.s.mKernelWidthAdaptionSpeed);
 }
 
+// The spreading fused into the pair build: kernel_width_init.comp:17-31 before
+// the walk, uint_to_float_but_gradual.comp:27-36 after it.
+void Scene::spread_begin() {
+    hipStream_t s = stream;
+    LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, P.s.mBaseKernelWidthOnTargetRadius,
+           ukw);
+}
+void Scene::spread_end() {
+    hipStream_t s = stream;
+    launch_fix_clamped_rows(counts, cap, pcap, row, s);
+    LAUNCH1(k_set_pairs_from_row, counts, row);
+    LAUNCH(k_kw_gradual, stride_grid(cap), ukw, kernel_width[cur], counts, P.s.mKernelWidthAdaptionSpeed);
+}
+
 // update_transfers.cpp:14-71
 void Scene::update_transfers() {
     hipStream_t s = stream;

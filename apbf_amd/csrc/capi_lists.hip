@@ -144,6 +144,10 @@ apbf_status apbf_find_value_ranges(const uint32_t* sorted_keys, const uint32_t* 
     return last_status();
 }
 
+size_t apbf_neighborhood_green_workspace_size(uint32_t capacity) {
+    return (size_t)C_COUNT * 4 + (size_t)(capacity + 1) * 4 + scan_workspace_bytes(capacity + 1);
+}
+
 apbf_status apbf_neighborhood_green(const int32_t* position, const float* range, const uint32_t* d_len,
                                     uint32_t capacity, float range_scale, const float* mn, const float* mx,
                                     uint32_t res_log2, const uint32_t* cell_start, const uint32_t* cell_end,
@@ -152,12 +156,11 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
     if (!position || !range || !d_len || !mn || !mx || !cell_start || !cell_end || !row_offset ||
         !neighbors || !d_n_pairs || !ws || res_log2 > 10)
         return APBF_ERR_INVALID_ARGUMENT;
-    // workspace: counts block (16 words) + counts per particle (capacity + 1) + scan workspace
-    size_t need = 64 + (size_t)(capacity + 1) * 4 + scan_workspace_bytes(capacity + 1);
-    if (ws_bytes < need) return APBF_ERR_WORKSPACE;
+    // workspace: count block + counts per particle (capacity + 1) + scan workspace
+    if (ws_bytes < apbf_neighborhood_green_workspace_size(capacity)) return APBF_ERR_WORKSPACE;
     hipStream_t s = S(stream);
     uint32_t* counts = static_cast<uint32_t*>(ws);
-    uint32_t* cnt = counts + 16;
+    uint32_t* cnt = counts + C_COUNT;
     void* scan_ws = cnt + capacity + 1;
     GridDesc g;
     for (int d = 0; d < 3; ++d) {
@@ -165,15 +168,16 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
         g.mx[d] = mx[d];
     }
     g.res = res_log2;
-    hipMemsetAsync(counts, 0, 64, s);
+    fill_u32(counts, 0u, C_COUNT, s);
     hipMemcpyAsync(counts + C_N, d_len, 4, hipMemcpyDeviceToDevice, s);
     launch_set_n_plus_1(counts, s);
     const int4* p4 = reinterpret_cast<const int4*>(position);
-    launch_green_count(p4, range, counts, capacity, range_scale, g, cell_start, cell_end, cnt, s);
+    launch_walk(false, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
+                pair_capacity, cnt, nullptr, nullptr, nullptr, nullptr, nullptr, s);
     scan_u32(cnt, row_offset, counts + C_N_PLUS_1, capacity + 1, false, nullptr, scan_ws, s);
     launch_pairs_total(counts, row_offset, pair_capacity, s);
-    launch_green_fill(p4, range, counts, capacity, range_scale, g, cell_start, cell_end, row_offset,
-                      neighbors, pair_capacity, s);
+    launch_walk(true, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
+                pair_capacity, nullptr, nullptr, row_offset, nullptr, neighbors, nullptr, s);
     hipMemcpyAsync(d_n_pairs, counts + C_PAIRS, 4, hipMemcpyDeviceToDevice, s);
     return last_status();
 }

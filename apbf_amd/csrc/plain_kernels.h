@@ -31,15 +31,17 @@ void launch_permute(const uint32_t* perm, const uint32_t* counts, uint32_t cap,
                     const ParticleArrays& src, const ParticleArrays& dst, hipStream_t s);
 void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, uint32_t* start,
                    uint32_t* end, hipStream_t s);
-void launch_green_count(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
-                        float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
-                        uint32_t* cnt, hipStream_t s);
-void launch_green_fill(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
-                       float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
-                       const uint32_t* row, uint32_t* col, uint32_t pair_cap, hipStream_t s);
+// Row-group neighbour walk (COUNT / FILL passes), optionally fused with the
+// kernel-width spreading and its short-list filter.
+void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
+                 int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
+                 float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
+                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s);
+void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s);
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
-void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
+void launch_kernel_consts(const float* kw, const float* radius, const float* inv_mass, const uint32_t* counts,
+                          uint32_t cap, int hid, int gid, float4* gc,
                           hipStream_t s);
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s);
@@ -49,14 +51,14 @@ void launch_density_error_finish(const double* part_sum, const float* part_max, 
 
 // rows.hip: kernels over CSR rows with a group of lanes per row
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                       const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
                        float* lambda, int4* acc, hipStream_t s);
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
                         const BoxSet& boxes, hipStream_t s);
 uint32_t density_rows_blocks(uint32_t cap);
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                         const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                         const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                          const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
                          float* part_max, hipStream_t s);
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,

@@ -131,62 +131,6 @@ __global__ __launch_bounds__(BLOCK) void k_ranges(const uint32_t* __restrict__ k
     }
 }
 
-// neighborhood_green.comp:57-86: walk the cells of [pos - range, pos + range]
-// (x fastest, then y, then z) and take every particle j != i with
-// distance <= range. COUNT_ONLY counts; otherwise neighbours are written from
-// row_offset[i], dropping those at or beyond pair_cap (neighbor_add.glsl:22-23).
-template <bool COUNT_ONLY>
-__global__ __launch_bounds__(BLOCK) void k_green(const int4* __restrict__ pos,
-                                                 const float* __restrict__ range_in,
-                                                 const uint32_t* __restrict__ counts, float range_scale,
-                                                 GridDesc g, const uint32_t* __restrict__ cs,
-                                                 const uint32_t* __restrict__ ce, uint32_t* __restrict__ cnt,
-                                                 const uint32_t* __restrict__ row, uint32_t* __restrict__ col,
-                                                 uint32_t pair_cap) {
-    const uint32_t n = counts[C_N];
-    GRID_STRIDE(i, n) {
-        float range = range_in[i] * range_scale;
-        v3f p = pos_world(pos[i]);
-        uint32_t lo[3], hi[3];
-        lo[0] = grid_cell(p.x - range, g.mn[0], g.mx[0], g.res);
-        lo[1] = grid_cell(p.y - range, g.mn[1], g.mx[1], g.res);
-        lo[2] = grid_cell(p.z - range, g.mn[2], g.mx[2], g.res);
-        hi[0] = grid_cell(p.x + range, g.mn[0], g.mx[0], g.res);
-        hi[1] = grid_cell(p.y + range, g.mn[1], g.mx[1], g.res);
-        hi[2] = grid_cell(p.z + range, g.mn[2], g.mx[2], g.res);
-        // Bound the walk to 4 * 2^res cells per axis: wider walks only revisit
-        // aliased cells, and an unbounded span (infinite range) would not end.
-        const uint32_t span = 4u << g.res;
-        for (int d = 0; d < 3; ++d)
-            if (hi[d] - lo[d] >= span && hi[d] >= lo[d]) hi[d] = lo[d] + span - 1u;
-        uint32_t c = 0;
-        uint32_t base = COUNT_ONLY ? 0u : row[i];
-        for (uint32_t z = lo[2]; z <= hi[2]; ++z) {
-            for (uint32_t y = lo[1]; y <= hi[1]; ++y) {
-                for (uint32_t x = lo[0]; x <= hi[0]; ++x) {
-                    uint32_t h = zhash(x, y, z, g.res);
-                    uint32_t s = cs[h], e = ce[h];
-                    for (uint32_t j = s; j < e; ++j) {
-                        if (j == i) continue;
-                        v3f q = pos_world(pos[j]);
-                        if (length(p - q) > range) continue;
-                        if (!COUNT_ONLY) {
-                            uint32_t slot = base + c;
-                            if (slot < pair_cap) col[slot] = j;
-                        }
-                        ++c;
-                    }
-                    if (x == 0xFFFFFFFFu) break;
-                }
-                if (y == 0xFFFFFFFFu) break;
-            }
-            if (z == 0xFFFFFFFFu) break;
-        }
-        if (COUNT_ONLY) cnt[i] = c;
-    }
-    if (COUNT_ONLY && blockIdx.x == 0 && threadIdx.x == 0) cnt[n] = 0;
-}
-
 // Pair totals after the row-offset scan: clamp to the capacity and flag overflow.
 __global__ void k_pairs_total(uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
                               uint32_t pair_cap) {
@@ -212,13 +156,24 @@ __global__ void k_set_n_plus_1(uint32_t* __restrict__ counts) {
 }
 
 // Kernel constants per particle (depend on the kernel width only).
+// Per particle: the Gaussian constants, the rest-density factor
+// pow(2 r, D) * inverse mass of incompressibility_2.comp:77, and the
+// particle's own density term of incompressibility_0.comp:43 (as integer bits).
 __global__ __launch_bounds__(BLOCK) void k_kernel_consts(const float* __restrict__ kw,
-                                                         const uint32_t* __restrict__ counts,
-                                                         float2* __restrict__ gc) {
+                                                         const float* __restrict__ radius,
+                                                         const float* __restrict__ inv_mass,
+                                                         const uint32_t* __restrict__ counts, int hid, int gid,
+                                                         float4* __restrict__ gc) {
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) {
-        GaussConst g = gauss_const(kw[i]);
-        gc[i] = make_float2(g.ivd, g.norm);
+        const float k = kw[i];
+        const float im = inv_mass[i];
+        const GaussConst g = gauss_const(k);
+        float w0;
+        v3f unused;
+        kernel_eval(hid, gid, mk(0, 0, 0), k, g, true, false, &w0, &unused);
+        const uint32_t self = to_u32(w0 / im * INCOMP_RES);
+        gc[i] = make_float4(g.ivd, g.norm, pow_f(2.0f * radius[i], DIM) * im, __builtin_bit_cast(float, self));
     }
 }
 
@@ -297,27 +252,16 @@ void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, ui
                    uint32_t* end, hipStream_t s) {
     LAUNCH(k_ranges, stride_grid(cap), keys, d_len, start, end);
 }
-void launch_green_count(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
-                        float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
-                        uint32_t* cnt, hipStream_t s) {
-    LAUNCH(k_green<true>, stride_grid(cap, 8192), pos, range, counts, scale, g, cs, ce, cnt,
-           (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u);
-}
-void launch_green_fill(const int4* pos, const float* range, const uint32_t* counts, uint32_t cap,
-                       float scale, const GridDesc& g, const uint32_t* cs, const uint32_t* ce,
-                       const uint32_t* row, uint32_t* col, uint32_t pair_cap, hipStream_t s) {
-    LAUNCH(k_green<false>, stride_grid(cap, 8192), pos, range, counts, scale, g, cs, ce,
-           (uint32_t*)nullptr, row, col, pair_cap);
-}
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s) {
     hipLaunchKernelGGL(k_pairs_total, dim3(1), dim3(64), 0, s, counts, row, pair_cap);
 }
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s) {
     hipLaunchKernelGGL(k_set_n_plus_1, dim3(1), dim3(64), 0, s, counts);
 }
-void launch_kernel_consts(const float* kw, const uint32_t* counts, uint32_t cap, float2* gc,
+void launch_kernel_consts(const float* kw, const float* radius, const float* inv_mass, const uint32_t* counts,
+                          uint32_t cap, int hid, int gid, float4* gc,
                           hipStream_t s) {
-    LAUNCH(k_kernel_consts, stride_grid(cap), kw, counts, gc);
+    LAUNCH(k_kernel_consts, stride_grid(cap), kw, radius, inv_mass, counts, hid, gid, gc);
 }
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s) {

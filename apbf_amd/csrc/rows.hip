@@ -12,8 +12,8 @@
 //   k_solve_rows: incompressibility_0/1 sums over row i, incompressibility_2
 //     (lambda, boundariness), then the position shifts of
 //     incompressibility_2.comp:106-109 (particle i) and _3.comp:62-66 (every
-//     neighbour j of row i, gradient recomputed from the same inputs) added
-//     with int32 atomics into a per-particle accumulator, and the boundariness
+//     neighbour j of row i, with the gradient of the first loop) added with
+//     int32 atomics into a per-particle accumulator, and the boundariness
 //     filter of _3.comp:41-46;
 //   k_solve_apply: position += accumulator (and the box collision that opens
 //     the next solver iteration).
@@ -64,12 +64,25 @@ __device__ __forceinline__ uint64_t group_mask() {
 
 #define ROW_LOOP_END }
 
-template <int L>
+// Kernel evaluation with the height / gradient kernel ids fixed at compile
+// time (KH, KG >= 0) or read at run time (-1).
+template <int KH, int KG>
+__device__ __forceinline__ void kernel_eval_k(int hid, int gid, v3f r, float kw, GaussConst g, bool need_w,
+                                              bool need_g, float* w, v3f* grad) {
+    kernel_eval(KH >= 0 ? KH : hid, KG >= 0 ? KG : gid, r, kw, g, need_w, need_g, w, grad);
+}
+
+// Gradients of the first C entries of a lane's share of the row stay in
+// registers between the two neighbour loops of k_solve_rows; longer rows
+// recompute the rest.
+constexpr int SOLVE_CACHE = 8;
+
+template <int L, int KH, int KG, int C>
 __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ pos,
                                                       const float* __restrict__ inv_mass,
                                                       const float* __restrict__ radius,
                                                       const float* __restrict__ kw,
-                                                      const float2* __restrict__ gc,
+                                                      const float4* __restrict__ gc,
                                                       const uint32_t* __restrict__ counts,
                                                       const uint32_t* __restrict__ row,
                                                       const uint32_t* __restrict__ col, uint32_t pcap,
@@ -77,28 +90,33 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                                                       float* __restrict__ lambda, int* __restrict__ acc) {
     ROW_LOOP_BEGIN(L)
     int4 pi = make_int4(0, 0, 0, 0);
-    float im = 1.0f, kwi = 1.0f;
+    float im = 1.0f, kwi = 1.0f, ird = 1.0f;
+    uint32_t self = 0;
     GaussConst g{1.0f, 1.0f};
     uint32_t rb = 0, re = 0;
     if (active) {
         pi = pos[i];
         im = inv_mass[i];
         kwi = kw[i];
-        const float2 g2 = gc[i];
-        g = GaussConst{g2.x, g2.y};
+        const float4 g4 = gc[i];
+        g = GaussConst{g4.x, g4.y};
+        ird = g4.z;
+        self = __builtin_bit_cast(uint32_t, g4.w);
         rb = row[i];
         re = min(row[i + 1], pcap);
     }
     // incompressibility_1.comp:43-68 over this lane's share of row i
     uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    for (uint32_t k = rb + gl; k < re; k += L) {
+    uint32_t cj[C];
+    v3f cg[C];
+    auto pair1 = [&](uint32_t k, uint32_t& j_out, v3f& g_out) {
         const uint32_t j = col[k];
         const int4 pj = pos[j];
         const float imN = inv_mass[j];
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         v3f grad;
-        kernel_eval(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
         const v3f wg = grad / imN;
         dens += to_u32(w / imN * INCOMP_RES);
         s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
@@ -112,6 +130,20 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
             m2 += (uint32_t)to_i32(wd.z);
             m3 += (uint32_t)to_i32(INCOMP_RES / imN);
         }
+        j_out = j;
+        g_out = grad;
+    };
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+        const uint32_t k = rb + gl + (uint32_t)t * L;
+        cj[t] = 0;
+        cg[t] = mk(0, 0, 0);
+        if (k < re) pair1(k, cj[t], cg[t]);
+    }
+    for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
+        uint32_t j;
+        v3f gr;
+        pair1(k, j, gr);
     }
     dens = gsum<L>(dens);
     sq = gsum<L>(sq);
@@ -124,15 +156,9 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
         m2 = gsum<L>(m2);
         m3 = gsum<L>(m3);
     }
-    // incompressibility_0.comp:43: the particle's own contribution
-    float w0;
-    v3f unused;
-    kernel_eval(a.hid, a.gid, mk(0, 0, 0), kwi, g, true, false, &w0, &unused);
-    dens += to_u32(w0 / im * INCOMP_RES);
+    dens += self;  // incompressibility_0.comp:43: the particle's own contribution
     // incompressibility_2.comp:77-109 (every lane of the group computes the same values)
     const int32_t gs0 = (int32_t)s0, gs1 = (int32_t)s1, gs2 = (int32_t)s2;
-    const float r = active ? radius[i] : 1.0f;
-    const float ird = pow_f(2.0f * r, DIM) * im;
     const float densf = (float)dens / INCOMP_RES;
     const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
     float sgs = (float)sq / INCOMP_RES;
@@ -167,20 +193,15 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
         }
     }
     // neighbour shifts (incompressibility_3.comp:62-66) and the boundariness
-    // filter (:41-46), over this lane's share of row i
+    // filter (:41-46), over this lane's share of row i, with the gradients of
+    // the first loop (positions are unchanged in between)
     const bool do_shift = active && lam < 0.0f;
     const bool do_filter = active && a.method == 2 && __builtin_bit_cast(uint32_t, b) != 0u;
     bool hit = false;
     if (do_shift || do_filter) {
         const float sc = lam * POS_RES;
         const v3f empty = -normalize(vec_i(gs0, gs1, gs2));
-        for (uint32_t k = rb + gl; k < re; k += L) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float w;
-            v3f grad;
-            kernel_eval(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+        auto pair3 = [&](uint32_t j, v3f grad) {
             if (do_filter && !hit && dot(empty, normalize(grad)) > 0.6f) hit = true;
             if (do_shift) {
                 const v3f d = grad * sc;
@@ -189,6 +210,20 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                 if (dy) atomicAdd(&acc[4 * j + 1], dy);
                 if (dz) atomicAdd(&acc[4 * j + 2], dz);
             }
+        };
+#pragma unroll
+        for (int t = 0; t < C; ++t) {
+            const uint32_t k = rb + gl + (uint32_t)t * L;
+            if (k < re) pair3(cj[t], cg[t]);
+        }
+        for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
+            const uint32_t j = col[k];
+            const int4 pj = pos[j];
+            const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+            float w;
+            v3f grad;
+            kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+            pair3(j, grad);
         }
     }
     const uint64_t ballot = __ballot(hit);
@@ -217,12 +252,12 @@ __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, i
 
 // Density error per particle (the sums of incompressibility_0/1), reduced to
 // per-block (sum, max) partials. Rows exist for particles below counts[C_ROWS].
-template <int L>
+template <int L, int KH>
 __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__ pos,
                                                         const float* __restrict__ inv_mass,
                                                         const float* __restrict__ radius,
                                                         const float* __restrict__ kw,
-                                                        const float2* __restrict__ gc,
+                                                        const float4* __restrict__ gc,
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col, uint32_t pcap,
@@ -235,15 +270,17 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
     const uint32_t rows = counts[C_ROWS];
     ROW_LOOP_BEGIN(L)
     int4 pi = make_int4(0, 0, 0, 0);
-    float im = 1.0f, kwi = 1.0f;
+    float kwi = 1.0f, ird = 1.0f;
+    uint32_t self = 0;
     GaussConst g{1.0f, 1.0f};
     uint32_t rb = 0, re = 0;
     if (active) {
         pi = pos[i];
-        im = inv_mass[i];
         kwi = kw[i];
-        const float2 g2 = gc[i];
-        g = GaussConst{g2.x, g2.y};
+        const float4 g4 = gc[i];
+        g = GaussConst{g4.x, g4.y};
+        ird = g4.z;
+        self = __builtin_bit_cast(uint32_t, g4.w);
         if (i < rows) {
             rb = row[i];
             re = min(row[i + 1], pcap);
@@ -256,15 +293,12 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
         const int4 pj = pos[j];
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
-        kernel_eval(hid, gid, diff, kwi, g, true, false, &w, &unused);
+        kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
         dens += to_u32(w / inv_mass[j] * INCOMP_RES);
     }
     dens = gsum<L>(dens);
     if (active && gl == 0) {
-        float w0;
-        kernel_eval(hid, gid, mk(0, 0, 0), kwi, g, true, false, &w0, &unused);
-        dens += to_u32(w0 / im * INCOMP_RES);
-        const float ird = pow_f(2.0f * radius[i], DIM) * im;
+        dens += self;
         const float e = fmax_g(0.0f, (float)dens / INCOMP_RES * ird - 1.0f);
         acc_sum += (double)e;
         acc_max = fmax_g(acc_max, e);
@@ -407,11 +441,16 @@ static uint32_t row_grid(uint32_t cap) {
 }
 
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                       const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                       const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
                        float* lambda, int4* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_solve_rows<ROW_LANES>, dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, inv_mass, radius, kw,
-                       gc, counts, row, col, pcap, a, boundariness, lambda, reinterpret_cast<int*>(acc));
+    int* ac = reinterpret_cast<int*>(acc);
+    if (a.hid == 1 && a.gid == 1)  // Gaussian height and gradient (the reference's defaults)
+        hipLaunchKernelGGL((k_solve_rows<ROW_LANES, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
+    else
+        hipLaunchKernelGGL((k_solve_rows<ROW_LANES, -1, -1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
 }
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
                         const BoxSet& boxes, hipStream_t s) {
@@ -419,11 +458,15 @@ void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_
 }
 uint32_t density_rows_blocks(uint32_t cap) { return row_grid(cap) < 2048u ? row_grid(cap) : 2048u; }
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
-                         const float2* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
+                         const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                          const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
                          float* part_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_density_rows<ROW_LANES>, dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos, inv_mass,
-                       radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+    if (hid == 1)
+        hipLaunchKernelGGL((k_density_rows<ROW_LANES, 1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+    else
+        hipLaunchKernelGGL((k_density_rows<ROW_LANES, -1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
 }
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,

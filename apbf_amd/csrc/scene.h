@@ -10,8 +10,8 @@ namespace apbf {
 enum KernelId : int {
     K_SOLVE_A = 0,      // k_solve_rows: density, lambda, boundariness, shifts (one iteration)
     K_SOLVE_B = 1,      // k_solve_apply: apply the shifts + next box collision
-    K_GREEN_COUNT = 2,  // k_green<count>: neighbour counting walk
-    K_GREEN_FILL = 3,   // k_green<fill>: neighbour writing walk
+    K_GREEN_COUNT = 2,  // k_walk_rows<count>: neighbour counting walk (with spreading when fused)
+    K_GREEN_FILL = 3,   // k_walk_rows<fill>: neighbour writing walk (with spreading when fused)
     K_SPREAD = 4,       // k_spread_rows<count>: kernel-width spreading (counting pass)
     K_FSM12 = 5,        // k_fsm12_rows: boundary-distance flood fill + nearest neighbour
     K_PERMUTE = 6,      // k_permute: particle SoA gather by the sort permutation
@@ -69,7 +69,7 @@ struct Scene {
     uint32_t *cnt = nullptr, *row = nullptr, *col = nullptr;
     uint32_t* col_tmp = nullptr;
     // solver
-    float2* gc = nullptr;
+    float4* gc = nullptr;  // per particle: Gaussian constants, rest-density factor, own density term
     float* lambda = nullptr;
     int4* shift = nullptr;  // per-particle shift accumulator of the solver (x, y, z, unused)
     // adaptive scratch
@@ -107,7 +107,7 @@ struct Scene {
     void velocity_handling(float dt);
     void particle_transfer(float dt);
     void kernel_width_from_bd();
-    void neighborhood_green(float range_scale);
+    void neighborhood_green(float range_scale, bool spread);
     void spread_kernel_width();
     void box_collision();
     void incompressibility();
@@ -117,7 +117,9 @@ struct Scene {
     apbf_status density_error(double* mean_err, double* max_err);
 
     // helpers
-    void build_pairs(float range_scale);
+    void build_pairs(float range_scale, bool spread);
+    void spread_begin();
+    void spread_end();
     void prepare_solver();
     SolverArgs solver_args() const;
     void incompressibility_iteration(const BoxSet& next_boxes);

@@ -172,7 +172,7 @@ void Scene::velocity_handling(float dt) {
 // ----------------------------------------------------- neighborhood_green
 // neighborhood_green.cpp:51-101: hash every particle, stable sort by hash,
 // permute the particle SoA, cell tables, pair search.
-void Scene::neighborhood_green(float range_scale) {
+void Scene::neighborhood_green(float range_scale, bool spread) {
     hipStream_t s = stream;
     uint32_t upper = 1u << (grid.res * 3u);
     launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s);
@@ -187,22 +187,30 @@ void Scene::neighborhood_green(float range_scale) {
     fill_u32(cell_start, 0u, n_cells, s);
     fill_u32(cell_end, 0u, n_cells, s);
     launch_ranges(skeys, counts + C_N, cap, cell_start, cell_end, s);
-    build_pairs(range_scale);
+    build_pairs(range_scale, spread);
 }
 
-void Scene::build_pairs(float range_scale) {
+// The pair build. With `spread` the walk also carries the kernel-width
+// spreading (kernel_width.cpp:22-71) and writes only its short list, so
+// neighborhood_green followed by spread_kernel_width costs two walks.
+void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;
+    const int on_t = P.s.mBaseKernelWidthOnTargetRadius;
+    uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list
     launch_set_n_plus_1(counts, s);
+    if (spread) spread_begin();
     bool tk = tbegin(K_GREEN_COUNT);
-    launch_green_count(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
-                       cnt, s);
+    launch_walk(false, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s);
     tend(tk);
-    scan_u32(cnt, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    launch_pairs_total(counts, row, pcap, s);
+    scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+    launch_pairs_total(counts, row_raw, pcap, s);
+    if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     tk = tbegin(K_GREEN_FILL);
-    launch_green_fill(pos[cur], kernel_width[cur], counts, cap, range_scale, grid, cell_start, cell_end,
-                      row, col, pcap, s);
+    launch_walk(true, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s);
     tend(tk);
+    if (spread) spread_end();
     pairs_valid = true;
 }
 
@@ -228,7 +236,8 @@ SolverArgs Scene::solver_args() const {
 }
 
 void Scene::prepare_solver() {
-    launch_kernel_consts(kernel_width[cur], counts, cap, gc, stream);
+    launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                         P.s.mGradientKernelId, gc, stream);
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
 }
 
@@ -265,8 +274,8 @@ void Scene::update(float dt) {
     velocity_handling(dt);
     if (!P.basic_pbf && (s.mMerge || s.mSplit)) particle_transfer(dt);
     if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_bd();
-    neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
-    if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
+    const bool spread = !P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance;
+    neighborhood_green(spread ? 1.5f : 1.0f, spread);  // spreading fused into the walk
     solver_loop();
     launch_count_particle_substeps(counts, particle_substeps, stream);
     if (measure_error) accumulate_density_error();
@@ -275,7 +284,8 @@ void Scene::update(float dt) {
 
 void Scene::accumulate_density_error() {
     hipStream_t s = stream;
-    launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
+    launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                         P.s.mGradientKernelId, gc, s);
     launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
                         P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
     launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
@@ -288,7 +298,7 @@ apbf_status Scene::stage(int st, float dt) {
         case APBF_STAGE_PARTICLE_TRANSFER: particle_transfer(dt); break;
         case APBF_STAGE_KERNEL_WIDTH_BD: kernel_width_from_bd(); break;
         case APBF_STAGE_NEIGHBORHOOD:
-            neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f, false);
             break;
         case APBF_STAGE_SPREAD: spread_kernel_width(); break;
         case APBF_STAGE_BOX: box_collision(); break;
@@ -302,7 +312,8 @@ apbf_status Scene::stage(int st, float dt) {
 apbf_status Scene::density_error(double* mean_err, double* max_err) {
     if (!pairs_valid) return APBF_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream;
-    launch_kernel_consts(kernel_width[cur], counts, cap, gc, s);
+    launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                         P.s.mGradientKernelId, gc, s);
     launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
                         P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
     std::vector<double> hs(err_blocks);

@@ -26,6 +26,8 @@ enum CountSlot : uint32_t {
     C_ERR_SUBSTEPS = 15,  // substeps accumulated in the density-error statistics
     C_N4 = 16,         // 4 n (words of the solver's shift accumulator)
     C_PAIRS_PEAK = 17, // largest neighbour-walk pair count (before the clamp) since the scene was created
+    C_CLAMP_ROW = 18,  // row cut by the pair capacity in a spreading walk
+    C_CLAMP_KEPT = 19, // entries of that row kept in the short list
     C_COUNT = 32
 };
 

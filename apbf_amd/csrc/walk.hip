@@ -1,0 +1,206 @@
+// Neighbour search over the cell tables (neighborhood_green.comp:57-86), one
+// group of L lanes of a wavefront per particle. The group visits the cells of
+// the particle's search box in the reference order (z, then y, then x, then
+// the particles of a cell in sorted order) and tests L candidates of a cell
+// per step; a wave ballot turns the accepted candidates of a step into
+// consecutive output slots, so every row keeps the sequential order and the
+// writes of a step are contiguous.
+//
+// Two passes: COUNT writes each row's length, FILL writes the rows at the
+// offsets of the scanned counts. Rows are clamped to the pair capacity the way
+// the sequential schedule fills a bounded list: entries whose slot in the full
+// list is at or beyond the capacity are dropped (neighbor_add.glsl:22-23).
+//
+// With SPREAD the walk also carries kernel_width.comp:22-56: every entry of
+// the (clamped) list spreads the particle's original kernel width to the
+// neighbour (atomicMax, order-free), and only the entries within the
+// particle's cutoff are written, which is the list the spreading pass of the
+// reference leaves behind (kernel_width.cpp:54-71). The full list is then
+// never materialised: COUNT counts both lists, FILL writes the short one.
+#include "scene.h"
+
+namespace apbf {
+
+constexpr int WALK_LANES = 16;
+
+struct WalkArgs {
+    float range_scale;
+    GridDesc g;
+    int on_target;  // spreading: kernel width from max(radius, target radius)
+    uint32_t pcap;
+};
+
+template <int L>
+__device__ __forceinline__ uint64_t walk_group_mask() {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = (L == 64) ? ~0ull : ((1ull << L) - 1ull);
+    return m << (lane & ~(L - 1));
+}
+
+template <int L, bool FILL, bool SPREAD>
+__global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ pos,
+                                                     const float* __restrict__ kw,
+                                                     const float* __restrict__ radius,
+                                                     const float* __restrict__ trad,
+                                                     const uint32_t* __restrict__ cs,
+                                                     const uint32_t* __restrict__ ce, uint32_t* __restrict__ counts,
+                                                     WalkArgs wa, uint32_t* __restrict__ cnt_raw,
+                                                     uint32_t* __restrict__ cnt_f,
+                                                     const uint32_t* __restrict__ row_raw,
+                                                     const uint32_t* __restrict__ row_f, uint32_t* __restrict__ col,
+                                                     uint32_t* __restrict__ ukw) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t gm = walk_group_mask<L>();
+    const uint32_t n = counts[C_N];
+    const int gl = threadIdx.x & (L - 1);
+    const uint32_t groups = BLOCK / L;
+    const GridDesc& g = wa.g;
+    for (uint32_t base = blockIdx.x * groups; base < n; base += gridDim.x * groups) {
+        const uint32_t i = base + threadIdx.x / L;
+        if (i >= n) continue;  // uniform within the group
+        const int4 pi = pos[i];
+        const v3f p = pos_world(pi);
+        const float kwi = kw[i];
+        const float range = kwi * wa.range_scale;
+        uint32_t lo[3], hi[3];
+        lo[0] = grid_cell(p.x - range, g.mn[0], g.mx[0], g.res);
+        lo[1] = grid_cell(p.y - range, g.mn[1], g.mx[1], g.res);
+        lo[2] = grid_cell(p.z - range, g.mn[2], g.mx[2], g.res);
+        hi[0] = grid_cell(p.x + range, g.mn[0], g.mx[0], g.res);
+        hi[1] = grid_cell(p.y + range, g.mn[1], g.mx[1], g.res);
+        hi[2] = grid_cell(p.z + range, g.mn[2], g.mx[2], g.res);
+        // Bound the walk to 4 * 2^res cells per axis: wider walks only revisit
+        // aliased cells, and an unbounded span (infinite range) would not end.
+        const uint32_t span = 4u << g.res;
+        for (int d = 0; d < 3; ++d)
+            if (hi[d] - lo[d] >= span && hi[d] >= lo[d]) hi[d] = lo[d] + span - 1u;
+        float okw = 0.0f, cutoff = 0.0f;
+        if (SPREAD) {
+            const float tr = wa.on_target ? trad[i] : 0.0f;
+            okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
+            cutoff = fmax_g(okw, kwi);
+        }
+        uint32_t craw = 0, cf = 0, braw = 0, bf = 0;
+        bool more = true;
+        if (FILL) {
+            braw = row_raw[i];
+            bf = SPREAD ? row_f[i] : braw;
+            more = braw < wa.pcap;
+        }
+        for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
+            for (uint32_t y = lo[1]; more && y <= hi[1]; ++y) {
+                for (uint32_t x = lo[0]; more && x <= hi[0]; ++x) {
+                    const uint32_t h = zhash(x, y, z, g.res);
+                    const uint32_t s = cs[h], e = ce[h];
+                    for (uint32_t b = s; more && b < e; b += L) {
+                        const uint32_t j = b + gl;
+                        bool green = false, keep = false;
+                        uint32_t uval = 0;
+                        if (j < e && j != i) {
+                            const int4 pj = pos[j];
+                            green = !(length(p - pos_world(pj)) > range);
+                            if (SPREAD && green) {
+                                const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+                                const float dist = length(diff);
+                                keep = dist <= cutoff;
+                                if (FILL) {
+                                    const float dfk = dist - okw;
+                                    const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
+                                    uval = to_u32(okw * infl * KW_RES);
+                                }
+                            }
+                        }
+                        const uint64_t rbal = __ballot(green) & gm;
+                        if (!FILL) {
+                            craw += (uint32_t)__popcll(rbal);
+                            if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep) & gm);
+                        } else {
+                            const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
+                            const bool incl = green && ridx < wa.pcap;
+                            if (SPREAD) {
+                                if (incl && uval) atomicMax(&ukw[j], uval);
+                                const bool kept = incl && keep;
+                                const uint64_t fbal = __ballot(kept) & gm;
+                                if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
+                                cf += (uint32_t)__popcll(fbal);
+                            } else if (incl) {
+                                col[ridx] = j;
+                            }
+                            craw += (uint32_t)__popcll(rbal);
+                            more = braw + craw < wa.pcap;  // later entries fall beyond the capacity
+                        }
+                    }
+                    if (x == 0xFFFFFFFFu) break;
+                }
+                if (y == 0xFFFFFFFFu) break;
+            }
+            if (z == 0xFFFFFFFFu) break;
+        }
+        if (gl == 0) {
+            if (!FILL) {
+                cnt_raw[i] = craw;
+                if (SPREAD) cnt_f[i] = cf;
+            } else if (SPREAD && braw <= wa.pcap && row_raw[i + 1] > wa.pcap) {
+                // the row the capacity cuts: its kept short-list length
+                counts[C_CLAMP_ROW] = i;
+                counts[C_CLAMP_KEPT] = cf;
+            }
+        }
+    }
+    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt_raw[n] = 0;
+        if (SPREAD) cnt_f[n] = 0;
+    }
+}
+
+// After a clamped spreading walk: rows after the cut row are empty, so their
+// offsets all equal the end of the cut row's kept entries.
+__global__ __launch_bounds__(BLOCK) void k_fix_clamped_rows(const uint32_t* __restrict__ counts, uint32_t pcap,
+                                                            uint32_t* __restrict__ row_f) {
+    if (counts[C_PAIRS_RAW] <= pcap) return;
+    const uint32_t n = counts[C_N];
+    const uint32_t p = counts[C_CLAMP_ROW];
+    const uint32_t end = row_f[p] + counts[C_CLAMP_KEPT];
+    for (uint32_t k = p + 1 + blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += gridDim.x * blockDim.x)
+        row_f[k] = end;
+}
+
+static uint32_t walk_grid(uint32_t cap) {
+    const uint32_t groups = BLOCK / WALK_LANES;
+    uint32_t g = div_up(cap ? cap : 1, groups);
+    return g < 16384u ? g : 16384u;
+}
+
+void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
+                 int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
+                 float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
+                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s) {
+    WalkArgs wa;
+    wa.range_scale = range_scale;
+    wa.g = g;
+    wa.on_target = on_target;
+    wa.pcap = pcap;
+    const dim3 grid(walk_grid(cap)), block(BLOCK);
+#define WALK(F, S)                                                                                                   \
+    hipLaunchKernelGGL((k_walk_rows<WALK_LANES, F, S>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, \
+                       cnt_raw, cnt_f, row_raw, row_f, col, ukw)
+    if (fill) {
+        if (spread)
+            WALK(true, true);
+        else
+            WALK(true, false);
+    } else {
+        if (spread)
+            WALK(false, true);
+        else
+            WALK(false, false);
+    }
+#undef WALK
+}
+
+void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s) {
+    hipLaunchKernelGGL(k_fix_clamped_rows, dim3(stride_grid(cap + 1)), dim3(BLOCK), 0, s, counts, pcap, row_f);
+}
+
+}  // namespace apbf

@@ -121,7 +121,7 @@ def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx,
     row = torch.empty(n + 1, dtype=torch.int32, device=dev)
     col = torch.empty(max(pair_cap, 1), dtype=torch.int32, device=dev)
     npairs = torch.zeros(1, dtype=torch.int32, device=dev)
-    ws = _ws(64 + (n + 1) * 4 + L.apbf_prefix_sum_workspace_size(n + 1), dev)
+    ws = _ws(L.apbf_neighborhood_green_workspace_size(n), dev)
     fmn = (C.c_float * 3)(*mn)
     fmx = (C.c_float * 3)(*mx)
     check(L.apbf_neighborhood_green(_p(position), _p(rng), _p(_len(n, dev)), n, float(range_scale), fmn, fmx,

@@ -220,7 +220,8 @@ apbf_status apbf_find_value_ranges(const uint32_t* sorted_keys, const uint32_t* 
  * row_offset[i+1] holds the neighbours of particle i in the reference's cell-
  * walk order; pairs beyond pair_capacity are dropped (the reference's clamp,
  * neighbor_add.glsl:22-23) and *d_n_pairs receives the clamped total.
- * `workspace` >= apbf_prefix_sum_workspace_size(capacity + 1). */
+ * `workspace` >= apbf_neighborhood_green_workspace_size(capacity). */
+size_t apbf_neighborhood_green_workspace_size(uint32_t capacity);
 apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
                                     const uint32_t* d_len, uint32_t capacity, float range_scale,
                                     const float* host_min3, const float* host_max3,
