@@ -45,6 +45,25 @@ APBF_HD uint32_t to_u32(float x) {
     return (uint32_t)x;
 }
 
+// Largest float t with sqrt(t) <= r (IEEE square root), so that for every
+// float d2 >= 0 the test sqrt(d2) <= r holds exactly when d2 <= t: a distance
+// comparison without the square root. -inf when no d2 qualifies (r < 0, NaN).
+// t starts at r * r, within a few units in the last place of the bound: each
+// float r is the rounded root of at most a handful of consecutive floats.
+APBF_HD float sqrt_le_bound(float r) {
+    if (!(r >= 0.0f)) return -__builtin_inff();
+    if (r == __builtin_inff()) return r;
+    float t = r * r;
+    for (int k = 0; k < 8 && __builtin_sqrtf(t) > r; ++k)
+        t = bits_f(__builtin_bit_cast(uint32_t, t) - 1u);  // t > 0 here: sqrt(0) <= r
+    for (int k = 0; k < 8; ++k) {
+        const float u = bits_f(__builtin_bit_cast(uint32_t, t) + 1u);
+        if (!(__builtin_sqrtf(u) <= r)) break;
+        t = u;
+    }
+    return t;
+}
+
 APBF_HD float exp_f(float x) {
     if (x != x) return x;
     if (x <= -86.0f) return 0.0f;

@@ -117,12 +117,30 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
         float w;
         v3f grad;
         kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
-        const v3f wg = grad / imN;
-        dens += to_u32(w / imN * INCOMP_RES);
+        const float gg = dot(grad, grad);
+        float wq, ggq;
+        v3f wg;
+        // Inverse masses are powers of two unless a merge mixed two masses.
+        // Dividing by a power of two whose reciprocal is a normal float is the
+        // exact scaling by that reciprocal (one rounding of the same real
+        // value), so the quotients below are the IEEE quotients either way.
+        const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
+        const uint32_t ie = (ib >> 23) & 0xFFu;
+        if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u)) {
+            const float rim = __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
+            wq = w * rim;
+            wg = grad * rim;
+            ggq = gg * rim;
+        } else {
+            wq = w / imN;
+            wg = grad / imN;
+            ggq = gg / imN;
+        }
+        dens += to_u32(wq * INCOMP_RES);
         s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
         s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
         s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
-        sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
+        sq += to_u32(ggq * INCOMP_RES);
         if (a.method == 1) {
             const v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / (imN * kwi);
             m0 += (uint32_t)to_i32(wd.x);
@@ -294,7 +312,15 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
-        dens += to_u32(w / inv_mass[j] * INCOMP_RES);
+        const float imN = inv_mass[j];
+        const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
+        const uint32_t ie = (ib >> 23) & 0xFFu;
+        float wq;
+        if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u))  // exact scaling, as in k_solve_rows
+            wq = w * __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
+        else
+            wq = w / imN;
+        dens += to_u32(wq * INCOMP_RES);
     }
     dens = gsum<L>(dens);
     if (active && gl == 0) {

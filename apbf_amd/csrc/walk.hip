@@ -21,7 +21,7 @@
 
 namespace apbf {
 
-constexpr int WALK_LANES = 16;
+constexpr int WALK_LANES = 8;
 
 struct WalkArgs {
     float range_scale;
@@ -75,11 +75,16 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
         const uint32_t span = 4u << g.res;
         for (int d = 0; d < 3; ++d)
             if (hi[d] - lo[d] >= span && hi[d] >= lo[d]) hi[d] = lo[d] + span - 1u;
-        float okw = 0.0f, cutoff = 0.0f;
+        // Distance tests compare squared distances with exact bounds (the
+        // reference compares rounded square roots; sqrt_le_bound makes the two
+        // agree for every input). A NaN range keeps every candidate.
+        const float t_range = (range != range) ? __builtin_inff() : sqrt_le_bound(range);
+        float okw = 0.0f, t_cut = 0.0f, t_okw = 0.0f;
         if (SPREAD) {
             const float tr = wa.on_target ? trad[i] : 0.0f;
             okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
-            cutoff = fmax_g(okw, kwi);
+            t_cut = sqrt_le_bound(fmax_g(okw, kwi));
+            t_okw = sqrt_le_bound(okw);
         }
         uint32_t craw = 0, cf = 0, braw = 0, bf = 0;
         bool more = true;
@@ -99,15 +104,21 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                         uint32_t uval = 0;
                         if (j < e && j != i) {
                             const int4 pj = pos[j];
-                            green = !(length(p - pos_world(pj)) > range);
+                            const v3f d = p - pos_world(pj);
+                            green = !(dot(d, d) > t_range);
                             if (SPREAD && green) {
                                 const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-                                const float dist = length(diff);
-                                keep = dist <= cutoff;
+                                const float d2 = dot(diff, diff);
+                                keep = d2 <= t_cut;
                                 if (FILL) {
-                                    const float dfk = dist - okw;
-                                    const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
-                                    uval = to_u32(okw * infl * KW_RES);
+                                    if (d2 <= t_okw) {  // distance <= okw: influence exactly 1
+                                        uval = to_u32(okw * KW_RES);
+                                    } else {
+                                        const float dfk = __builtin_sqrtf(d2) - okw;
+                                        const float infl =
+                                            fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
+                                        uval = to_u32(okw * infl * KW_RES);
+                                    }
                                 }
                             }
                         }

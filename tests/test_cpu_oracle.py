@@ -258,3 +258,25 @@ def test_library_without_device_fails_loudly():
     p, _ = cfg.dam_break(n_side=4)
     with pytest.raises(_lib.ApbfError):
         Scene(p)
+
+
+def test_power_of_two_division_is_exact_scaling():
+    """The neighbour sums divide by inverse masses; for a power of two m whose
+    reciprocal is a normal float, x / m equals x * (1 / m) bit for bit (the
+    HIP kernels take that path when a whole wavefront qualifies)."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([
+        rng.standard_normal(4000).astype(np.float32) * np.float32(10.0) ** rng.integers(-30, 30, 4000),
+        np.array([0.0, -0.0, np.inf, -np.inf, 1e-45, -1e-45, 1e-40, 3.4e38, -3.4e38, 1.17549435e-38],
+                 np.float32),
+        (rng.integers(1, 1 << 23, 500, dtype=np.uint32)).view(np.float32),  # subnormals
+    ]).astype(np.float32)
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        for e in range(1, 254):
+            for sign in (0, 1):
+                m = np.array([(sign << 31) | (e << 23)], np.uint32).view(np.float32)[0]
+                rim = np.array([(sign << 31) | ((254 - e) << 23)], np.uint32).view(np.float32)[0]
+                assert rim == np.float32(1.0) / m
+                q = x / m
+                s = x * rim
+                assert np.array_equal(q.view(np.uint32), s.view(np.uint32)), (e, sign)

@@ -225,3 +225,46 @@ def test_copy_scattered_read(device):
     vec = np.array(FIX["gpu_list_concatenation_2"]["expected"], np.float32)
     out3 = ops.copy_scattered_read(T(vec.view(np.uint32).ravel(), device), T([4, 0, 2], device), 12)
     assert np.array_equal(N(out3).view(np.float32).reshape(3, 3), vec[[4, 0, 2]])
+
+
+def _world_dist(pa, pb):
+    """Float32 distance between fixed-point positions the way the walk forms it:
+    world coordinates first, then difference, dot product and square root."""
+    wa = pa[:, :3].astype(np.float32) / np.float32(262144.0)
+    wb = pb[:, :3].astype(np.float32) / np.float32(262144.0)
+    d = wa - wb
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    return np.sqrt(d2.astype(np.float32))
+
+
+def test_neighborhood_green_boundary_ranges(device):
+    """Ranges equal to a neighbour's rounded distance, and one float step either
+    side: the walk's squared-distance bound must keep exactly the neighbours
+    whose rounded distance is <= range."""
+    from apbf_amd import ops
+    import torch
+    mn, mx, res = (0.0, 0.0, 0.0), (8.0, 8.0, 8.0), 3
+    p, h = _sorted_cloud(3375, 21, mn, mx, res)
+    rng = np.random.default_rng(8)
+    partner = np.clip(np.arange(len(p)) + rng.integers(-40, 40, len(p)), 0, len(p) - 1)
+    partner[partner == np.arange(len(p))] = (partner[partner == np.arange(len(p))] + 1) % len(p)
+    rad = _world_dist(p, p[partner]).astype(np.float32)
+    step = rng.integers(-1, 2, len(p))
+    rad = np.where(step < 0, np.nextafter(rad, np.float32(0)), np.where(step > 0, np.nextafter(rad, np.float32(np.inf)), rad)).astype(np.float32)
+    rad[:5] = np.array([0.0, -1.0, np.nan, np.inf, 1e-30], np.float32)
+    rs, re_ = O.find_value_ranges(h, 1 << (3 * res))
+    cap = 20_000_000
+    want = O.neighborhood_green(p, rad, rs, re_, 1.0, mn, mx, res, cap)
+    row, col, npairs = ops.neighborhood_green(torch.from_numpy(p).to(device), torch.from_numpy(rad).to(device),
+                                              T(rs, device), T(re_, device), 1.0, mn, mx, res, cap)
+    npairs = int(N(npairs)[0])
+    assert npairs == len(want)
+    got = np.stack([np.repeat(np.arange(len(p)), np.diff(N(row))), N(col)[:npairs]], 1)
+    assert np.array_equal(got, want)
+    # the chosen partner sits exactly on the boundary for step 0 (kept) and -1 (dropped)
+    pairs = set(map(tuple, got.tolist()))
+    for i in range(5, len(p)):
+        if step[i] == 0:
+            assert (i, partner[i]) in pairs
+        elif step[i] < 0:
+            assert (i, partner[i]) not in pairs

@@ -120,3 +120,68 @@ def test_transfer_capacity_clamp(device):
         g.update(p["dt"])
         r.update(p["dt"])
         _check(g, r, f"substep {step}")
+
+
+@pytest.mark.parametrize("bd_mode", [False, True])
+def test_pair_capacity_clamp(device, bd_mode):
+    """A pair capacity below the walk's pair count cuts the neighbour list in
+    the middle of a row: the spreading walk (bd_mode False) and the plain walk
+    (bd_mode True) must drop the same entries as the sequential schedule."""
+    p, st = cfg.small_water_drop(seed=4, active=True)
+    if bd_mode:
+        p["settings"]["mBaseKernelWidthOnBoundaryDistance"] = 1
+    g0 = OracleScene(p)
+    g0.set_state(st)
+    g0.stage("velocity", p["dt"])
+    if bd_mode:
+        g0.stage("kernel_width_bd", p["dt"])
+    g0.stage("neighborhood", p["dt"])
+    full = len(g0.get_state()["pairs"])
+    p["cap_pairs"] = int(full * 0.6) + 7  # not a row boundary in general
+    g, r = _pair(p, st)
+    for step in range(3):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step}")
+    assert g.overflow_flags() & 1
+
+
+def test_spreading_boundary_widths(device):
+    """Kernel widths equal to a neighbour's rounded distance (and one float step
+    either side) put pairs exactly on the cutoff of the spreading filter, and
+    an exact lattice puts pairs exactly at the original kernel width, where
+    the spread influence changes formula. Gravity and velocities are zero so
+    the positions reach the neighbour search unchanged."""
+    p, st = cfg.small_water_drop(seed=6)
+    p["gravity"] = (0.0, 0.0, 0.0)
+    n = len(st["radius"])
+    rng = np.random.default_rng(12)
+    st["velocity"][:] = 0.0
+    # an exact lattice block at spacing 2 (= 2 r for the coarse particles)
+    k = min(n, 512)
+    ijk = np.stack(np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij"), -1).reshape(-1, 3)[:k]
+    base = st["position"][:, :3].min(0)
+    st["position"][:k, :3] = base + (ijk * 2 * 262144).astype(np.int32)
+    st["position"][:k, 3] = 0
+    st["radius"][:k] = 1.0
+    st["inverse_mass"][:k] = 0.125
+    st["target_radius"][:k] = 1.0
+    st["pos_backup"] = st["position"].copy()
+    # widths at exact rounded distances to a random other particle
+    pos = st["position"]
+    partner = np.clip(np.arange(n) + rng.integers(-30, 30, n), 0, n - 1)
+    partner[partner == np.arange(n)] = (partner[partner == np.arange(n)] + 1) % n
+    diff = (pos[partner, :3].astype(np.int64) - pos[:, :3].astype(np.int64)).astype(np.int32)
+    dw = diff.astype(np.float32) / np.float32(262144.0)
+    d2 = (dw[:, 0] * dw[:, 0] + dw[:, 1] * dw[:, 1]) + dw[:, 2] * dw[:, 2]
+    dist = np.sqrt(d2.astype(np.float32))
+    step = rng.integers(-1, 2, n)
+    kw = np.where(step < 0, np.nextafter(dist, np.float32(0)),
+                  np.where(step > 0, np.nextafter(dist, np.float32(np.inf)), dist)).astype(np.float32)
+    use = dist > st["kernel_width"]  # only where the width, not the radius, sets the cutoff
+    st["kernel_width"] = np.where(use, kw, st["kernel_width"]).astype(np.float32)
+    g, r = _pair(p, st)
+    for step_i in range(2):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step_i}")
