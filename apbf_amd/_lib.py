@@ -104,7 +104,7 @@ PROTOTYPES = {
     "apbf_scene_density_error": (_i32, [_vp, _pd, _pd]),
     "apbf_scene_overflow_flags": (_i32, [_vp, _pu]),
     "apbf_scene_set_error_measurement": (_i32, [_vp, _i32]),
-    "apbf_scene_error_stats": (_i32, [_vp, _pd, _pd, _pd, _pu, _i32]),
+    "apbf_scene_error_stats": (_i32, [_vp, _pd, _pd, _pd, _pd, _pu, _i32]),
     "apbf_scene_particle_substeps": (_i32, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _i32]),
     "apbf_scene_time_kernel": (_i32, [_vp, _i32, _u32]),
     "apbf_scene_diagnostics": (_i32, [_vp, _vp, _u32]),

@@ -191,29 +191,38 @@ __global__ __launch_bounds__(BLOCK) void k_density_error_finish(const double* __
                                                                 uint32_t blocks, uint32_t* __restrict__ counts,
                                                                 double* __restrict__ stats) {
     __shared__ double ssum[BLOCK];
+    __shared__ double sabs[BLOCK];
     __shared__ double smax[BLOCK];
-    double a = 0.0, m = 0.0;
+    double a = 0.0, ab = 0.0, m = 0.0;
     for (uint32_t b = threadIdx.x; b < blocks; b += BLOCK) {
         a += part_sum[b];
+        ab += part_sum[blocks + b];
         m = m > (double)part_max[b] ? m : (double)part_max[b];
     }
     ssum[threadIdx.x] = a;
+    sabs[threadIdx.x] = ab;
     smax[threadIdx.x] = m;
     __syncthreads();
     for (int s = BLOCK / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            sabs[threadIdx.x] += sabs[threadIdx.x + s];
             smax[threadIdx.x] = smax[threadIdx.x] > smax[threadIdx.x + s] ? smax[threadIdx.x] : smax[threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        // stats: sum of per-substep means, max, last mean, last max, sum of
+        // per-substep absolute means, last absolute mean
         uint32_t n = counts[C_N];
         double mean = n ? ssum[0] / (double)n : 0.0;
+        double mean_abs = n ? sabs[0] / (double)n : 0.0;
         stats[0] += mean;
         stats[1] = stats[1] > smax[0] ? stats[1] : smax[0];
         stats[2] = mean;
         stats[3] = smax[0];
+        stats[4] += mean_abs;
+        stats[5] = mean_abs;
         counts[C_ERR_SUBSTEPS] += 1u;
     }
 }

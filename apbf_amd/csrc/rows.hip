@@ -269,7 +269,9 @@ __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, i
 }
 
 // Density error per particle (the sums of incompressibility_0/1), reduced to
-// per-block (sum, max) partials. Rows exist for particles below counts[C_ROWS].
+// per-block partials: the sum of max(0, rho/rho0 - 1) in part_sum[block], the
+// sum of |rho/rho0 - 1| in part_sum[gridDim.x + block], the max in part_max.
+// Rows exist for particles below counts[C_ROWS].
 template <int L, int KH>
 __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__ pos,
                                                         const float* __restrict__ inv_mass,
@@ -282,8 +284,9 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
                                                         int hid, int gid, double* __restrict__ part_sum,
                                                         float* __restrict__ part_max) {
     __shared__ double ssum[BLOCK];
+    __shared__ double sabs[BLOCK];
     __shared__ float smax[BLOCK];
-    double acc_sum = 0.0;
+    double acc_sum = 0.0, acc_abs = 0.0;
     float acc_max = 0.0f;
     const uint32_t rows = counts[C_ROWS];
     ROW_LOOP_BEGIN(L)
@@ -325,23 +328,28 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
     dens = gsum<L>(dens);
     if (active && gl == 0) {
         dens += self;
-        const float e = fmax_g(0.0f, (float)dens / INCOMP_RES * ird - 1.0f);
+        const float dev = (float)dens / INCOMP_RES * ird - 1.0f;
+        const float e = fmax_g(0.0f, dev);
         acc_sum += (double)e;
+        acc_abs += (double)__builtin_fabsf(dev);
         acc_max = fmax_g(acc_max, e);
     }
     ROW_LOOP_END
     ssum[threadIdx.x] = acc_sum;
+    sabs[threadIdx.x] = acc_abs;
     smax[threadIdx.x] = acc_max;
     __syncthreads();
     for (int s = BLOCK / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            sabs[threadIdx.x] += sabs[threadIdx.x + s];
             smax[threadIdx.x] = fmax_g(smax[threadIdx.x], smax[threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         part_sum[blockIdx.x] = ssum[0];
+        part_sum[gridDim.x + blockIdx.x] = sabs[0];
         part_max[blockIdx.x] = smax[0];
     }
 }

@@ -8,10 +8,10 @@ namespace apbf {
 
 // Kernels that can be timed with HIP events (apbf_scene_time_kernel).
 enum KernelId : int {
-    K_SOLVE_A = 0,      // k_solve_rows: density, lambda, boundariness, shifts (one iteration)
-    K_SOLVE_B = 1,      // k_solve_apply: apply the shifts + next box collision
-    K_GREEN_COUNT = 2,  // k_walk_rows<count>: neighbour counting walk (with spreading when fused)
-    K_GREEN_FILL = 3,   // k_walk_rows<fill>: neighbour writing walk (with spreading when fused)
+    K_SOLVE_ROWS = 0,      // k_solve_rows: density, lambda, boundariness, shifts (one iteration)
+    K_SOLVE_APPLY = 1,      // k_solve_apply: apply the shifts + next box collision
+    K_WALK_COUNT = 2,  // k_walk_rows<count>: neighbour counting walk (with spreading when fused)
+    K_WALK_FILL = 3,   // k_walk_rows<fill>: neighbour writing walk (with spreading when fused)
     K_SPREAD = 4,       // k_spread_rows<count>: kernel-width spreading (counting pass)
     K_FSM12 = 5,        // k_fsm12_rows: boundary-distance flood fill + nearest neighbour
     K_PERMUTE = 6,      // k_permute: particle SoA gather by the sort permutation
@@ -30,7 +30,7 @@ struct Scene {
     int tcur = 0;          // current transfer buffer set
     bool pairs_valid = false;    // CSR rows refer to the current particle slots
     bool measure_error = false;  // accumulate the density error after every solver loop
-    double* err_stats = nullptr; // [sum of per-substep means, max, last mean, last max]
+    double* err_stats = nullptr; // [sum of means, max, last mean, last max, sum of abs means, last abs mean]
     unsigned long long* particle_substeps = nullptr;  // sum over substeps of the solver's particle count
 
     // Optional HIP-event timing of one kernel (KernelId) on the scene stream.
@@ -83,7 +83,7 @@ struct Scene {
     int claim_rounds = 6;
     // density error partials
     uint32_t err_blocks = 0;
-    double* part_sum = nullptr;
+    double* part_sum = nullptr;  // 2 x err_blocks: compression-only sums, then absolute sums
     float* part_max = nullptr;
     void* ws_mem = nullptr;
     size_t ws_bytes = 0;

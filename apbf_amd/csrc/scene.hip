@@ -126,9 +126,9 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(t_flag, tcap + 1);
     A(t_scan, tcap + 1);
     err_blocks = density_rows_blocks(cap);
-    A(part_sum, err_blocks);
+    A(part_sum, 2 * err_blocks);
     A(part_max, err_blocks);
-    A(err_stats, 4);
+    A(err_stats, 6);
     A(particle_substeps, 2);
 #undef A
     size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
@@ -137,7 +137,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     ws_bytes = ws;
     HIP_OK(hipMalloc(&ws_mem, ws_bytes));
     HIP_OK(hipMemset(counts, 0, C_COUNT * 4));
-    HIP_OK(hipMemset(err_stats, 0, 4 * sizeof(double)));
+    HIP_OK(hipMemset(err_stats, 0, 6 * sizeof(double)));
     HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
@@ -199,14 +199,14 @@ void Scene::build_pairs(float range_scale, bool spread) {
     uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list
     launch_set_n_plus_1(counts, s);
     if (spread) spread_begin();
-    bool tk = tbegin(K_GREEN_COUNT);
+    bool tk = tbegin(K_WALK_COUNT);
     launch_walk(false, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                 cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s);
     tend(tk);
     scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     launch_pairs_total(counts, row_raw, pcap, s);
     if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    tk = tbegin(K_GREEN_FILL);
+    tk = tbegin(K_WALK_FILL);
     launch_walk(true, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                 cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s);
     tend(tk);
@@ -244,11 +244,11 @@ void Scene::prepare_solver() {
 void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
     hipStream_t s = stream;
     SolverArgs a = solver_args();
-    bool tk = tbegin(K_SOLVE_A);
+    bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
                       boundariness[cur], lambda, shift, s);
     tend(tk);
-    tk = tbegin(K_SOLVE_B);
+    tk = tbegin(K_SOLVE_APPLY);
     launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s);
     tend(tk);
 }
@@ -451,10 +451,10 @@ apbf_status apbf_scene_set_error_measurement(void* scene, int32_t enable) {
 }
 
 apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* max_err, double* last_mean,
-                                   uint32_t* substeps, int32_t reset) {
+                                   double* mean_abs_of_means, uint32_t* substeps, int32_t reset) {
     if (!scene) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
-    double st[4];
+    double st[6];
     uint32_t n = 0;
     if (hipMemcpyAsync(st, s->err_stats, sizeof(st), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
         hipMemcpyAsync(&n, s->counts + apbf::C_ERR_SUBSTEPS, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
@@ -463,6 +463,7 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
     if (mean_of_means) *mean_of_means = n ? st[0] / (double)n : 0.0;
     if (max_err) *max_err = st[1];
     if (last_mean) *last_mean = st[2];
+    if (mean_abs_of_means) *mean_abs_of_means = n ? st[4] / (double)n : 0.0;
     if (substeps) *substeps = n;
     if (reset) {
         if (hipMemsetAsync(s->err_stats, 0, sizeof(st), s->stream) != hipSuccess ||

@@ -124,12 +124,13 @@ class Scene:
         check(self._L.apbf_scene_set_error_measurement(self._h, int(enable)), "set_error_measurement")
 
     def error_stats(self, reset: bool = False):
-        """(mean over substeps of the mean density error, max error, last mean, substeps)."""
-        m, x, l = C.c_double(), C.c_double(), C.c_double()
+        """(mean over substeps of the mean density error, max error, last mean,
+        mean over substeps of the mean absolute deviation, substeps)."""
+        m, x, l, a = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         n = C.c_uint32()
-        check(self._L.apbf_scene_error_stats(self._h, C.byref(m), C.byref(x), C.byref(l), C.byref(n),
+        check(self._L.apbf_scene_error_stats(self._h, C.byref(m), C.byref(x), C.byref(l), C.byref(a), C.byref(n),
                                              int(reset)), "error_stats")
-        return m.value, x.value, l.value, n.value
+        return m.value, x.value, l.value, a.value, n.value
 
     def particle_substeps(self, reset: bool = False):
         """(sum of particle counts, sum of pair counts) over substeps since the last reset."""
@@ -138,7 +139,7 @@ class Scene:
               "particle_substeps")
         return v.value, p.value
 
-    KERNELS = {"solve_a": 0, "solve_b": 1, "green_count": 2, "green_fill": 3, "spread": 4, "fsm12": 5,
+    KERNELS = {"solve_rows": 0, "solve_apply": 1, "walk_count": 2, "walk_fill": 3, "spread": 4, "fsm12": 5,
                "permute": 6}
 
     def time_kernel(self, name: str | None, max_launches: int = 4096) -> None:

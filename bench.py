@@ -20,18 +20,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 # Algorithmic bytes of each timeable kernel, per particle and per neighbour
-# pair, from the fixed byte model of the substep (unpadded elements, inputs read
-# and outputs written once, neighbour gathers assumed cache-served).
+# pair of the solver's list, from the fixed byte model of the substep
+# (unpadded elements: xyz 12 B, scalars 4 B, pair entry 8 B; inputs read and
+# outputs written once; neighbour-side gathers assumed cache-served).
 BYTE_MODEL = {
-    "solve_a": (36.0, 8.0),      # density / lambda pass
-    "solve_b": (28.0, 8.0),      # position-delta pass
-    "green_fill": (16.0, 8.0),   # pair build
-    "green_count": (16.0, 0.0),  # counting walk of the pair build (reads the particle, writes a count)
-    "spread": (24.0, 16.0),      # kernel-width spreading
-    "fsm12": (38.0, 8.0),        # split/merge find, pair part
+    # density/lambda pass (36 B + 8 B/pair) plus the pair reads of the
+    # position-delta pass (8 B/pair): the row pass carries both
+    "solve_rows": (36.0, 16.0),
+    # position-delta pass, particle part (28 B) + the next box collision (28 B)
+    "solve_apply": (56.0, 0.0),
     "permute": (100.0, 0.0),     # permute 48 B of state: index 4 + gather 48 + write 48
+    "fsm12": (38.0, 16.0),       # split/merge find, flood fill and nearest neighbour
 }
 
 
@@ -43,6 +45,32 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def committed_traffic(kernel: str, workload_name: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel
+    calibration), or None when no summary covers this kernel and workload."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload_name}_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(n_bytes: int = 1 << 30) -> None:
+    """A plain device copy of known size (read n_bytes, write n_bytes), three
+    times, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "launches": 3}), flush=True)
 
 
 def workload(name: str):
@@ -86,10 +114,15 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="water_drop")
-    ap.add_argument("--timed-kernel", default="solve_a", choices=sorted(BYTE_MODEL))
+    ap.add_argument("--timed-kernel", default="solve_rows", choices=sorted(BYTE_MODEL))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build is single-GPU (the multi-GPU design is "
@@ -128,7 +161,7 @@ def main():
     ps, pairs = scene.particle_substeps()
     k_ms, k_launches = scene.kernel_time()
     scene.time_kernel(None)
-    mean_err, max_err, last_err, err_steps = scene.error_stats()
+    mean_err, max_err, last_err, mean_abs, err_steps = scene.error_stats()
     n_now, nt_now, np_now = scene.counts()
     diag = scene.diagnostics()
     if diag["overflow_flags"]:
@@ -179,7 +212,8 @@ def main():
             "pair_capacity": params["cap_pairs"],
             "capacity_clamped": bool(diag["overflow_flags"]),
         },
-        "density_error": {"mean_of_substep_means": mean_err, "max": max_err, "substeps": err_steps},
+        "density_error": {"mean_of_substep_means": mean_err, "max": max_err,
+                          "mean_abs_of_substep_means": mean_abs, "substeps": err_steps},
         "roofline": {
             "bound": "hbm",
             "kernel": args.timed_kernel,
@@ -187,7 +221,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": committed_traffic(args.timed_kernel, args.workload),
             "launches_per_step": launches_per_step,
             "avg_launch_ms": avg_launch_s * 1e3 if k_launches else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,

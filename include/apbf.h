@@ -146,18 +146,20 @@ apbf_status apbf_scene_density_error(void* scene, double* mean_err, double* max_
  * every substep ends its solver loop with a density pass (as above) whose mean
  * and max are added to device-resident accumulators, without synchronising.
  * apbf_scene_error_stats returns the mean over substeps of the per-substep mean
- * error, the maximum error, the last substep's mean and the substep count, and
- * clears them when `reset` is non-zero (synchronises). */
+ * error max(0, rho/rho0 - 1), the maximum error, the last substep's mean, the
+ * mean over substeps of the mean absolute deviation |rho/rho0 - 1| and the
+ * substep count, and clears them when `reset` is non-zero (synchronises). */
 apbf_status apbf_scene_set_error_measurement(void* scene, int32_t enable);
 apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* max_err,
-                                   double* last_mean, uint32_t* substeps, int32_t reset);
+                                   double* last_mean, double* mean_abs_of_means, uint32_t* substeps,
+                                   int32_t reset);
 /* Sums over substeps of the number of particles the solver ran on (the unit of
  * the particle-substeps/s metric) and of the neighbour pairs, kept on the
  * device; cleared when `reset` is non-zero. Synchronises. */
 apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t* pairs_total,
                                          int32_t reset);
 /* Time every launch of one kernel with HIP events on the scene stream:
- * kernel_id 0 k_solve_a, 1 k_solve_b, 2 neighbour count walk, 3 neighbour fill
+ * kernel_id 0 solver row pass, 1 solver apply pass, 2 neighbour count walk, 3 neighbour fill
  * walk, 4 kernel-width spreading, 5 boundary-distance flood fill, 6 particle
  * permutation; -1 stops timing. At most max_launches launches are recorded.
  * apbf_scene_kernel_time returns the summed duration and the launch count of

@@ -11,6 +11,14 @@
  *
  * Only tests/, the smoke check and the CPU-baseline leg of the benchmark load
  * this library; the product never does.
+ *
+ * Pinning: list algorithms against the reference's test.cpp vectors; hash,
+ * codes, ranges and neighbour sets against closed forms and brute force;
+ * exp/pow against libm; the substep stages by invariants and closed forms
+ * (tests/test_oracle_invariants.py). Parity unpinned: the flood-fill distance
+ * and nearest-neighbour choice inside find_split_and_merge_1/2 (checked only
+ * through the legality of the resulting claims), and everything that depends
+ * on GLSL built-in rounding of the real driver.
  */
 #ifndef APBF_ORACLE_H
 #define APBF_ORACLE_H

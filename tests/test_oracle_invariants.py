@@ -1,0 +1,288 @@
+"""Pins of the oracle's substep stages by what the mathematics fixes: closed
+forms of the elementwise stages, invariants of the incompressibility solve
+(translation, mirror, isolated particle, momentum), the spreading filter
+against its predicate, conservation in particle transfer, and box collision
+containment, and the legality of the split/merge claims. CPU only."""
+import numpy as np
+
+from apbf_amd import config as cfg
+from oracle.oracle import OracleScene
+
+F = np.float32
+RES = F(262144.0)
+
+
+def _compressed_block(n_side=8, squeeze=0.85, seed=0):
+    p, st = cfg.dam_break(n_side=n_side, seed=seed)
+    pos = st["position"].astype(np.int64)
+    pos[:, :3] = (pos[:, :3] * squeeze).astype(np.int64)
+    st["position"] = pos.astype(np.int32)
+    st["pos_backup"] = st["position"].copy()
+    return p, st
+
+
+def _run(p, st, stages, dt=1.0 / 120.0):
+    o = OracleScene(p)
+    o.set_state(st)
+    before = o.get_state()
+    for s in stages:
+        o.stage(s, dt)
+    return o, before, o.get_state()
+
+
+def test_incompressibility_translation_invariance():
+    """Integer translation of every position (and of the hash domain) gives the
+    same pair lists, lambdas and position deltas bit for bit
+    (incompressibility_1.comp:51 uses integer differences only)."""
+    p, st = _compressed_block()
+    o1, b1, a1 = _run(p, st, ["neighborhood", "incompressibility"])
+    t = np.array([3 * 262144, 262144 + 12345, -2 * 262144 - 77], np.int64)
+    st2 = dict(st)
+    st2["position"] = st["position"].copy()
+    st2["position"][:, :3] = (st["position"][:, :3].astype(np.int64) + t).astype(np.int32)
+    st2["pos_backup"] = st2["position"].copy()
+    p2 = dict(p)
+    tw = (t / 262144.0).astype(np.float64)
+    p2["hash_min"] = tuple(float(F(a) + F(b)) for a, b in zip(p["hash_min"], tw))
+    p2["hash_max"] = tuple(float(F(a) + F(b)) for a, b in zip(p["hash_max"], tw))
+    o2, b2, a2 = _run(p2, st2, ["neighborhood", "incompressibility"])
+    assert np.array_equal(a1["pairs"], a2["pairs"])
+    assert np.array_equal(o1.lambdas(), o2.lambdas())
+    d1 = a1["position"][:, :3].astype(np.int64) - b1["position"][_order_after(b1, a1)][:, :3]
+    d2 = a2["position"][:, :3].astype(np.int64) - b2["position"][_order_after(b2, a2)][:, :3]
+    assert np.array_equal(d1, d2)
+    assert np.abs(d1).max() > 0  # the block is compressed: the solve moves particles
+    assert np.array_equal(a1["boundariness"], a2["boundariness"])
+
+
+def _order_after(before, after):
+    """Index into `before` for each particle of `after` (the neighbourhood stage
+    permutes particles; velocity is untouched by the solve and unique here)."""
+    vb = before["pos_backup"][:, :3].astype(np.int64)
+    va = after["pos_backup"][:, :3].astype(np.int64)
+    ib = np.lexsort((vb[:, 2], vb[:, 1], vb[:, 0]))
+    ia = np.lexsort((va[:, 2], va[:, 1], va[:, 0]))
+    out = np.empty(len(va), np.int64)
+    out[ia] = ib
+    assert np.array_equal(vb[out], va)
+    return out
+
+
+def test_incompressibility_mirror_symmetry():
+    """Negating x of every position gives the same pair set, the same lambdas
+    per particle and x-negated deltas (truncation toward zero is symmetric)."""
+    p, st = _compressed_block(seed=3)
+    o1, b1, a1 = _run(p, st, ["neighborhood", "incompressibility"])
+    st2 = dict(st)
+    st2["position"] = st["position"].copy()
+    st2["position"][:, 0] = -st["position"][:, 0]
+    st2["pos_backup"] = st2["position"].copy()
+    p2 = dict(p)
+    p2["hash_min"] = (-p["hash_max"][0], p["hash_min"][1], p["hash_min"][2])
+    p2["hash_max"] = (-p["hash_min"][0], p["hash_max"][1], p["hash_max"][2])
+    o2, b2, a2 = _run(p2, st2, ["neighborhood", "incompressibility"])
+    # match particles: mirror the second state's backups back
+    k1 = a1["pos_backup"][:, :3].astype(np.int64)
+    k2 = a2["pos_backup"][:, :3].astype(np.int64) * np.array([-1, 1, 1])
+    i1 = np.lexsort((k1[:, 2], k1[:, 1], k1[:, 0]))
+    i2 = np.lexsort((k2[:, 2], k2[:, 1], k2[:, 0]))
+    assert np.array_equal(k1[i1], k2[i2])
+    m = np.empty(len(i1), np.int64)  # particle m[a] of run 2 is particle a of run 1
+    m[i1] = i2
+    assert np.array_equal(o1.lambdas(), o2.lambdas()[m])
+    d1 = a1["position"][:, :3].astype(np.int64) - a1["pos_backup"][:, :3]
+    d2 = (a2["position"][:, :3].astype(np.int64) - a2["pos_backup"][:, :3])[m] * np.array([-1, 1, 1])
+    assert np.array_equal(d1, d2)
+    s1 = set(map(tuple, a1["pairs"].tolist()))
+    inv = np.empty_like(m)
+    inv[m] = np.arange(len(m))
+    s2 = set((int(inv[i]), int(inv[j])) for i, j in a2["pairs"].tolist())
+    assert s1 == s2
+
+
+def test_isolated_particle_lambda_closed_form():
+    """A particle without neighbours: density = uint(W(0)/m^-1 * 2^18) / 2^18,
+    G = S = 0, lambda = under / (rho0^-2 * 0.01) / ((2 r_min)^D / rho0^-1 * m^-1)
+    (incompressibility_2.comp:77-99), with W(0) = c = 0.6 / (kw/2)^3 for the
+    Gaussian (kernels.glsl:84-89, 104)."""
+    p, st = cfg.dam_break(n_side=4, spacing=0.5)
+    pos = st["position"].astype(np.int64)
+    pos[:, :3] = pos[:, :3] * 20  # spacing 10 h: no neighbours
+    st["position"] = pos.astype(np.int32)
+    st["pos_backup"] = st["position"].copy()
+    p["hash_max"] = tuple(x * 20 for x in p["hash_max"])
+    p["boxes"] = []
+    o, b, a = _run(p, st, ["neighborhood", "incompressibility"])
+    assert len(a["pairs"]) == 0
+    lam = o.lambdas()
+    r = a["radius"].astype(np.float64)
+    im = a["inverse_mass"].astype(np.float64)
+    kw = a["kernel_width"].astype(np.float64)
+    c = 0.6 / (kw / 2) ** 3
+    dens = np.floor(c / im * 262144.0) / 262144.0
+    ird = (2 * r) ** 3 * im
+    under = 1 - dens * ird
+    rmin = p["settings"]["mSmallestTargetRadius"]
+    want = under / (ird * ird * 0.01) / ((2 * rmin) ** 3 / ird * im)
+    assert np.allclose(lam, want, rtol=2e-5, atol=0)
+    # an isolated particle is under-pressured: lambda >= 0, so nothing moves
+    assert np.all(lam > 0)
+    assert np.array_equal(a["position"], a["pos_backup"])
+
+
+def test_incompressibility_conserves_momentum():
+    """With rest density 1 (m^-1 = 1/(2r)^3), the mass-weighted position deltas
+    of one constraint cancel: (1/m_i^-1) dx_i + sum_j (1/m_j^-1) dx_j = 0 up
+    to one fixed-point unit of truncation per shifted pair and component."""
+    p, st = _compressed_block(n_side=9, squeeze=0.8, seed=5)
+    o, b, a = _run(p, st, ["neighborhood", "incompressibility"])
+    d = a["position"][:, :3].astype(np.int64) - a["pos_backup"][:, :3]
+    mass = 1.0 / a["inverse_mass"].astype(np.float64)
+    total = (d * mass[:, None]).sum(0)
+    lam = o.lambdas()
+    rows = np.bincount(a["pairs"][:, 0], minlength=len(lam)) if len(a["pairs"]) else np.zeros(len(lam))
+    shifted_pairs = rows[lam < 0].sum() + (lam < 0).sum()
+    bound = 2.0 * shifted_pairs * mass.max()
+    moved = (np.abs(d) * mass[:, None]).sum()
+    assert moved > 50 * bound  # the solve moved particles far more than the truncation bound
+    assert np.all(np.abs(total) <= bound), (total, bound)
+
+
+def test_spreading_filter_and_widths():
+    """spread_kernel_width (kernel_width.comp:32-60): the kept list is the raw
+    list filtered by d <= max(kw0_i, kw_old_i) in raw order, and the new width
+    is the multiplicative step toward max(kw0_j, max_i kw0_i * infl_ij)."""
+    p, st = cfg.small_water_drop(seed=7)
+    o = OracleScene(p)
+    o.set_state(st)
+    o.stage("neighborhood", p["dt"])
+    raw = o.get_state()
+    o.stage("spread", p["dt"])
+    out = o.get_state()
+    pos = raw["position"][:, :3]
+    i, j = raw["pairs"][:, 0].astype(np.int64), raw["pairs"][:, 1].astype(np.int64)
+    diff = (pos[j].astype(np.int64) - pos[i]).astype(np.int32).astype(F) / RES
+    d2 = (diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2]
+    dist = np.sqrt(d2.astype(F))
+    s = p["settings"]
+    tr = raw["target_radius"] if s["mBaseKernelWidthOnTargetRadius"] else np.zeros_like(raw["radius"])
+    kw0 = (np.maximum(raw["radius"], tr) * F(4.0)).astype(F)
+    keep = dist <= np.maximum(kw0[i], raw["kernel_width"][i])
+    assert np.array_equal(out["pairs"], raw["pairs"][keep])
+    # spread values
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = ((dist - kw0[i]).astype(F) / (kw0[i] * F(0.5))).astype(F)
+    infl = np.maximum(F(0), F(1) - np.maximum(F(0), np.nan_to_num(q, nan=0.0))).astype(F)
+    val = np.floor((kw0[i] * infl).astype(F) * F(262144.0)).astype(np.uint64)
+    ukw = np.floor(kw0 * F(262144.0)).astype(np.uint64)
+    np.maximum.at(ukw, j, val)
+    target = (ukw.astype(F) * F(1.0 / 262144.0)).astype(F)
+    old = raw["kernel_width"]
+    step = F(s["mKernelWidthAdaptionSpeed"])
+    up = (old * (F(1) + step)).astype(F)
+    down = (old * (F(1) - step)).astype(F)
+    want = np.where(target > old, np.minimum(up, target), np.maximum(down, target)).astype(F)
+    assert np.array_equal(out["kernel_width"], want)
+
+
+def test_particle_transfer_conserves_mass_and_volume():
+    """particle_transfer.comp:37-84 moves mass and volume r^D from source to
+    target: totals over all particles are conserved up to float rounding."""
+    p, st = cfg.small_water_drop(seed=9, active=True)
+    o = OracleScene(p)
+    o.set_state(st)
+    for _ in range(6):
+        o.update(p["dt"])
+        if o.counts()[2] > 0:
+            break
+    before = o.get_state()
+    assert len(before["t_source"]) > 0
+    o.stage("particle_transfer", p["dt"])
+    after = o.get_state()
+    m0 = (1.0 / before["inverse_mass"].astype(np.float64)).sum()
+    m1 = (1.0 / after["inverse_mass"].astype(np.float64)).sum()
+    v0 = (before["radius"].astype(np.float64) ** 3).sum()
+    v1 = (after["radius"].astype(np.float64) ** 3).sum()
+    assert abs(m1 - m0) <= 1e-5 * m0
+    assert abs(v1 - v0) <= 1e-5 * v0
+    assert not np.allclose(before["radius"][: len(after["radius"])], after["radius"])
+
+
+def test_velocity_handling_closed_form():
+    """velocity_handling.cpp:15-31: v = (x - x_prev) / (dt_last 2^18); x_prev = x;
+    v += g dt; x += ivec3(v (dt 2^18)), truncated."""
+    p, st = cfg.dam_break(n_side=5, seed=1)
+    rng = np.random.default_rng(2)
+    st["pos_backup"] = st["position"].copy()
+    st["pos_backup"][:, :3] -= rng.integers(-3000, 3000, (len(st["radius"]), 3)).astype(np.int32)
+    st["last_delta_time"] = 0.01
+    dt = F(1.0 / 120.0)
+    o, b, a = _run(p, st, ["velocity"], dt=float(dt))
+    x = b["position"][:, :3].astype(F)
+    xp = b["pos_backup"][:, :3].astype(F)
+    v = ((x - xp) / (F(0.01) * RES)).astype(F)
+    v = (v + np.array(p["gravity"], F) * dt).astype(F)
+    step = (v * (dt * RES)).astype(F)
+    want = b["position"][:, :3].astype(np.int64) + np.trunc(step).astype(np.int64)
+    assert np.array_equal(a["position"][:, :3].astype(np.int64), want)
+    assert np.array_equal(a["pos_backup"], b["position"])
+    assert np.array_equal(a["velocity"][:, :3], v)
+
+
+def test_box_collision_pushes_out_of_walls():
+    """box_collision.comp:35-60: after the stage no particle centre lies inside
+    a wall box padded by its radius; particles far from every wall keep their
+    position exactly."""
+    p, st = cfg.dam_break(n_side=6, seed=4)
+    pos = st["position"].copy()
+    rng = np.random.default_rng(3)
+    lo = np.array(p["hash_min"], np.float64)
+    hi = np.array(p["hash_max"], np.float64)
+    w = lo + rng.uniform(-0.4, 1.2, (len(pos), 3)) * (hi - lo)  # some outside the interior
+    pos[:, :3] = np.trunc(w * 262144.0).astype(np.int32)
+    st["position"] = pos
+    st["pos_backup"] = pos.copy()
+    o, b, a = _run(p, st, ["box"])
+    r = a["radius"].astype(np.float64)
+    xa = a["position"][:, :3] / 262144.0
+    xb = b["position"][:, :3] / 262144.0
+    inside_any = np.zeros(len(r), bool)
+    near_any = np.zeros(len(r), bool)
+    for bmin, bmax in p["boxes"]:
+        bmin, bmax = np.array(bmin), np.array(bmax)
+        inside = np.all((xa > bmin - r[:, None] + 1e-5) & (xa < bmax + r[:, None] - 1e-5), axis=1)
+        inside_any |= inside
+        near_any |= np.all((xb > bmin - r[:, None] - 0.1) & (xb < bmax + r[:, None] + 0.1), axis=1)
+    # boxes are pushed out of in order, so a corner particle may end inside an
+    # earlier box; every particle that touched at most one box must be clear
+    touched = np.zeros(len(r), int)
+    for bmin, bmax in p["boxes"]:
+        bmin, bmax = np.array(bmin), np.array(bmax)
+        touched += np.all((xb > bmin - r[:, None] - 0.1) & (xb < bmax + r[:, None] + 0.1), axis=1)
+    assert not np.any(inside_any & (touched <= 1))
+    assert np.array_equal(a["position"][~near_any], b["position"][~near_any])
+    assert near_any.sum() > 10 and (~near_any).sum() > 10
+
+
+def test_update_transfers_is_a_legal_schedule():
+    """find_split_and_merge_3.comp:96-121 claims source and target with
+    atomicExchange(transferring): whatever the schedule, no particle takes part
+    in two transfers, exactly the claimed particles are flagged, merges have a
+    positive time left and splits a non-positive one, and every split particle
+    satisfies the split predicate r_t 2^(1/D) 0.99 <= r at decision time."""
+    p, st = cfg.small_water_drop(seed=11, active=True)
+    o = OracleScene(p)
+    o.set_state(st)
+    seen_merge = seen_split = False
+    for _ in range(8):
+        o.update(p["dt"])
+        s = o.get_state()
+        src, tgt, ttl = s["t_source"], s["t_target"], s["t_time_left"]
+        ids = np.concatenate([src, tgt])
+        assert len(np.unique(ids)) == len(ids)
+        flagged = np.nonzero(s["transferring"])[0]
+        assert np.array_equal(np.sort(flagged), np.sort(ids))
+        merge = ttl > 0
+        seen_merge |= bool(merge.any())
+        seen_split |= bool((~merge).any())
+    assert seen_merge and seen_split
