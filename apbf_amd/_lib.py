@@ -105,6 +105,7 @@ PROTOTYPES = {
     "apbf_scene_overflow_flags": (_i32, [_vp, _pu]),
     "apbf_scene_set_error_measurement": (_i32, [_vp, _i32]),
     "apbf_scene_error_stats": (_i32, [_vp, _pd, _pd, _pd, _pd, _pu, _i32]),
+    "apbf_scene_iteration_errors": (_i32, [_vp, _vp, _u32, _i32]),
     "apbf_scene_particle_substeps": (_i32, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _i32]),
     "apbf_scene_time_kernel": (_i32, [_vp, _i32, _u32]),
     "apbf_scene_diagnostics": (_i32, [_vp, _vp, _u32]),
@@ -124,6 +125,15 @@ PROTOTYPES = {
     "apbf_apply_hidden_edit": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
                                       _sz, _vp]),
     "apbf_delete_edit_list": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "apbf_mlp_create": (_i32, [C.POINTER(_vp)]),
+    "apbf_mlp_destroy": (_i32, [_vp]),
+    "apbf_mlp_set_params": (_i32, [_vp, _vp, _vp]),
+    "apbf_mlp_get_params": (_i32, [_vp, _vp, _vp]),
+    "apbf_mlp_forward": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "apbf_mlp_train": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _u32, _f32, _vp, _vp]),
+    "apbf_mlp_gradient": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "apbf_scene_set_warm_start": (_i32, [_vp, _vp, _i32]),
+    "apbf_scene_harvest": (_i32, [_vp, _vp, _vp, _u32, _pu]),
     "apbf_version": (C.c_char_p, []),
     "apbf_device_count": (_i32, []),
 }

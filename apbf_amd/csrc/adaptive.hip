@@ -117,6 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_particles(const uint32_t* __r
         d.kernel_width[o] = s.kernel_width[i];
         d.boundariness[o] = s.boundariness[i];
         d.boundary_distance[o] = s.boundary_distance[i];
+        if (s.lam_prev) d.lam_prev[o] = s.lam_prev[i];
     }
 }
 
@@ -470,6 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_split_finish(const uint32_t* __restri
         a.kernel_width[d] = a.kernel_width[s];
         a.boundariness[d] = a.boundariness[s];
         a.boundary_distance[d] = a.boundary_distance[s];
+        if (a.lam_prev) a.lam_prev[d] = a.lam_prev[s];
         p.x = iadd(p.x, to_i32(r * POS_RES * 0.1f));
         a.pos[d] = p;
         a.radius[d] = 0.0f;
@@ -675,6 +677,7 @@ apbf_status Scene::set_state(uint32_t n, const int32_t* h_pos, const float* h_ve
     HIP_OK(hipMemcpy(kernel_width[cur], h_kw, (size_t)n * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(boundariness[cur], h_b, (size_t)n * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(boundary_distance[cur], h_bd, (size_t)n * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(lam_prev[cur], 0, (size_t)cap * 4));
     if (nt) {
         HIP_OK(hipMemcpy(t_source[tcur], ts.data(), (size_t)nt * 4, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(t_target[tcur], tt.data(), (size_t)nt * 4, hipMemcpyHostToDevice));

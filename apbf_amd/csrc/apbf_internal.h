@@ -56,6 +56,7 @@ void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, const uint32_t
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void fill_u32_dev(uint32_t* p, uint32_t value, const uint32_t* d_len, uint32_t capacity,
                   hipStream_t s);
+void copy_u32_dev(uint32_t* dst, const uint32_t* src, const uint32_t* d_len, uint32_t capacity, hipStream_t s);
 void iota_u32(uint32_t* p, const uint32_t* d_len, uint32_t capacity, hipStream_t s);
 void gather_words(const uint32_t* src, uint32_t* dst, const uint32_t* edit,
                   const uint32_t* d_len, uint32_t capacity, uint32_t words, hipStream_t s);

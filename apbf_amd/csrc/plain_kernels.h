@@ -17,6 +17,7 @@ struct ParticleArrays {
     float* kernel_width;
     float* boundariness;
     uint32_t* boundary_distance;
+    float* lam_prev;     // total lambda of the previous substep (warm-start feature); may be null
     uint32_t* prev_old;  // optional: receives the source slot of every gathered particle
 };
 
@@ -54,6 +55,10 @@ void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radi
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
                        float* lambda, int4* acc, hipStream_t s);
+void launch_features_rows(const int4* pos, const float4* vel, const float* inv_mass, const float* radius,
+                          const float* kw, const float4* gc, const float* boundariness, const float* lam_prev,
+                          const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                          uint32_t pcap, const SolverArgs& a, float dt, float* feat, hipStream_t s);
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
                         const BoxSet& boxes, hipStream_t s);
 uint32_t density_rows_blocks(uint32_t cap);

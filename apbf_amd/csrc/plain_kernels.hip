@@ -107,6 +107,7 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const uint32_t* __restrict__ 
         dst.kernel_width[i] = src.kernel_width[o];
         dst.boundariness[i] = src.boundariness[o];
         dst.boundary_distance[i] = src.boundary_distance[o];
+        if (src.lam_prev) dst.lam_prev[i] = src.lam_prev[o];
         if (src.prev_old) dst.prev_old[i] = o;
     }
 }

@@ -276,6 +276,13 @@ __global__ void k_fill_dev(uint32_t* __restrict__ p, uint32_t v, const uint32_t*
          i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+__global__ void k_copy_dev(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                           const uint32_t* __restrict__ d_len) {
+    const uint32_t n = *d_len;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 __global__ void k_iota(uint32_t* __restrict__ p, const uint32_t* __restrict__ d_len) {
     const uint32_t n = *d_len;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -302,6 +309,9 @@ void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
 }
 void fill_u32_dev(uint32_t* p, uint32_t value, const uint32_t* d_len, uint32_t capacity, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_dev, dim3(stride_grid(capacity)), dim3(BLOCK), 0, s, p, value, d_len);
+}
+void copy_u32_dev(uint32_t* dst, const uint32_t* src, const uint32_t* d_len, uint32_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_dev, dim3(stride_grid(capacity)), dim3(BLOCK), 0, s, dst, src, d_len);
 }
 void iota_u32(uint32_t* p, const uint32_t* d_len, uint32_t capacity, hipStream_t s) {
     hipLaunchKernelGGL(k_iota, dim3(stride_grid(capacity)), dim3(BLOCK), 0, s, p, d_len);

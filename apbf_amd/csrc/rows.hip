@@ -88,6 +88,9 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                                                       const uint32_t* __restrict__ col, uint32_t pcap,
                                                       SolverArgs a, float* __restrict__ boundariness,
                                                       float* __restrict__ lambda, int* __restrict__ acc) {
+    __shared__ double s_err;
+    if (threadIdx.x == 0) s_err = 0.0;
+    __syncthreads();
     ROW_LOOP_BEGIN(L)
     int4 pi = make_int4(0, 0, 0, 0);
     float im = 1.0f, kwi = 1.0f, ird = 1.0f;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     sgs += wgss * im;
     const float under = 1.0f - densf * ird;
     float b = active ? boundariness[i] : 0.0f;
-    if (a.update_b) {
+    if (a.update_b && !a.lam_override) {
         float bb;
         if (a.method == 1) {
             const float totalMass = (float)(int32_t)m3 / INCOMP_RES + 1.0f / im;
@@ -200,7 +203,10 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     }
     float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
     lam /= a.lambda_scale_base / ird * im;
+    if (a.lam_override) lam = active ? a.lam_override[i] : 0.0f;
     if (active && gl == 0) {
+        if (a.it_err) atomicAdd(&s_err, (double)fmax_g(0.0f, densf * ird - 1.0f));
+        if (a.lam_acc) a.lam_acc[i] += fminf(lam, 0.0f);
         lambda[i] = lam;
         if (!(lam >= 0.0f)) {  // own shift; a NaN lambda shifts by ivec3(NaN) = 0
             const v3f d = selfGrad * (lam * im * POS_RES);
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     // filter (:41-46), over this lane's share of row i, with the gradients of
     // the first loop (positions are unchanged in between)
     const bool do_shift = active && lam < 0.0f;
-    const bool do_filter = active && a.method == 2 && __builtin_bit_cast(uint32_t, b) != 0u;
+    const bool do_filter = active && a.method == 2 && !a.lam_override && __builtin_bit_cast(uint32_t, b) != 0u;
     bool hit = false;
     if (do_shift || do_filter) {
         const float sc = lam * POS_RES;
@@ -247,6 +253,108 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     const uint64_t ballot = __ballot(hit);
     if (ballot & group_mask<L>()) b = 0.0f;
     if (active && gl == 0) boundariness[i] = b;
+    ROW_LOOP_END
+    if (a.it_err) {  // the grid-stride trip count is uniform per block
+        __syncthreads();
+        if (threadIdx.x == 0 && s_err != 0.0) atomicAdd(a.it_err, s_err / (double)counts[C_N]);
+    }
+}
+
+// Warm-start features of particle i (16 floats, learned lambda warm start):
+// the sums of incompressibility_1 over row i and the quantities
+// incompressibility_2 derives from them (density ratio, self-gradient length,
+// lambda of a cold start), plus neighbour count, previous total lambda, radius
+// ratio to the neighbours, kernel width / radius, boundariness, rest-density
+// factor, log radius, mean neighbour distance / kernel width and the distance
+// moved per substep / radius.
+template <int L, int KH, int KG>
+__global__ __launch_bounds__(BLOCK) void k_features_rows(const int4* __restrict__ pos,
+                                                         const float4* __restrict__ vel,
+                                                         const float* __restrict__ inv_mass,
+                                                         const float* __restrict__ radius,
+                                                         const float* __restrict__ kw,
+                                                         const float4* __restrict__ gc,
+                                                         const float* __restrict__ boundariness,
+                                                         const float* __restrict__ lam_prev,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ row,
+                                                         const uint32_t* __restrict__ col, uint32_t pcap,
+                                                         SolverArgs a, float dt, float* __restrict__ feat) {
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    float im = 1.0f, kwi = 1.0f, ird = 1.0f;
+    uint32_t self = 0;
+    GaussConst g{1.0f, 1.0f};
+    uint32_t rb = 0, re = 0;
+    if (active) {
+        pi = pos[i];
+        im = inv_mass[i];
+        kwi = kw[i];
+        const float4 g4 = gc[i];
+        g = GaussConst{g4.x, g4.y};
+        ird = g4.z;
+        self = __builtin_bit_cast(uint32_t, g4.w);
+        rb = row[i];
+        re = min(row[i + 1], pcap);
+    }
+    uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0;
+    float rsum = 0.0f, dsum = 0.0f;
+    for (uint32_t k = rb + gl; k < re; k += L) {
+        const uint32_t j = col[k];
+        const int4 pj = pos[j];
+        const float imN = inv_mass[j];
+        const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+        float w;
+        v3f grad;
+        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+        const v3f wg = grad / imN;
+        dens += to_u32(w / imN * INCOMP_RES);
+        s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
+        s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
+        s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
+        sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
+        rsum += radius[j];
+        dsum += length(diff);
+    }
+    dens = gsum<L>(dens) + self;
+    sq = gsum<L>(sq);
+    s0 = gsum<L>(s0);
+    s1 = gsum<L>(s1);
+    s2 = gsum<L>(s2);
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) {
+        rsum += __shfl_xor(rsum, m, 64);
+        dsum += __shfl_xor(dsum, m, 64);
+    }
+    if (active && gl == 0) {
+        const float c = (float)(re > rb ? re - rb : 0u);
+        const float densf = (float)dens / INCOMP_RES;
+        const v3f wgs = vec_i((int32_t)s0, (int32_t)s1, (int32_t)s2) / INCOMP_RES;
+        const float wgss = dot(wgs, wgs);
+        const float sgs = (float)sq / INCOMP_RES + wgss * im;
+        const float under = 1.0f - densf * ird;
+        float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
+        lam /= a.lambda_scale_base / ird * im;
+        const float r = radius[i];
+        const float4 v = vel[i];
+        float* f = feat + (size_t)i * 16;
+        f[0] = c * (1.0f / 64.0f);
+        f[1] = densf * ird;
+        f[2] = under;
+        f[3] = sqrtf(wgss) * ird;
+        f[4] = (float)sq / INCOMP_RES * ird * ird;
+        f[5] = ird * ird * (sgs + INCOMP_EPSILON);
+        f[6] = lam;
+        f[7] = fminf(lam, 0.0f);
+        f[8] = lam_prev[i];
+        f[9] = c > 0.0f ? r * c / rsum : 1.0f;
+        f[10] = kwi / r;
+        f[11] = boundariness[i];
+        f[12] = ird;
+        f[13] = log2f(r);
+        f[14] = c > 0.0f ? dsum / (c * kwi) : 0.0f;
+        f[15] = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z) * dt / r;
+    }
     ROW_LOOP_END
 }
 
@@ -486,6 +594,18 @@ void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radi
         hipLaunchKernelGGL((k_solve_rows<ROW_LANES, -1, -1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
                            inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
 }
+void launch_features_rows(const int4* pos, const float4* vel, const float* inv_mass, const float* radius,
+                          const float* kw, const float4* gc, const float* boundariness, const float* lam_prev,
+                          const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                          uint32_t pcap, const SolverArgs& a, float dt, float* feat, hipStream_t s) {
+    if (a.hid == 1 && a.gid == 1)
+        hipLaunchKernelGGL((k_features_rows<ROW_LANES, 1, 1>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, vel,
+                           inv_mass, radius, kw, gc, boundariness, lam_prev, counts, row, col, pcap, a, dt, feat);
+    else
+        hipLaunchKernelGGL((k_features_rows<ROW_LANES, -1, -1>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos, vel,
+                           inv_mass, radius, kw, gc, boundariness, lam_prev, counts, row, col, pcap, a, dt, feat);
+}
+
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
                         const BoxSet& boxes, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_apply, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, pos, acc, radius, counts, boxes);

@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 
+#include "mlp.h"
 #include "plain_kernels.h"
 
 namespace apbf {
@@ -18,6 +19,8 @@ enum KernelId : int {
     K_COUNT_IDS = 7
 };
 
+constexpr int MAX_ITER_RECORD = 64;  // solver iterations with a recorded density error
+
 struct Scene {
     apbf_scene_params P{};
     hipStream_t stream = nullptr;
@@ -32,6 +35,13 @@ struct Scene {
     bool measure_error = false;  // accumulate the density error after every solver loop
     double* err_stats = nullptr; // [sum of means, max, last mean, last max, sum of abs means, last abs mean]
     unsigned long long* particle_substeps = nullptr;  // sum over substeps of the solver's particle count
+    double* iter_err = nullptr;  // per solver iteration: sum over substeps of the mean error it started from
+    // learned lambda warm start (mode 0 off, 1 harvest, 2 warm start)
+    int ws_mode = 0;
+    apbf_mlp* ws_model = nullptr;
+    float* feat = nullptr;     // cap x 16 features of the current substep
+    float* lam_acc = nullptr;  // per particle: sum of min(lambda, 0) over this substep's iterations
+    float* lam_hat = nullptr;  // predicted lambdas
 
     // Optional HIP-event timing of one kernel (KernelId) on the scene stream.
     int timed_kernel = -1;
@@ -57,6 +67,7 @@ struct Scene {
     float* kernel_width[2] = {};
     float* boundariness[2] = {};
     uint32_t* boundary_distance[2] = {};
+    float* lam_prev[2] = {};  // total lambda applied to the particle in the previous substep
     // hidden transfer list (list_definitions.h:15-16), two buffer sets
     uint32_t* t_source[2] = {};
     uint32_t* t_target[2] = {};
@@ -122,7 +133,7 @@ struct Scene {
     void spread_end();
     void prepare_solver();
     SolverArgs solver_args() const;
-    void incompressibility_iteration(const BoxSet& next_boxes);
+    void incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override = nullptr);
     void solver_loop();
     void remap_transfers_after_permute();
     void accumulate_density_error();

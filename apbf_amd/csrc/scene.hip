@@ -49,6 +49,7 @@ ParticleArrays Scene::arrays(int set) {
     a.kernel_width = kernel_width[set];
     a.boundariness = boundariness[set];
     a.boundary_distance = boundary_distance[set];
+    a.lam_prev = lam_prev[set];
     a.prev_old = nullptr;
     return a;
 }
@@ -89,6 +90,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
         A(kernel_width[s], cap);
         A(boundariness[s], cap);
         A(boundary_distance[s], cap);
+        A(lam_prev[s], cap);
         A(t_source[s], tcap);
         A(t_target[s], tcap);
         A(t_time_left[s], tcap);
@@ -129,6 +131,10 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(part_sum, 2 * err_blocks);
     A(part_max, err_blocks);
     A(err_stats, 6);
+    A(iter_err, MAX_ITER_RECORD);
+    A(feat, (size_t)cap * MLP_IN);
+    A(lam_acc, cap);
+    A(lam_hat, cap);
     A(particle_substeps, 2);
 #undef A
     size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
@@ -138,6 +144,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     HIP_OK(hipMalloc(&ws_mem, ws_bytes));
     HIP_OK(hipMemset(counts, 0, C_COUNT * 4));
     HIP_OK(hipMemset(err_stats, 0, 6 * sizeof(double)));
+    HIP_OK(hipMemset(iter_err, 0, MAX_ITER_RECORD * sizeof(double)));
     HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
@@ -147,12 +154,12 @@ Scene::~Scene() {
     for (int s = 0; s < 2; ++s) {
         hipFree(pos[s]); hipFree(vel[s]); hipFree(inv_mass[s]); hipFree(radius[s]);
         hipFree(backup[s]); hipFree(transferring[s]); hipFree(target_radius[s]);
-        hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]);
+        hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]); hipFree(lam_prev[s]);
         hipFree(t_source[s]); hipFree(t_target[s]); hipFree(t_time_left[s]);
     }
     void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_flag, a_scan,
-                  a_list, a_t0, t_flag, t_scan, err_stats, particle_substeps};
+                  a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps};
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
 }
@@ -232,6 +239,9 @@ SolverArgs Scene::solver_args() const {
     a.b_sgl_factor = P.s.mBoundarinessSelfGradLengthFactor;
     a.b_under_factor = P.s.mBoundarinessUnderpressureFactor;
     a.lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, DIM);
+    a.it_err = nullptr;
+    a.lam_override = nullptr;
+    a.lam_acc = nullptr;
     return a;
 }
 
@@ -241,9 +251,13 @@ void Scene::prepare_solver() {
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
 }
 
-void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
+void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override) {
     hipStream_t s = stream;
     SolverArgs a = solver_args();
+    if (measure_error && it >= 0 && it < MAX_ITER_RECORD) a.it_err = iter_err + it;
+    if (ws_mode != 0 && it >= 0) a.lam_acc = lam_acc;
+    a.lam_override = lam_override;
+    if (lam_override) a.lam_acc = lam_acc;
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
                       boundariness[cur], lambda, shift, s);
@@ -255,17 +269,32 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes) {
 
 void Scene::incompressibility() {
     prepare_solver();
-    incompressibility_iteration(no_boxes);
+    incompressibility_iteration(no_boxes, -1);
 }
 
 void Scene::solver_loop() {
     if (P.solver_iterations <= 0) return;
     prepare_solver();
     box_collision();
+    if (ws_mode != 0) {
+        // learned warm start: features of the state the solver starts from, then
+        // (mode 2) one shift pass with the predicted lambdas before the iterations
+        fill_u32_dev(reinterpret_cast<uint32_t*>(lam_acc), 0u, counts + C_N, cap, stream);
+        launch_features_rows(pos[cur], vel[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc,
+                             boundariness[cur], lam_prev[cur], counts, cap, row, col, pcap, solver_args(), last_dt,
+                             feat, stream);
+        if (ws_mode == 2 && ws_model) {
+            launch_mlp_infer(ws_model->params, ws_model->norm, feat, counts + C_N, cap, lam_hat, stream);
+            incompressibility_iteration(boxes, -1, lam_hat);
+        }
+    }
     for (int it = 0; it < P.solver_iterations; ++it) {
         bool last = it == P.solver_iterations - 1;
-        incompressibility_iteration(last ? no_boxes : boxes);
+        incompressibility_iteration(last ? no_boxes : boxes, it);
     }
+    if (ws_mode != 0)
+        copy_u32_dev(reinterpret_cast<uint32_t*>(lam_prev[cur]), reinterpret_cast<const uint32_t*>(lam_acc),
+                     counts + C_N, cap, stream);
 }
 
 // ----------------------------------------------------------------- update
@@ -471,6 +500,52 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
             hipStreamSynchronize(s->stream) != hipSuccess)
             return APBF_ERR_HIP;
     }
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_warm_start(void* scene, apbf_mlp* model, int32_t mode) {
+    if (!scene || mode < 0 || mode > 2 || (mode == 2 && !model)) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (s->ws_mode == 0 && mode != 0) {  // the lambda history starts empty
+        if (hipMemsetAsync(s->lam_prev[s->cur], 0, (size_t)s->cap * 4, s->stream) != hipSuccess) return APBF_ERR_HIP;
+    }
+    s->ws_mode = mode;
+    s->ws_model = model;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_harvest(void* scene, float* features, float* targets, uint32_t capacity, uint32_t* count) {
+    if (!scene || !count) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    uint32_t n = 0;
+    if (hipMemcpyAsync(&n, s->counts + apbf::C_N, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    *count = n;
+    if (s->ws_mode == 0) return APBF_ERR_INVALID_ARGUMENT;
+    if (n > capacity) return APBF_ERR_CAPACITY;
+    if ((features && hipMemcpyAsync(features, s->feat, (size_t)n * apbf::MLP_IN * 4, hipMemcpyDeviceToDevice,
+                                    s->stream) != hipSuccess) ||
+        (targets && hipMemcpyAsync(targets, s->lam_acc, (size_t)n * 4, hipMemcpyDeviceToDevice, s->stream) !=
+                        hipSuccess) ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_iteration_errors(void* scene, double* out, uint32_t max_iters, int32_t reset) {
+    if (!scene || (!out && max_iters)) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    double e[apbf::MAX_ITER_RECORD];
+    uint32_t n = 0;
+    if (hipMemcpyAsync(e, s->iter_err, sizeof(e), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipMemcpyAsync(&n, s->counts + apbf::C_ERR_SUBSTEPS, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    for (uint32_t k = 0; k < max_iters; ++k) out[k] = (k < apbf::MAX_ITER_RECORD && n) ? e[k] / (double)n : 0.0;
+    if (reset && (hipMemsetAsync(s->iter_err, 0, sizeof(e), s->stream) != hipSuccess ||
+                  hipStreamSynchronize(s->stream) != hipSuccess))
+        return APBF_ERR_HIP;
     return APBF_OK;
 }
 

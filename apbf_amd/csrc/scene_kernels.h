@@ -46,6 +46,9 @@ struct BoxSet {
 struct SolverArgs {
     int hid, gid, method, update_b;
     float b_speed, b_sgl_factor, b_under_factor, lambda_scale_base;
+    double* it_err;  // when set: += mean density error of the positions this iteration starts from
+    const float* lam_override;  // when set: shift with these lambdas, no boundariness update or filter
+    float* lam_acc;             // when set: += min(lambda, 0) per particle
 };
 
 // -------------------------------------------------------- shared device code

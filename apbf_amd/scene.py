@@ -132,6 +132,13 @@ class Scene:
                                              int(reset)), "error_stats")
         return m.value, x.value, l.value, a.value, n.value
 
+    def iteration_errors(self, max_iters: int = 16, reset: bool = False) -> np.ndarray:
+        """Mean density error before each solver iteration, averaged over the
+        measured substeps."""
+        out = np.zeros(max_iters, np.float64)
+        check(self._L.apbf_scene_iteration_errors(self._h, _ptr(out), max_iters, int(reset)), "iteration_errors")
+        return out
+
     def particle_substeps(self, reset: bool = False):
         """(sum of particle counts, sum of pair counts) over substeps since the last reset."""
         v, p = C.c_uint64(), C.c_uint64()

@@ -73,6 +73,60 @@ def calibrate_copy(n_bytes: int = 1 << 30) -> None:
     print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "launches": 3}), flush=True)
 
 
+def timed_run(params, state, steps: int, mode: str, model=None, iterations=None):
+    """`steps` substeps from `state` (after 2 untimed ones) with the plain solver
+    or the learned warm start; wall time, error and per-iteration errors."""
+    import torch
+    from apbf_amd import learned as Lm
+    from apbf_amd.scene import Scene
+    p = dict(params)
+    if iterations is not None:
+        p["solver_iterations"] = iterations
+    s = Scene(p)
+    s.set_state(state)
+    s.set_error_measurement(True)
+    if mode == "warm":
+        Lm.set_mode(s, model, "warm")
+    for _ in range(2):
+        s.update(p["dt"])
+    s.iteration_errors(reset=True)
+    s.error_stats(reset=True)
+    s.particle_substeps(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.update(p["dt"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    before = s.iteration_errors(max_iters=p["solver_iterations"]).tolist()
+    mean_err = s.error_stats()[0]
+    ps, _ = s.particle_substeps()
+    s.close()
+    return {"iterations": p["solver_iterations"], "warm_start": mode == "warm", "ms_per_step": el / steps * 1e3,
+            "particle_substeps_per_s": ps / el, "mean_density_error": mean_err,
+            "mean_error_before_iteration": before}
+
+
+def learned_comparison(scene, params, args, log):
+    """Harvest (feature, total-lambda) pairs from plain substeps, train the
+    warm-start network on the GPU, then run the plain solver and the warm
+    start from the same state."""
+    from apbf_amd import learned as Lm
+    model, rep = Lm.fit_warm_start(scene, params["dt"], harvest_substeps=3, epochs=6)
+    log(f"warm-start network trained on {rep['samples']} pairs in {rep['train_s']:.2f} s "
+        f"(relative rms error {rep['relative_rms_error']:.3f})")
+    st = scene.get_state(pairs=False)
+    st["fluid_index"] = st["fluid_index"][: len(st["radius"])]
+    steps = max(10, min(args.steps, 30))
+    plain = timed_run(params, st, steps, "off")
+    warm = timed_run(params, st, steps, "warm", model, iterations=args.learned_iterations)
+    log(f"plain {plain['ms_per_step']:.2f} ms err {plain['mean_density_error']:.2e}; "
+        f"learned {warm['ms_per_step']:.2f} ms err {warm['mean_density_error']:.2e}")
+    model.close()
+    return {"steps": steps, "training": rep, "plain": plain, "learned": warm,
+            "tolerance": 1e-3}
+
+
 def workload(name: str):
     from apbf_amd import config as cfg
     if name == "water_drop":
@@ -117,6 +171,11 @@ def main():
     ap.add_argument("--timed-kernel", default="solve_rows", choices=sorted(BYTE_MODEL))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--iterations", type=int, default=None,
+                    help="override the workload's solver iterations (studies of iterations to tolerance)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--learned-iterations", type=int, default=3,
+                    help="Jacobi iterations after the learned warm start")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -133,6 +192,8 @@ def main():
     from apbf_amd.scene import Scene
 
     params, state = workload(args.workload)
+    if args.iterations is not None:
+        params["solver_iterations"] = args.iterations
     dt = params["dt"]
     torch.cuda.init()
     scene = Scene(params)
@@ -145,6 +206,7 @@ def main():
         scene.update(dt)
     torch.cuda.synchronize()
     log("warmup done")
+    scene.iteration_errors(reset=True)
     scene.error_stats(reset=True)
     scene.particle_substeps(reset=True)
     scene.time_kernel(args.timed_kernel, max_launches=args.steps * (params["solver_iterations"] + 2))
@@ -161,6 +223,8 @@ def main():
     ps, pairs = scene.particle_substeps()
     k_ms, k_launches = scene.kernel_time()
     scene.time_kernel(None)
+    iters = params["solver_iterations"]
+    before_it = scene.iteration_errors(max_iters=max(iters, 1)).tolist()[:iters]
     mean_err, max_err, last_err, mean_abs, err_steps = scene.error_stats()
     n_now, nt_now, np_now = scene.counts()
     diag = scene.diagnostics()
@@ -177,6 +241,10 @@ def main():
     bytes_per_launch = per_p * mean_n + per_pair * mean_pairs
     avg_launch_s = (k_ms / 1e3) / k_launches if k_launches else float("nan")
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else None
+
+    learned = None
+    if not args.no_learned:
+        learned = learned_comparison(scene, params, args, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -213,7 +281,8 @@ def main():
             "capacity_clamped": bool(diag["overflow_flags"]),
         },
         "density_error": {"mean_of_substep_means": mean_err, "max": max_err,
-                          "mean_abs_of_substep_means": mean_abs, "substeps": err_steps},
+                          "mean_abs_of_substep_means": mean_abs, "substeps": err_steps,
+                          "mean_before_iteration": before_it},
         "roofline": {
             "bound": "hbm",
             "kernel": args.timed_kernel,
@@ -227,6 +296,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "plain_vs_learned": learned,
     }
     print(json.dumps(out), flush=True)
 

@@ -153,6 +153,13 @@ apbf_status apbf_scene_set_error_measurement(void* scene, int32_t enable);
 apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* max_err,
                                    double* last_mean, double* mean_abs_of_means, uint32_t* substeps,
                                    int32_t reset);
+/* With error measurement on, the solver also records the mean density error
+ * of the positions each iteration starts from (its own density sums, no
+ * extra pass): out[k] = mean over substeps of that error before iteration k,
+ * for k < max_iters (iterations beyond the solver's count read 0). The error
+ * after the last iteration is apbf_scene_error_stats. Synchronises; `reset`
+ * clears the record (the substep count is cleared by apbf_scene_error_stats). */
+apbf_status apbf_scene_iteration_errors(void* scene, double* out, uint32_t max_iters, int32_t reset);
 /* Sums over substeps of the number of particles the solver ran on (the unit of
  * the particle-substeps/s metric) and of the neighbour pairs, kept on the
  * device; cleared when `reset` is non-zero. Synchronises. */
@@ -263,6 +270,47 @@ apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_del
                                   uint32_t deleted_capacity, const uint32_t* d_hidden_len,
                                   uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len,
                                   void* workspace, size_t workspace_bytes, apbf_stream stream);
+
+/* ------------------------------------------------- learned lambda warm start */
+/* No counterpart in the reference (it recomputes lambda from zero every
+ * iteration and discards it, incompressibility.cpp:22-25): a 16-64-64-1 ReLU
+ * network, BF16 on the matrix cores with FP32 accumulation, that predicts
+ * from 16 per-particle features the total lambda the plain Jacobi solve
+ * applies in a substep. Parameters are a flat FP32 vector of
+ * APBF_MLP_PARAMS values (W1[64][16], b1[64], W2[64][64], b2[64], w3[64],
+ * b3) plus APBF_MLP_NORM normalisation values (feature means[16], feature
+ * scales[16], output scale). */
+#define APBF_MLP_FEATURES 16
+#define APBF_MLP_PARAMS 5313
+#define APBF_MLP_NORM 33
+typedef struct apbf_mlp apbf_mlp;
+apbf_status apbf_mlp_create(apbf_mlp** out_model);
+apbf_status apbf_mlp_destroy(apbf_mlp* model);
+apbf_status apbf_mlp_set_params(apbf_mlp* model, const float* host_params, const float* host_norm);
+apbf_status apbf_mlp_get_params(apbf_mlp* model, float* host_params, float* host_norm);
+/* out[s] = prediction for features[s * 16 .. +16), s < *d_len (device arrays). */
+apbf_status apbf_mlp_forward(apbf_mlp* model, const float* features, const uint32_t* d_len, uint32_t capacity,
+                             float* out, apbf_stream stream);
+/* `steps` Adam steps (lr, betas 0.9/0.999) on 0.5 * mean((y - target/out_scale)^2)
+ * over batches perm[(k * batch) % n ..] (device arrays, n samples); the loss of
+ * each step goes to host_losses[steps] when not NULL (synchronises). */
+apbf_status apbf_mlp_train(apbf_mlp* model, const float* features, const float* targets, const uint32_t* perm,
+                           uint32_t n, uint32_t batch, uint32_t steps, float lr, float* host_losses,
+                           apbf_stream stream);
+/* Gradient of the same loss over all n samples, without an update. */
+apbf_status apbf_mlp_gradient(apbf_mlp* model, const float* features, const float* targets, const uint32_t* perm,
+                              uint32_t n, float* host_grad, float* host_loss, apbf_stream stream);
+
+/* Scene hooks. mode 0: plain solver. mode 1 (harvest): before the solver
+ * loop the scene writes the features of every particle, and after it the
+ * total lambda applied (sum over iterations of min(lambda, 0)); read them
+ * with apbf_scene_harvest. mode 2 (warm start): the features go through
+ * `model`, and one shift pass with the predicted lambdas runs before the
+ * solver's iterations. The model must outlive its use by the scene. */
+apbf_status apbf_scene_set_warm_start(void* scene, apbf_mlp* model, int32_t mode);
+/* Copies the features (n x 16) and targets (n) of the last harvested substep
+ * to device buffers of `capacity` rows; *count receives n. Synchronises. */
+apbf_status apbf_scene_harvest(void* scene, float* features, float* targets, uint32_t capacity, uint32_t* count);
 
 /* ------------------------------------------------------------ utilities */
 /* Library version string and the device it was built for ("gfx950"). */
