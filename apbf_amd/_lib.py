@@ -125,6 +125,16 @@ PROTOTYPES = {
     "apbf_apply_hidden_edit": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
                                       _sz, _vp]),
     "apbf_delete_edit_list": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "apbf_scene_capture": (_i32, [_vp, _f32]),
+    "apbf_scene_replay": (_i32, [_vp, _u32]),
+    "apbf_batch_create": (_i32, [C.POINTER(_vp), _u32, C.POINTER(_vp)]),
+    "apbf_batch_destroy": (_i32, [_vp]),
+    "apbf_batch_update": (_i32, [_vp, _f32]),
+    "apbf_batch_capture": (_i32, [_vp, _f32]),
+    "apbf_batch_replay": (_i32, [_vp, _u32]),
+    "apbf_batch_synchronize": (_i32, [_vp]),
+    "apbf_scene_set_launch_shape": (_i32, [_vp, _i32, _i32]),
+    "apbf_scene_autotune": (_i32, [_vp, _f32, _u32, _pi, _pi, _pf]),
     "apbf_mlp_create": (_i32, [C.POINTER(_vp)]),
     "apbf_mlp_destroy": (_i32, [_vp]),
     "apbf_mlp_set_params": (_i32, [_vp, _vp, _vp]),

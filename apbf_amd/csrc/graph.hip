@@ -1,0 +1,258 @@
+// Substeps as HIP graphs, batches of independent scenes, and the launch-shape
+// autotuner.
+//
+// A substep is a fixed sequence of kernels whose sizes come from the device
+// count block, so stream capture records it once and a graph launch replays
+// it without host work. The host-side buffer roles (current particle and
+// transfer buffer sets) flip a fixed number of times per substep, so two
+// substeps always return them to their starting roles: the captured graph
+// holds two substeps and every replay advances the scene by two.
+//
+// A batch captures the substeps of many scenes into one graph: a fork event
+// lets every scene's stream join the capture, each scene records its two
+// substeps on its own stream, and join events bring them back. One graph
+// launch then steps every scene, and scenes of uneven size run side by side
+// on the device instead of one after another.
+#include <vector>
+
+#include "scene.h"
+
+using namespace apbf;
+
+namespace {
+
+struct HostRoles {
+    int cur, tcur;
+    const void *row, *col, *u0, *col_tmp;
+    bool operator==(const HostRoles& o) const {
+        return cur == o.cur && tcur == o.tcur && row == o.row && col == o.col && u0 == o.u0 && col_tmp == o.col_tmp;
+    }
+};
+HostRoles roles(const Scene* s) { return HostRoles{s->cur, s->tcur, s->row, s->col, s->u0, s->col_tmp}; }
+
+apbf_status ensure_stream(Scene* s) {
+    if (s->stream) return APBF_OK;
+    if (!s->own_stream && hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess)
+        return APBF_ERR_HIP;
+    s->stream = s->own_stream;
+    return APBF_OK;
+}
+
+void drop_graph(Scene* s) {
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+}
+
+}  // namespace
+
+namespace apbf {
+void scene_release_graph(Scene* s) {
+    drop_graph(s);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    s->own_stream = nullptr;
+}
+}  // namespace apbf
+
+struct apbf_batch {
+    std::vector<Scene*> scenes;
+    std::vector<hipStream_t> own;
+    std::vector<hipEvent_t> joins;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+apbf_status apbf_scene_capture(void* scene, float dt) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (s->timed_kernel >= 0) return APBF_ERR_INVALID_ARGUMENT;  // event timing is not captured
+    apbf_status st = ensure_stream(s);
+    if (st != APBF_OK) return st;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
+    drop_graph(s);
+    const HostRoles before = roles(s);
+    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return APBF_ERR_HIP;
+    for (int k = 0; k < 2; ++k) s->update(dt);
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(s->stream, &g) != hipSuccess || !g) return APBF_ERR_HIP;
+    if (!(roles(s) == before)) {  // cannot happen for the substep sequence; refuse rather than replay wrongly
+        (void)hipGraphDestroy(g);
+        return APBF_ERR_INVALID_ARGUMENT;
+    }
+    s->graph = g;
+    if (hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        drop_graph(s);
+        return APBF_ERR_HIP;
+    }
+    s->graph_dt = dt;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_replay(void* scene, uint32_t times) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (!s->graph_exec) return APBF_ERR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < times; ++k)
+        if (hipGraphLaunch(s->graph_exec, s->stream) != hipSuccess) return APBF_ERR_HIP;
+    s->pairs_valid = true;
+    return APBF_OK;
+}
+
+apbf_status apbf_batch_create(void* const* scenes, uint32_t n, apbf_batch** out) {
+    if (!scenes || !n || !out) return APBF_ERR_INVALID_ARGUMENT;
+    apbf_batch* b = new apbf_batch();
+    for (uint32_t k = 0; k < n; ++k) {
+        Scene* s = static_cast<Scene*>(scenes[k]);
+        if (!s) {
+            apbf_batch_destroy(b);
+            return APBF_ERR_INVALID_ARGUMENT;
+        }
+        if (!s->stream) {  // every scene gets a stream of its own so the batch can run them side by side
+            if (ensure_stream(s) != APBF_OK) {
+                apbf_batch_destroy(b);
+                return APBF_ERR_HIP;
+            }
+        }
+        b->scenes.push_back(s);
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            apbf_batch_destroy(b);
+            return APBF_ERR_HIP;
+        }
+        b->joins.push_back(e);
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        apbf_batch_destroy(b);
+        return APBF_ERR_HIP;
+    }
+    *out = b;
+    return APBF_OK;
+}
+
+apbf_status apbf_batch_destroy(apbf_batch* b) {
+    if (!b) return APBF_ERR_INVALID_ARGUMENT;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return APBF_OK;
+}
+
+// One substep of every scene, each on its own stream (no graph).
+apbf_status apbf_batch_update(apbf_batch* b, float dt) {
+    if (!b) return APBF_ERR_INVALID_ARGUMENT;
+    for (Scene* s : b->scenes) s->update(dt);
+    return hipGetLastError() == hipSuccess ? APBF_OK : APBF_ERR_HIP;
+}
+
+apbf_status apbf_batch_capture(apbf_batch* b, float dt) {
+    if (!b) return APBF_ERR_INVALID_ARGUMENT;
+    for (Scene* s : b->scenes)
+        if (s->timed_kernel >= 0 || hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_INVALID_ARGUMENT;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    std::vector<HostRoles> before;
+    for (Scene* s : b->scenes) before.push_back(roles(s));
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess ||
+        hipEventRecord(b->fork, b->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    bool ok = true;
+    for (size_t k = 0; k < b->scenes.size() && ok; ++k) {
+        Scene* s = b->scenes[k];
+        ok = hipStreamWaitEvent(s->stream, b->fork, 0) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i) s->update(dt);
+        ok = ok && hipEventRecord(b->joins[k], s->stream) == hipSuccess &&
+             hipStreamWaitEvent(b->stream, b->joins[k], 0) == hipSuccess;
+    }
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(b->stream, &g) != hipSuccess || !g || !ok) {
+        if (g) (void)hipGraphDestroy(g);
+        return APBF_ERR_HIP;
+    }
+    for (size_t k = 0; k < b->scenes.size(); ++k)
+        if (!(roles(b->scenes[k]) == before[k])) {
+            (void)hipGraphDestroy(g);
+            return APBF_ERR_INVALID_ARGUMENT;
+        }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_batch_replay(apbf_batch* b, uint32_t times) {
+    if (!b || !b->exec) return APBF_ERR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < times; ++k)
+        if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return APBF_ERR_HIP;
+    for (Scene* s : b->scenes) s->pairs_valid = true;
+    return APBF_OK;
+}
+
+apbf_status apbf_batch_synchronize(apbf_batch* b) {
+    if (!b) return APBF_ERR_INVALID_ARGUMENT;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return APBF_ERR_HIP;
+    for (Scene* s : b->scenes)
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes) {
+    if (!scene || (walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) || (solve_lanes != 8 && solve_lanes != 16))
+        return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->walk_lanes = walk_lanes;
+    s->solve_lanes = solve_lanes;
+    drop_graph(s);  // a captured graph holds the old shapes
+    return APBF_OK;
+}
+
+// Times `substeps` substeps under every launch shape (the scene advances by
+// 6 * substeps) and keeps the fastest. All shapes compute identical results.
+apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk, int32_t* best_solve,
+                                float* best_ms) {
+    if (!scene || substeps == 0) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return APBF_ERR_HIP;
+    const int walks[3] = {4, 8, 16}, solves[2] = {8, 16};
+    float best = 1e30f;
+    int bw = s->walk_lanes, bs = s->solve_lanes;
+    apbf_status st = APBF_OK;
+    for (int w : walks)
+        for (int v : solves) {
+            s->walk_lanes = w;
+            s->solve_lanes = v;
+            float ms = 0.0f;
+            if (hipEventRecord(e0, s->stream) != hipSuccess) st = APBF_ERR_HIP;
+            for (uint32_t k = 0; k < substeps; ++k) s->update(dt);
+            if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                st = APBF_ERR_HIP;
+            if (st != APBF_OK) break;
+            if (ms < best) {
+                best = ms;
+                bw = w;
+                bs = v;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->walk_lanes = bw;
+    s->solve_lanes = bs;
+    drop_graph(s);
+    if (best_walk) *best_walk = bw;
+    if (best_solve) *best_solve = bs;
+    if (best_ms) *best_ms = best / (float)substeps;
+    return st;
+}
+
+}  // extern "C"

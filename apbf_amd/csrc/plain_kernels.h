@@ -37,7 +37,8 @@ void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, ui
 void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
-                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s);
+                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
+                 int lanes = 8);
 void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s);
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
@@ -54,7 +55,7 @@ void launch_density_error_finish(const double* part_sum, const float* part_max, 
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
-                       float* lambda, int4* acc, hipStream_t s);
+                       float* lambda, int4* acc, hipStream_t s, int lanes = 16);
 void launch_features_rows(const int4* pos, const float4* vel, const float* inv_mass, const float* radius,
                           const float* kw, const float4* gc, const float* boundariness, const float* lam_prev,
                           const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,

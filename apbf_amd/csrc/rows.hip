@@ -576,8 +576,8 @@ __global__ __launch_bounds__(BLOCK) void k_spread_rows(const int4* __restrict__ 
 }
 
 // ------------------------------------------------------------ launch wrappers
-static uint32_t row_grid(uint32_t cap) {
-    const uint32_t groups = BLOCK / ROW_LANES;
+static uint32_t row_grid(uint32_t cap, uint32_t lanes = ROW_LANES) {
+    const uint32_t groups = BLOCK / lanes;
     uint32_t g = div_up(cap ? cap : 1, groups);
     return g < 8192u ? g : 8192u;
 }
@@ -585,9 +585,12 @@ static uint32_t row_grid(uint32_t cap) {
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
-                       float* lambda, int4* acc, hipStream_t s) {
+                       float* lambda, int4* acc, hipStream_t s, int lanes) {
     int* ac = reinterpret_cast<int*>(acc);
-    if (a.hid == 1 && a.gid == 1)  // Gaussian height and gradient (the reference's defaults)
+    if (a.hid == 1 && a.gid == 1 && lanes == 8)  // Gaussian height and gradient, 8 lanes per row
+        hipLaunchKernelGGL((k_solve_rows<8, 1, 1, 2 * SOLVE_CACHE>), dim3(row_grid(cap, 8)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
+    else if (a.hid == 1 && a.gid == 1)  // Gaussian height and gradient (the reference's defaults)
         hipLaunchKernelGGL((k_solve_rows<ROW_LANES, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
                            inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
     else

@@ -92,6 +92,14 @@ struct Scene {
     uint32_t *a_flag = nullptr, *a_scan = nullptr, *a_list = nullptr, *a_t0 = nullptr;
     uint32_t *t_flag = nullptr, *t_scan = nullptr;
     int claim_rounds = 6;
+    // launch shapes picked by the autotuner (all compute identical results)
+    int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8 or 16
+    int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
+    // captured substeps (two per graph: the buffer roles repeat after two)
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    float graph_dt = 0.0f;
     // density error partials
     uint32_t err_blocks = 0;
     double* part_sum = nullptr;  // 2 x err_blocks: compression-only sums, then absolute sums
@@ -138,5 +146,7 @@ struct Scene {
     void remap_transfers_after_permute();
     void accumulate_density_error();
 };
+
+void scene_release_graph(Scene* s);  // graph.hip
 
 }  // namespace apbf

@@ -151,6 +151,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
 }
 
 Scene::~Scene() {
+    scene_release_graph(this);
     for (int s = 0; s < 2; ++s) {
         hipFree(pos[s]); hipFree(vel[s]); hipFree(inv_mass[s]); hipFree(radius[s]);
         hipFree(backup[s]); hipFree(transferring[s]); hipFree(target_radius[s]);
@@ -208,14 +209,16 @@ void Scene::build_pairs(float range_scale, bool spread) {
     if (spread) spread_begin();
     bool tk = tbegin(K_WALK_COUNT);
     launch_walk(false, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
-                cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s);
+                cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s,
+                walk_lanes);
     tend(tk);
     scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     launch_pairs_total(counts, row_raw, pcap, s);
     if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
     tk = tbegin(K_WALK_FILL);
     launch_walk(true, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
-                cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s);
+                cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
+                walk_lanes);
     tend(tk);
     if (spread) spread_end();
     pairs_valid = true;
@@ -260,7 +263,7 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const 
     if (lam_override) a.lam_acc = lam_acc;
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
-                      boundariness[cur], lambda, shift, s);
+                      boundariness[cur], lambda, shift, s, solve_lanes);
     tend(tk);
     tk = tbegin(K_SOLVE_APPLY);
     launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s);

@@ -177,8 +177,8 @@ __global__ __launch_bounds__(BLOCK) void k_fix_clamped_rows(const uint32_t* __re
         row_f[k] = end;
 }
 
-static uint32_t walk_grid(uint32_t cap) {
-    const uint32_t groups = BLOCK / WALK_LANES;
+static uint32_t walk_grid(uint32_t cap, uint32_t lanes) {
+    const uint32_t groups = BLOCK / lanes;
     uint32_t g = div_up(cap ? cap : 1, groups);
     return g < 16384u ? g : 16384u;
 }
@@ -186,27 +186,38 @@ static uint32_t walk_grid(uint32_t cap) {
 void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
-                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s) {
+                 const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
+                 int lanes) {
     WalkArgs wa;
     wa.range_scale = range_scale;
     wa.g = g;
     wa.on_target = on_target;
     wa.pcap = pcap;
-    const dim3 grid(walk_grid(cap)), block(BLOCK);
-#define WALK(F, S)                                                                                                   \
-    hipLaunchKernelGGL((k_walk_rows<WALK_LANES, F, S>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, \
-                       cnt_raw, cnt_f, row_raw, row_f, col, ukw)
-    if (fill) {
-        if (spread)
-            WALK(true, true);
-        else
-            WALK(true, false);
-    } else {
-        if (spread)
-            WALK(false, true);
-        else
-            WALK(false, false);
+    if (lanes != 4 && lanes != 16) lanes = WALK_LANES;
+    const dim3 grid(walk_grid(cap, (uint32_t)lanes)), block(BLOCK);
+#define WALK(L, F, S)                                                                                                \
+    hipLaunchKernelGGL((k_walk_rows<L, F, S>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
+                       cnt_f, row_raw, row_f, col, ukw)
+#define WALK_L(L)                  \
+    if (fill) {                    \
+        if (spread)                \
+            WALK(L, true, true);   \
+        else                       \
+            WALK(L, true, false);  \
+    } else {                       \
+        if (spread)                \
+            WALK(L, false, true);  \
+        else                       \
+            WALK(L, false, false); \
     }
+    if (lanes == 4) {
+        WALK_L(4)
+    } else if (lanes == 16) {
+        WALK_L(16)
+    } else {
+        WALK_L(WALK_LANES)
+    }
+#undef WALK_L
 #undef WALK
 }
 

@@ -132,6 +132,24 @@ class Scene:
                                              int(reset)), "error_stats")
         return m.value, x.value, l.value, a.value, n.value
 
+    def capture(self, dt: float) -> None:
+        """Capture two substeps into a HIP graph (replay advances by two)."""
+        check(self._L.apbf_scene_capture(self._h, float(dt)), "capture")
+
+    def replay(self, times: int = 1) -> None:
+        check(self._L.apbf_scene_replay(self._h, int(times)), "replay")
+
+    def set_launch_shape(self, walk_lanes: int, solve_lanes: int) -> None:
+        check(self._L.apbf_scene_set_launch_shape(self._h, int(walk_lanes), int(solve_lanes)), "set_launch_shape")
+
+    def autotune(self, dt: float, substeps: int = 3):
+        """Fastest (walk lanes, solver lanes, ms per substep); the scene advances
+        by 6 * substeps."""
+        w, s, ms = C.c_int32(), C.c_int32(), C.c_float()
+        check(self._L.apbf_scene_autotune(self._h, float(dt), int(substeps), C.byref(w), C.byref(s), C.byref(ms)),
+              "autotune")
+        return w.value, s.value, ms.value
+
     def iteration_errors(self, max_iters: int = 16, reset: bool = False) -> np.ndarray:
         """Mean density error before each solver iteration, averaged over the
         measured substeps."""
@@ -171,3 +189,38 @@ class Scene:
         f = C.c_uint32()
         check(self._L.apbf_scene_overflow_flags(self._h, C.byref(f)), "overflow_flags")
         return f.value
+
+
+class Batch:
+    """Independent scenes stepped together (apbf_batch_*)."""
+
+    def __init__(self, scenes):
+        self._L = _lib.lib()
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s._h for s in self.scenes])
+        h = C.c_void_p()
+        check(self._L.apbf_batch_create(arr, len(self.scenes), C.byref(h)), "apbf_batch_create")
+        self._h = h
+
+    def update(self, dt: float) -> None:
+        check(self._L.apbf_batch_update(self._h, float(dt)), "apbf_batch_update")
+
+    def capture(self, dt: float) -> None:
+        check(self._L.apbf_batch_capture(self._h, float(dt)), "apbf_batch_capture")
+
+    def replay(self, times: int = 1) -> None:
+        check(self._L.apbf_batch_replay(self._h, int(times)), "apbf_batch_replay")
+
+    def synchronize(self) -> None:
+        check(self._L.apbf_batch_synchronize(self._h), "apbf_batch_synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.apbf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

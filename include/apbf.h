@@ -271,6 +271,35 @@ apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_del
                                   uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len,
                                   void* workspace, size_t workspace_bytes, apbf_stream stream);
 
+/* ------------------------------------------ graphs, batches, launch shapes */
+/* No counterpart in the reference, which records one command buffer with a
+ * barrier after every dispatch (shader_provider.cpp:20-47). */
+/* Captures two substeps of dt into a HIP graph (the scene's buffer roles
+ * repeat every two substeps); apbf_scene_replay launches it `times` times,
+ * advancing the scene by 2 * times substeps with results identical to
+ * apbf_scene_update. The scene gets a stream of its own if it had none.
+ * Changing the launch shape drops the graph. */
+apbf_status apbf_scene_capture(void* scene, float dt);
+apbf_status apbf_scene_replay(void* scene, uint32_t times);
+/* Many independent scenes stepped together: apbf_batch_update enqueues one
+ * substep of every scene, each on its own stream; apbf_batch_capture records
+ * two substeps of every scene into one graph (fork/join across the scenes'
+ * streams) that apbf_batch_replay launches. */
+typedef struct apbf_batch apbf_batch;
+apbf_status apbf_batch_create(void* const* scenes, uint32_t n_scenes, apbf_batch** out_batch);
+apbf_status apbf_batch_destroy(apbf_batch* batch);
+apbf_status apbf_batch_update(apbf_batch* batch, float dt);
+apbf_status apbf_batch_capture(apbf_batch* batch, float dt);
+apbf_status apbf_batch_replay(apbf_batch* batch, uint32_t times);
+apbf_status apbf_batch_synchronize(apbf_batch* batch);
+/* Launch shapes: lanes per particle in the neighbour walk (4, 8, 16) and per
+ * row in the solver (8, 16); every shape computes identical results.
+ * apbf_scene_autotune times `substeps` substeps under each shape (advancing
+ * the scene by 6 * substeps) and keeps the fastest. */
+apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes);
+apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk_lanes,
+                                int32_t* best_solve_lanes, float* best_ms_per_substep);
+
 /* ------------------------------------------------- learned lambda warm start */
 /* No counterpart in the reference (it recomputes lambda from zero every
  * iteration and discards it, incompressibility.cpp:22-25): a 16-64-64-1 ReLU

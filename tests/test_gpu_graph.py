@@ -1,0 +1,96 @@
+"""HIP-graph substeps, batches of scenes and launch shapes: every path must
+give the state of plain eager updates bit for bit."""
+import numpy as np
+import pytest
+
+from apbf_amd import config as cfg
+from statecmp import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(p, st):
+    from apbf_amd.scene import Scene
+    s = Scene(p)
+    s.set_state(st)
+    return s
+
+
+def _same(a, b, where):
+    bad = compare(a.get_state(), b.get_state(), check_pairs=True)
+    assert not bad, f"{where}: " + "; ".join(bad)
+
+
+@pytest.mark.parametrize("kind", ["dam_break", "water_drop_active"])
+def test_scene_graph_replay_matches_eager(device, kind):
+    if kind == "dam_break":
+        p, st = cfg.dam_break(n_side=12)
+    else:
+        p, st = cfg.small_water_drop(seed=2, active=True)
+    dt = p["dt"]
+    eager, graphed = _scene(p, st), _scene(p, st)
+    eager.update(dt)  # the graph is captured with last_dt == dt
+    graphed.update(dt)
+    graphed.capture(dt)
+    for r in range(4):  # eager work and synchronisation between the replays
+        eager.update(dt)
+        eager.update(dt)
+        graphed.replay(1)
+        _same(eager, graphed, f"replay {r}")
+
+
+def test_batch_graph_matches_individual_scenes(device):
+    from apbf_amd.scene import Batch
+    cases = [cfg.dam_break(n_side=10), cfg.small_water_drop(seed=1), cfg.small_water_drop(seed=3, active=True),
+             cfg.dam_break(n_side=7, seed=2)]
+    singles = [_scene(p, st) for p, st in cases]
+    batched = [_scene(p, st) for p, st in cases]
+    dt = cases[0][0]["dt"]
+    for s in singles + batched:
+        s.update(dt)
+    batch = Batch(batched)
+    batch.update(dt)  # eager batch substep
+    for s in singles:
+        s.update(dt)
+    batch.synchronize()
+    for k, (a, b) in enumerate(zip(singles, batched)):
+        _same(a, b, f"eager batch, scene {k}")
+    batch.capture(dt)
+    for r in range(3):
+        batch.replay(1)
+        batch.synchronize()
+        for s in singles:
+            s.update(dt)
+            s.update(dt)
+        for k, (a, b) in enumerate(zip(singles, batched)):
+            _same(a, b, f"batch replay {r}, scene {k}")
+    batch.close()
+
+
+@pytest.mark.parametrize("walk,solve", [(4, 8), (4, 16), (8, 8), (16, 8), (16, 16)])
+def test_launch_shapes_match_the_oracle(device, walk, solve):
+    from oracle.oracle import OracleScene
+    p, st = cfg.small_water_drop(seed=4, active=True)
+    g = _scene(p, st)
+    g.set_launch_shape(walk, solve)
+    r = OracleScene(p)
+    r.set_state(st)
+    for k in range(4):
+        g.update(p["dt"])
+        r.update(p["dt"])
+    bad = compare(g.get_state(), r.get_state(), check_pairs=True)
+    assert not bad, "; ".join(bad)
+
+
+def test_autotune_keeps_results(device):
+    p, st = cfg.small_water_drop(seed=5)
+    dt = p["dt"]
+    tuned, plain = _scene(p, st), _scene(p, st)
+    w, s, ms = tuned.autotune(dt, substeps=2)
+    assert w in (4, 8, 16) and s in (8, 16) and ms > 0
+    for _ in range(12):  # the autotuner advanced the scene by 6 x 2 substeps
+        plain.update(dt)
+    _same(tuned, plain, "after autotune")
+    tuned.update(dt)
+    plain.update(dt)
+    _same(tuned, plain, "tuned substep")
