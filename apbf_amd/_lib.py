@@ -125,6 +125,9 @@ PROTOTYPES = {
     "apbf_apply_hidden_edit": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
                                       _sz, _vp]),
     "apbf_delete_edit_list": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "apbf_scene_set_gravity": (_i32, [_vp, _pf]),
+    "apbf_scene_set_range_scale": (_i32, [_vp, _f32]),
+    "apbf_scene_set_position_range": (_i32, [_vp, _pf, _pf, _u32]),
     "apbf_scene_capture": (_i32, [_vp, _f32]),
     "apbf_scene_replay": (_i32, [_vp, _u32]),
     "apbf_batch_create": (_i32, [C.POINTER(_vp), _u32, C.POINTER(_vp)]),

@@ -49,5 +49,21 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+HOST_CHECK_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_api_check.cpp")
+HOST_CHECK = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_api_check")
+
+
+def build_host_check() -> str:
+    """The C++ driver of include/apbf.hpp used by the tests (links the library)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"), "-o", HOST_CHECK,
+           HOST_CHECK_SRC, "-L", HERE, "-lapbf_hip", "-Wl,-rpath,$ORIGIN/../../apbf_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for the host interface check")
+    return HOST_CHECK
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_host_check())

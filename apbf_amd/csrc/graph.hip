@@ -48,9 +48,13 @@ void drop_graph(Scene* s) {
 }  // namespace
 
 namespace apbf {
+void scene_drop_graph(Scene* s) { drop_graph(s); }
 void scene_release_graph(Scene* s) {
     drop_graph(s);
-    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    if (s->own_stream) {
+        if (s->stream == s->own_stream) s->stream = nullptr;
+        (void)hipStreamDestroy(s->own_stream);
+    }
     s->own_stream = nullptr;
 }
 }  // namespace apbf

@@ -29,6 +29,7 @@ struct Scene {
     GridDesc grid{};
     BoxSet boxes{}, no_boxes{};
     float last_dt = 1.0f;  // velocity_handling.h:17
+    float range_scale_override = -1.0f;  // neighborhood_green::set_range_scale; < 0: the mode's default
     int cur = 0;           // current particle buffer set
     int tcur = 0;          // current transfer buffer set
     bool pairs_valid = false;    // CSR rows refer to the current particle slots
@@ -147,6 +148,7 @@ struct Scene {
     void accumulate_density_error();
 };
 
-void scene_release_graph(Scene* s);  // graph.hip
+void scene_drop_graph(Scene* s);     // graph.hip: drops a captured graph
+void scene_release_graph(Scene* s);  // graph.hip: graph and the scene's own stream
 
 }  // namespace apbf

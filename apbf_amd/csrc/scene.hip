@@ -307,7 +307,8 @@ void Scene::update(float dt) {
     if (!P.basic_pbf && (s.mMerge || s.mSplit)) particle_transfer(dt);
     if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_bd();
     const bool spread = !P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance;
-    neighborhood_green(spread ? 1.5f : 1.0f, spread);  // spreading fused into the walk
+    const float scale = range_scale_override >= 0.0f ? range_scale_override : (spread ? 1.5f : 1.0f);
+    neighborhood_green(scale, spread);  // spreading fused into the walk
     solver_loop();
     launch_count_particle_substeps(counts, particle_substeps, stream);
     if (measure_error) accumulate_density_error();
@@ -330,7 +331,10 @@ apbf_status Scene::stage(int st, float dt) {
         case APBF_STAGE_PARTICLE_TRANSFER: particle_transfer(dt); break;
         case APBF_STAGE_KERNEL_WIDTH_BD: kernel_width_from_bd(); break;
         case APBF_STAGE_NEIGHBORHOOD:
-            neighborhood_green((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f, false);
+            neighborhood_green(range_scale_override >= 0.0f ? range_scale_override
+                                                            : (P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f
+                                                                                                                    : 1.5f,
+                               false);
             break;
         case APBF_STAGE_SPREAD: spread_kernel_width(); break;
         case APBF_STAGE_BOX: box_collision(); break;
@@ -503,6 +507,49 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
             hipStreamSynchronize(s->stream) != hipSuccess)
             return APBF_ERR_HIP;
     }
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_gravity(void* scene, const float* gravity3) {
+    if (!scene || !gravity3) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    for (int d = 0; d < 3; ++d) s->P.gravity[d] = gravity3[d];
+    scene_drop_graph(s);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_range_scale(void* scene, float scale) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->range_scale_override = scale;
+    scene_drop_graph(s);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2) {
+    if (!scene || !min3 || !max3 || res_log2 == 0 || res_log2 > 10) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    const uint32_t cells = 1u << (3u * res_log2);
+    if (cells != s->n_cells) {  // new cell tables (outside any substep)
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
+        uint32_t *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, (size_t)cells * 4) != hipSuccess || hipMalloc(&b, (size_t)cells * 4) != hipSuccess) {
+            if (a) (void)hipFree(a);
+            return APBF_ERR_HIP;
+        }
+        (void)hipFree(s->cell_start);
+        (void)hipFree(s->cell_end);
+        s->cell_start = a;
+        s->cell_end = b;
+        s->n_cells = cells;
+    }
+    for (int d = 0; d < 3; ++d) {
+        s->grid.mn[d] = s->P.hash_min[d] = min3[d];
+        s->grid.mx[d] = s->P.hash_max[d] = max3[d];
+    }
+    s->grid.res = s->P.hash_res_log2 = res_log2;
+    s->pairs_valid = false;
+    scene_drop_graph(s);
     return APBF_OK;
 }
 

@@ -271,6 +271,16 @@ apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_del
                                   uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len,
                                   void* workspace, size_t workspace_bytes, apbf_stream stream);
 
+/* Scene settings the reference's subsystems take through setters:
+ * velocity_handling::set_acceleration (velocity_handling.h:12),
+ * neighborhood_green::set_range_scale (neighborhood_green.h:12; a negative
+ * scale restores the default of pool.cpp:83: 1.5 with spreading, else 1.0)
+ * and neighborhood_green::set_position_range (neighborhood_green.h:13). Call
+ * them between substeps; they drop a captured graph. */
+apbf_status apbf_scene_set_gravity(void* scene, const float* gravity3);
+apbf_status apbf_scene_set_range_scale(void* scene, float scale);
+apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2);
+
 /* ------------------------------------------ graphs, batches, launch shapes */
 /* No counterpart in the reference, which records one command buffer with a
  * barrier after every dispatch (shader_provider.cpp:20-47). */

@@ -1,0 +1,170 @@
+// Test driver for include/apbf.hpp: runs the same substeps once through the
+// subsystem classes in the order of pool.cpp:67-106 and once through
+// scene::update, and requires identical states. Prints one line per case and
+// exits non-zero on a mismatch. Built by __graft_entry__.build().
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/apbf.hpp"
+
+using namespace apbf_pbd;
+
+static apbf_settings reference_settings() {
+    apbf_settings s{};
+    s.mHeightKernelId = 1;
+    s.mGradientKernelId = 1;
+    s.mMerge = 1;
+    s.mSplit = 1;
+    s.mBaseKernelWidthOnTargetRadius = 1;
+    s.mBaseKernelWidthOnBoundaryDistance = 0;
+    s.mUpdateTargetRadius = 1;
+    s.mUpdateBoundariness = 1;
+    s.mNeighborListSorted = 0;
+    s.mBoundarinessCalculationMethod = 2;
+    s.mBoundarinessAdaptionSpeed = 0.5f;
+    s.mKernelWidthAdaptionSpeed = 0.01f;
+    s.mBoundarinessSelfGradLengthFactor = 2.0f;
+    s.mBoundarinessUnderpressureFactor = 4.0f;
+    s.mMergeDuration = 2.0f;
+    s.mSmallestTargetRadius = 0.5f;
+    s.mTargetRadiusOffset = 0.0f;
+    s.mTargetRadiusScaleFactor = 0.05f;
+    return s;
+}
+
+// Lattice block of n^3 particles of radius r at spacing 2r (plus a jitter of
+// +-0.05 r from a fixed LCG), optionally with a second, finer block above it.
+static state make_state(int n, float r, bool fine_drop) {
+    state st;
+    uint32_t lcg = 12345u;
+    auto rnd = [&]() {
+        lcg = lcg * 1664525u + 1013904223u;
+        return (float)(lcg >> 8) / 16777216.0f - 0.5f;
+    };
+    auto add = [&](float x, float y, float z, float rad) {
+        const float p[3] = {x, y, z};
+        for (int d = 0; d < 3; ++d) st.position.push_back((int32_t)std::trunc((double)p[d] * 262144.0));
+        st.position.push_back(0);
+        for (int d = 0; d < 4; ++d) st.velocity.push_back(0.0f);
+        st.inverse_mass.push_back(1.0f / ((2 * rad) * (2 * rad) * (2 * rad)));
+        st.radius.push_back(rad);
+        st.target_radius.push_back(rad);
+        st.kernel_width.push_back(4.0f * rad);
+        st.boundariness.push_back(1.0f);
+        st.transferring.push_back(0u);
+        st.boundary_distance.push_back((uint32_t)std::trunc((double)rad * 262144.0));
+    };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k)
+                add(1.0f + r + 2 * r * i + 0.1f * r * rnd(), 1.0f + r + 2 * r * j + 0.1f * r * rnd(),
+                    1.0f + r + 2 * r * k + 0.1f * r * rnd(), r);
+    if (fine_drop)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n / 2; ++j)
+                for (int k = 0; k < n; ++k)
+                    add(1.0f + r + r * i + 0.05f * r * rnd(), 1.0f + 2 * r * n + 2 * r + r * j + 0.05f * r * rnd(),
+                        1.0f + r + r * k + 0.05f * r * rnd(), 0.5f * r);
+    st.pos_backup = st.position;
+    return st;
+}
+
+static apbf_scene_params make_params(const state& st, bool basic, float extent) {
+    apbf_scene_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.s = reference_settings();
+    p.basic_pbf = basic ? 1 : 0;
+    p.solver_iterations = 3;
+    p.split_duration = 0.0f;
+    p.gravity[1] = -9.81f;
+    for (int d = 0; d < 3; ++d) {
+        p.hash_min[d] = 0.0f;
+        p.hash_max[d] = extent;
+    }
+    p.hash_res_log2 = 4;
+    // floor and four side walls, 4 thick (pool.cpp:30-40)
+    const float lo = 0.0f, hi = extent, t = 4.0f;
+    const float boxes[5][6] = {{lo - t, lo - t, lo - t, hi + t, lo, hi + t},
+                               {lo - t, lo - t, lo - t, lo, hi + t, hi + t},
+                               {hi, lo - t, lo - t, hi + t, hi + t, hi + t},
+                               {lo - t, lo - t, lo - t, hi + t, hi + t, lo},
+                               {lo - t, lo - t, hi, hi + t, hi + t, hi + t}};
+    p.n_boxes = 5;
+    for (int b = 0; b < 5; ++b)
+        for (int d = 0; d < 3; ++d) {
+            p.box_min[b][d] = boxes[b][d];
+            p.box_max[b][d] = boxes[b][3 + d];
+        }
+    p.cap_particles = 4 * st.size();
+    p.cap_transfers = st.size();
+    p.cap_pairs = 400 * st.size();
+    return p;
+}
+
+template <class T>
+static bool same(const std::vector<T>& a, const std::vector<T>& b) {
+    return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(T)) == 0);
+}
+
+static bool same_state(const state& a, const state& b) {
+    return same(a.position, b.position) && same(a.velocity, b.velocity) && same(a.inverse_mass, b.inverse_mass) &&
+           same(a.radius, b.radius) && same(a.kernel_width, b.kernel_width) && same(a.boundariness, b.boundariness) &&
+           same(a.boundary_distance, b.boundary_distance) && same(a.target_radius, b.target_radius) &&
+           same(a.t_time_left, b.t_time_left) && a.t_source.size() == b.t_source.size();
+}
+
+static int run_case(const char* name, bool basic) {
+    const state st = make_state(10, 0.5f, !basic);
+    const apbf_scene_params p = make_params(st, basic, 16.0f);
+    const float dt = 1.0f / 120.0f;
+    scene whole(p), parts(p);
+    whole.set_state(st);
+    parts.set_state(st);
+    velocity_handling velocity;
+    particle_transfer transfer;
+    neighborhood_green neighborhood;
+    spread_kernel_width spread;
+    box_collision box;
+    incompressibility solve;
+    update_transfers transfers;
+    velocity.set_data(&parts).set_acceleration(vec3{0.0f, -9.81f, 0.0f});
+    transfer.set_data(&parts);
+    neighborhood.set_data(&parts).set_position_range(vec3{0, 0, 0}, vec3{16, 16, 16}, 4);
+    spread.set_data(&parts);
+    box.set_data(&parts);
+    solve.set_data(&parts);
+    transfers.set_data(&parts);
+    for (int step = 0; step < 6; ++step) {
+        whole.update(dt);
+        // pool::update, pool.cpp:67-106
+        velocity.apply(dt);
+        if (!basic) transfer.apply(dt);
+        neighborhood.set_range_scale(basic ? 1.0f : 1.5f).apply();
+        if (!basic) spread.apply();
+        for (int it = 0; it < p.solver_iterations; ++it) {
+            box.apply();
+            solve.apply();
+        }
+        if (!basic) transfers.apply();
+        if (!same_state(whole.get_state(), parts.get_state())) {
+            std::printf("%s: substep %d differs\n", name, step);
+            return 1;
+        }
+    }
+    std::printf("%s: ok, %u particles\n", name, whole.get_state().size());
+    return 0;
+}
+
+int main() {
+    if (apbf_device_count() <= 0) {
+        std::printf("no device\n");
+        return 2;
+    }
+    try {
+        return run_case("basic pbf", true) | run_case("adaptive", false);
+    } catch (const error& e) {
+        std::printf("error: %s\n", e.what());
+        return 1;
+    }
+}
