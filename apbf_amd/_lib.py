@@ -126,6 +126,7 @@ PROTOTYPES = {
                                       _sz, _vp]),
     "apbf_delete_edit_list": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _sz, _vp]),
     "apbf_scene_set_gravity": (_i32, [_vp, _pf]),
+    "apbf_scene_set_neighborhood_method": (_i32, [_vp, _i32]),
     "apbf_scene_set_range_scale": (_i32, [_vp, _f32]),
     "apbf_scene_set_position_range": (_i32, [_vp, _pf, _pf, _u32]),
     "apbf_scene_capture": (_i32, [_vp, _f32]),

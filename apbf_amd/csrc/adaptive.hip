@@ -19,8 +19,6 @@
 
 namespace apbf {
 
-#define GRID_STRIDE(i, n)                                                              \
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 enum : uint32_t { CL_NONE = 0, CL_UNDECIDED = 1, CL_WIN = 2, CL_LOSE = 3 };
 enum : uint32_t { P_NONE = 0, P_MERGE = 1, P_SPLIT = 2 };

@@ -30,6 +30,10 @@ constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 
+// Grid-stride loop over [0, n).
+#define GRID_STRIDE(i, n)                                                              \
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Grid for a grid-stride kernel over up to `capacity` items.

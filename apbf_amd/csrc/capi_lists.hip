@@ -5,8 +5,6 @@
 
 namespace apbf {
 
-#define GRID_STRIDE(i, n)                                                              \
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 __global__ __launch_bounds__(BLOCK) void k_count_values(const uint32_t* __restrict__ vals,
                                                         const uint32_t* __restrict__ d_len, uint32_t limit,

@@ -14,8 +14,6 @@
 
 namespace apbf {
 
-#define GRID_STRIDE(i, n)                                                              \
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 // velocity_handling::apply (velocity_handling.cpp:15-31): infer_velocity.comp:25-26,
 // position backup, apply_acceleration.comp:23-24, apply_velocity.comp:24-25, fused.

@@ -128,6 +128,8 @@ struct Scene {
     void particle_transfer(float dt);
     void kernel_width_from_bd();
     void neighborhood_green(float range_scale, bool spread);
+    void neighborhood_bs(float range_scale);  // neighborhood_binary_search (bsearch.hip)
+    int nb_method = 0;                        // 0: neighborhood_green, 1: neighborhood_binary_search
     void spread_kernel_width();
     void box_collision();
     void incompressibility();

@@ -308,7 +308,12 @@ void Scene::update(float dt) {
     if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_bd();
     const bool spread = !P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance;
     const float scale = range_scale_override >= 0.0f ? range_scale_override : (spread ? 1.5f : 1.0f);
-    neighborhood_green(scale, spread);  // spreading fused into the walk
+    if (nb_method == 1) {
+        neighborhood_bs(scale);
+        if (spread) spread_kernel_width();
+    } else {
+        neighborhood_green(scale, spread);  // spreading fused into the walk
+    }
     solver_loop();
     launch_count_particle_substeps(counts, particle_substeps, stream);
     if (measure_error) accumulate_density_error();
@@ -330,12 +335,16 @@ apbf_status Scene::stage(int st, float dt) {
         case APBF_STAGE_VELOCITY: velocity_handling(dt); break;
         case APBF_STAGE_PARTICLE_TRANSFER: particle_transfer(dt); break;
         case APBF_STAGE_KERNEL_WIDTH_BD: kernel_width_from_bd(); break;
-        case APBF_STAGE_NEIGHBORHOOD:
-            neighborhood_green(range_scale_override >= 0.0f ? range_scale_override
-                                                            : (P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f
-                                                                                                                    : 1.5f,
-                               false);
+        case APBF_STAGE_NEIGHBORHOOD: {
+            const float scale = range_scale_override >= 0.0f ? range_scale_override
+                                : (P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f
+                                                                                        : 1.5f;
+            if (nb_method == 1)
+                neighborhood_bs(scale);
+            else
+                neighborhood_green(scale, false);
             break;
+        }
         case APBF_STAGE_SPREAD: spread_kernel_width(); break;
         case APBF_STAGE_BOX: box_collision(); break;
         case APBF_STAGE_INCOMPRESSIBILITY: incompressibility(); break;
@@ -507,6 +516,14 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
             hipStreamSynchronize(s->stream) != hipSuccess)
             return APBF_ERR_HIP;
     }
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method) {
+    if (!scene || method < 0 || method > 1) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->nb_method = method;
+    scene_drop_graph(s);
     return APBF_OK;
 }
 

@@ -132,6 +132,11 @@ class Scene:
                                              int(reset)), "error_stats")
         return m.value, x.value, l.value, a.value, n.value
 
+    def set_neighborhood_method(self, method: str) -> None:
+        """'green' (hash grid) or 'binary_search' (z-order codes)."""
+        check(self._L.apbf_scene_set_neighborhood_method(self._h, {"green": 0, "binary_search": 1}[method]),
+              "set_neighborhood_method")
+
     def capture(self, dt: float) -> None:
         """Capture two substeps into a HIP graph (replay advances by two)."""
         check(self._L.apbf_scene_capture(self._h, float(dt)), "capture")

@@ -278,6 +278,11 @@ apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_del
  * and neighborhood_green::set_position_range (neighborhood_green.h:13). Call
  * them between substeps; they drop a captured graph. */
 apbf_status apbf_scene_set_gravity(void* scene, const float* gravity3);
+/* The neighbour search of the scene's neighbourhood stage, the compile-time
+ * NEIGHBORHOOD_TYPE of the reference (cpu_gpu_shared_config.h:54-60):
+ * 0 neighborhood_green (hash grid, default), 1 neighborhood_binary_search
+ * (z-order codes of the raw positions, neighborhood_binary_search.cpp:22-75). */
+apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method);
 apbf_status apbf_scene_set_range_scale(void* scene, float scale);
 apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2);
 

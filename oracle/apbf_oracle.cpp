@@ -483,6 +483,145 @@ struct Scene {
         }
     }
 
+    // ------------------------------------- neighborhood_binary_search::apply
+    // neighborhood_binary_search.cpp:22-75: three chained stable sorts by the
+    // 32-bit sections of the 96-bit z-order code of the raw positions
+    // (lexicographic by section 2, 1, 0), the particles permuted into that
+    // order, then neighborhood_binary_search.comp per particle.
+    int nb_method = 0;  // 0: neighborhood_green, 1: neighborhood_binary_search
+
+    void neighborhood_bs(float range_scale) {
+        uint32_t nh = H();
+        std::vector<uint32_t> idx(nh), keys(nh), sk, si;
+        for (uint32_t i = 0; i < nh; ++i) idx[i] = i;
+        for (uint32_t s = 0; s < 3u; ++s) {
+            for (uint32_t j = 0; j < nh; ++j) keys[j] = position_code(position[idx[j]], s);
+            radix_sort(keys.data(), idx.data(), nh, 0xFFFFFFFFu, sk, si);
+            idx = si;
+        }
+        hidden_particle_edit(idx);
+        {
+            std::vector<uint32_t> io(F()), fk, fi;
+            for (uint32_t i = 0; i < F(); ++i) io[i] = i;
+            radix_sort(fidx.data(), io.data(), F(), P.cap_particles, fk, fi);
+            fidx = fk;
+            gather(target_radius, fi);
+            gather(kernel_width, fi);
+            gather(boundariness, fi);
+            gather(boundary_distance, fi);
+        }
+        pairs.clear();
+        binary_search_walk(range_scale);
+    }
+
+    struct C96 {
+        uint32_t v[3];
+    };
+    static bool gt96(const C96& a, const C96& b) {  // greater96
+        if (a.v[2] != b.v[2]) return a.v[2] > b.v[2];
+        if (a.v[1] != b.v[1]) return a.v[1] > b.v[1];
+        return a.v[0] > b.v[0];
+    }
+    static C96 plus96(const C96& a, const C96& b) {
+        C96 r{{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}};
+        bool y = r.v[0] < a.v[0];
+        bool z = r.v[1] < a.v[1] || (y && r.v[1] == 0xFFFFFFFFu);
+        r.v[1] += y ? 1u : 0u;
+        r.v[2] += z ? 1u : 0u;
+        return r;
+    }
+    static C96 minus96(const C96& a, const C96& b) {
+        bool y = a.v[0] < b.v[0];
+        bool z = a.v[1] < b.v[1] || (y && a.v[1] == b.v[1]);
+        return C96{{a.v[0] - b.v[0], a.v[1] - b.v[1] - (y ? 1u : 0u), a.v[2] - b.v[2] - (z ? 1u : 0u)}};
+    }
+    static C96 shl96(const C96& a, uint32_t d) {  // leftShift96 for 0 < d < 32
+        return C96{{a.v[0] << d, (a.v[1] << d) | (a.v[0] >> (32u - d)), (a.v[2] << d) | (a.v[1] >> (32u - d))}};
+    }
+    static C96 and96(const C96& a, const C96& b) { return C96{{a.v[0] & b.v[0], a.v[1] & b.v[1], a.v[2] & b.v[2]}}; }
+    static C96 or96(const C96& a, const C96& b) { return C96{{a.v[0] | b.v[0], a.v[1] | b.v[1], a.v[2] | b.v[2]}}; }
+    static C96 not96(const C96& a) { return C96{{~a.v[0], ~a.v[1], ~a.v[2]}}; }
+
+    // ceil(log2(x)) of the exact logarithm for positive finite x (the shader's
+    // uint(ceil(log2(range * POS_RESOLUTION)))); 0 for x <= 1 and NaN, and
+    // UINT_MAX for +inf (the saturating float-to-uint conversion).
+    static uint32_t ceil_log2_u(float x) {
+        if (!(x > 1.0f)) return 0u;
+        if (x == INFINITY) return 0xFFFFFFFFu;
+        int e;
+        float m = std::frexp(x, &e);  // x = m * 2^e, m in [0.5, 1)
+        return (uint32_t)(m == 0.5f ? e - 1 : e);
+    }
+
+    void binary_search_walk(float range_scale) {  // neighborhood_binary_search.comp:215-276
+        const uint32_t n = F();
+        const uint32_t cap = P.cap_pairs;
+        uint32_t len = 0;
+        std::vector<C96> codes(n);
+        for (uint32_t id = 0; id < n; ++id)
+            for (uint32_t s = 0; s < 3u; ++s) codes[id].v[s] = position_code(position[fidx[id]], s);
+        const C96 ones{{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}};
+        // The shader keeps its workgroup's 256 codes in shared memory; entries of
+        // the last workgroup past the list end are never written there (their
+        // invocations return first). They are taken as the largest code.
+        auto block_code = [&](uint32_t k) { return k < n ? codes[k] : ones; };
+        auto search = [&](const C96& code, uint32_t id) {  // binary_search, :148-190
+            const uint32_t b0 = (id / 256u) * 256u;
+            uint32_t lo = 0u, hi = n;
+            if (!gt96(block_code(b0), code) && !gt96(code, block_code(b0 + 255u))) {
+                lo = b0;
+                hi = b0 + 256u;
+            }
+            uint32_t mid = (lo + hi) / 2u;
+            for (int it = 0; it < 31; ++it) {
+                const C96 c = (mid - b0 < 256u) ? block_code(mid) : codes[mid];
+                if (gt96(code, c)) lo = mid + 1u;
+                else hi = mid;
+                mid = std::min((lo + hi) / 2u, n - 1u);
+            }
+            return lo;
+        };
+        const C96 xm{{011111111111u, 022222222222u, 04444444444u}};
+        const C96 ym{{022222222222u, 04444444444u, 011111111111u}};
+        const C96 zm{{04444444444u, 011111111111u, 022222222222u}};
+        for (uint32_t id = 0; id < n; ++id) {
+            const float range = kernel_width[id] * range_scale;
+            const I4 ip = position[fidx[id]];
+            const C96 code = codes[id];
+            const uint32_t digits = ceil_log2_u(range * POS_RES) * 3u;
+            C96 mask;
+            mask.v[0] = (digits < 32u ? 1u << digits : 0u) - 1u;
+            mask.v[1] = (digits < 64u ? 1u << (std::max(digits, 32u) - 32u) : 0u) - 1u;
+            mask.v[2] = (digits < 96u ? 1u << (std::max(digits, 64u) - 64u) : 0u) - 1u;
+            const C96 center{{code.v[0] - (code.v[0] & mask.v[0]), code.v[1] - (code.v[1] & mask.v[1]),
+                              code.v[2] - (code.v[2] & mask.v[2])}};
+            const C96 step = plus96(mask, C96{{1u, 0u, 0u}});
+            C96 X[3], Y[3], Z[3];
+            X[0] = and96(minus96(and96(center, xm), step), xm);
+            X[2] = and96(plus96(or96(center, not96(xm)), step), xm);
+            Y[0] = and96(minus96(and96(center, ym), shl96(step, 1u)), ym);
+            Y[2] = and96(plus96(or96(center, not96(ym)), shl96(step, 1u)), ym);
+            Z[0] = and96(minus96(and96(center, zm), shl96(step, 2u)), zm);
+            Z[2] = and96(plus96(or96(center, not96(zm)), shl96(step, 2u)), zm);
+            X[1] = and96(center, xm);
+            Y[1] = and96(center, ym);
+            Z[1] = and96(center, zm);
+            const V3 pos = pos_world(ip);
+            for (uint32_t c = 0; c < 27u; ++c) {
+                const C96 cell = or96(or96(X[c % 3u], Y[(c / 3u) % 3u]), Z[c / 9u]);
+                const C96 last = or96(cell, mask);
+                for (uint32_t idN = search(cell, id); idN < n && !gt96(codes[idN], last); ++idN) {
+                    const V3 posN = pos_world(position[fidx[idN]]);
+                    if (id != idN && length(sub(pos, posN)) <= range) {
+                        uint32_t i = len++;  // neighbor_add.glsl
+                        if (i < cap) pairs.push_back(U2{id, idN});
+                        else len = cap;
+                    }
+                }
+            }
+        }
+    }
+
     // neighborhood_green.comp:49-87 with neighbor_add.glsl:230-244 (unsorted list)
     void green(const Grid& g, const uint32_t* cs, const uint32_t* ce, float range_scale) {
         uint32_t cap = P.cap_pairs;
@@ -870,7 +1009,10 @@ struct Scene {
         velocity_handling(dt);
         if (!P.basic_pbf && (s.mMerge || s.mSplit)) particle_transfer(dt);
         if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_boundary_distance();
-        neighborhood((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+        if (nb_method == 1)
+            neighborhood_bs((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+        else
+            neighborhood((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
         if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
         for (int i = 0; i < P.solver_iterations; ++i) {
             box_collision();
@@ -966,6 +1108,8 @@ uint32_t orc_get_splits(void* h, uint32_t* split_list) {
     return (uint32_t)s->split_list.size();
 }
 
+void orc_set_neighborhood_method(void* h, int method) { static_cast<Scene*>(h)->nb_method = method; }
+
 void orc_stage(void* h, int stage, float dt) {
     Scene* s = static_cast<Scene*>(h);
     const auto& st = s->P.s;
@@ -974,7 +1118,10 @@ void orc_stage(void* h, int stage, float dt) {
         case ORC_STAGE_PARTICLE_TRANSFER: s->particle_transfer(dt); break;
         case ORC_STAGE_KERNEL_WIDTH_BD: s->kernel_width_from_boundary_distance(); break;
         case ORC_STAGE_NEIGHBORHOOD:
-            s->neighborhood((s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            if (s->nb_method == 1)
+                s->neighborhood_bs((s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            else
+                s->neighborhood((s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
             break;
         case ORC_STAGE_SPREAD: s->spread_kernel_width(); break;
         case ORC_STAGE_BOX: s->box_collision(); break;

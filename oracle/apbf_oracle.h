@@ -106,6 +106,8 @@ enum {
     ORC_STAGE_INCOMPRESSIBILITY = 6,/* incompressibility::apply */
     ORC_STAGE_UPDATE_TRANSFERS = 7  /* update_transfers::apply */
 };
+/* 0: neighborhood_green (default), 1: neighborhood_binary_search. */
+void orc_set_neighborhood_method(void* scene, int method);
 void orc_stage(void* h, int stage, float dt);
 /* One full substep (pool::update / waterdrop::update with the time machine off). */
 void orc_update(void* h, float dt);

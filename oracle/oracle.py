@@ -84,6 +84,7 @@ def _declare(L):
     L.orc_get_splits.restype = u32
     L.orc_get_splits.argtypes = [vp, pu]
     L.orc_stage.argtypes = [vp, C.c_int, f32]
+    L.orc_set_neighborhood_method.argtypes = [vp, C.c_int]
     L.orc_update.argtypes = [vp, f32]
     L.orc_density_error.argtypes = [vp, _p(f64), _p(f64)]
     L.orc_stable_radix_sort.argtypes = [pu, pu, u32, u32, pu, pu]
@@ -245,6 +246,9 @@ class OracleScene:
         out = np.zeros(max(n, 1), np.uint32)
         self._L.orc_get_splits(self._h, _ptr(out, C.c_uint32))
         return out[:n]
+
+    def set_neighborhood_method(self, method: str) -> None:
+        self._L.orc_set_neighborhood_method(self._h, {"green": 0, "binary_search": 1}[method])
 
     def stage(self, name: str, dt: float = 0.0) -> None:
         self._L.orc_stage(self._h, STAGE[name], float(dt))

@@ -185,3 +185,27 @@ def test_spreading_boundary_widths(device):
         g.update(p["dt"])
         r.update(p["dt"])
         _check(g, r, f"substep {step_i}")
+
+
+@pytest.mark.parametrize("kind", ["dam_break", "water_drop_active"])
+def test_binary_search_neighborhood_matches_oracle(device, kind):
+    """neighborhood_binary_search: the z-order sort, the 27-cell lower-bound
+    walk and the substeps built on it, bit for bit."""
+    if kind == "dam_break":
+        p, st = cfg.dam_break(n_side=14)
+        rng = np.random.default_rng(3)
+        st["kernel_width"] = (st["kernel_width"] * rng.uniform(0.5, 1.7, len(st["radius"]))).astype(np.float32)
+    else:
+        p, st = cfg.small_water_drop(seed=8, active=True)
+    g, r = _pair(p, st)
+    g.set_neighborhood_method("binary_search")
+    r.set_neighborhood_method("binary_search")
+    g.stage("velocity", p["dt"])
+    r.stage("velocity", p["dt"])
+    g.stage("neighborhood", p["dt"])
+    r.stage("neighborhood", p["dt"])
+    _check(g, r, "after the neighbourhood stage")
+    for step in range(4):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step}")

@@ -286,3 +286,62 @@ def test_update_transfers_is_a_legal_schedule():
         seen_merge |= bool(merge.any())
         seen_split |= bool((~merge).any())
     assert seen_merge and seen_split
+
+
+def _bs_state(seed, n_side=9, spacing=0.5):
+    p, st = cfg.dam_break(n_side=n_side, spacing=spacing, seed=seed)
+    rng = np.random.default_rng(seed)
+    st["kernel_width"] = (st["kernel_width"] * rng.uniform(0.6, 1.6, len(st["radius"]))).astype(np.float32)
+    return p, st
+
+
+def test_binary_search_neighborhood_order_and_set():
+    """neighborhood_binary_search (neighborhood_binary_search.cpp:22-75): the
+    particles end in the order of the 96-bit z-order code of their raw
+    positions (a stable lexicographic sort by code sections 2, 1, 0), and the
+    pair set equals the brute-force predicate d(i, j) <= range_i, i != j."""
+    from oracle import oracle as O
+    for seed in (0, 1, 2):
+        p, st = _bs_state(seed)
+        o = OracleScene(p)
+        o.set_state(st)
+        o.set_neighborhood_method("binary_search")
+        o.stage("neighborhood", 0.0)
+        a = o.get_state()
+        codes = np.stack([O.position_code(a["position"], s) for s in range(3)], 1).astype(np.uint64)
+        key = (codes[:, 2] << np.uint64(32)) | codes[:, 1]
+        order = np.lexsort((codes[:, 0], codes[:, 1], codes[:, 2]))
+        assert np.array_equal(order, np.arange(len(order)))  # already sorted, stably
+        pos = a["position"][:, :3].astype(np.float32) / F(262144.0)
+        d = pos[:, None, :] - pos[None, :, :]
+        d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        dist = np.sqrt(d2.astype(F))
+        rng_i = a["kernel_width"][:, None]
+        want = np.argwhere((dist <= rng_i) & ~np.eye(len(pos), dtype=bool))
+        got = a["pairs"].astype(np.int64)
+        assert np.array_equal(np.unique(got, axis=0), want)
+        assert len(got) == len(want)  # no duplicates
+        # rows are contiguous and in id order
+        assert np.all(np.diff(got[:, 0]) >= 0)
+
+
+def test_binary_search_substeps_track_green():
+    """A plain dam break stepped with either neighbourhood search: both find the
+    same pair sets, so the order-free integer sums keep the states equal (no
+    walls: the box collision jitter depends on the particle's slot, which the
+    two searches order differently)."""
+    p, st = cfg.dam_break(n_side=8, seed=4)
+    p["boxes"] = []
+    a, b = OracleScene(p), OracleScene(p)
+    a.set_state(st)
+    b.set_state(st)
+    b.set_neighborhood_method("binary_search")
+    for _ in range(4):
+        a.update(p["dt"])
+        b.update(p["dt"])
+    sa, sb = a.get_state(), b.get_state()
+    ka = sa["pos_backup"][:, :3].astype(np.int64)
+    kb = sb["pos_backup"][:, :3].astype(np.int64)
+    ia = np.lexsort((ka[:, 2], ka[:, 1], ka[:, 0]))
+    ib = np.lexsort((kb[:, 2], kb[:, 1], kb[:, 0]))
+    assert np.array_equal(sa["position"][ia], sb["position"][ib])
