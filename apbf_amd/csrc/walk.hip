@@ -86,6 +86,26 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             t_cut = sqrt_le_bound(fmax_g(okw, kwi));
             t_okw = sqrt_le_bound(okw);
         }
+        // Cell culling: a cell whose box lies farther than the range holds no
+        // accepted candidate. The cell boxes are widened by margins far above
+        // the rounding of the hash (grid_cell) and of the distance test, and
+        // the cells at the grid border (which also hold clamped or aliased
+        // particles) are never culled, so culling drops no accepted pair. A
+        // search box reaching past the grid end can visit an aliased bucket
+        // twice; such walks are not culled.
+        const uint32_t last_cell = (1u << g.res) - 1u;
+        const bool cull = t_range < __builtin_inff() && range >= 0.0f && hi[0] <= last_cell && hi[1] <= last_cell &&
+                          hi[2] <= last_cell;
+        const float thr = range * range * 1.001f + 1e-30f;
+        auto axis_d2 = [&](int d, uint32_t c, float pc) -> float {
+            if (!cull || c == 0u || c >= last_cell) return 0.0f;
+            const float w = (g.mx[d] - g.mn[d]) / (float)(1u << g.res);
+            const float m = 1e-4f * w + 1e-6f * (__builtin_fabsf(g.mn[d]) + __builtin_fabsf(g.mx[d]));
+            const float a = g.mn[d] + (float)c * w - m;
+            const float b = g.mn[d] + (float)(c + 1u) * w + m;
+            const float t = pc < a ? a - pc : (pc > b ? pc - b : 0.0f);
+            return t * t;
+        };
         uint32_t craw = 0, cf = 0, braw = 0, bf = 0;
         bool more = true;
         if (FILL) {
@@ -94,10 +114,13 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             more = braw < wa.pcap;
         }
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
-            for (uint32_t y = lo[1]; more && y <= hi[1]; ++y) {
-                for (uint32_t x = lo[0]; more && x <= hi[0]; ++x) {
+            const float dz2 = axis_d2(2, z, p.z);
+            for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
+                const float dyz2 = dz2 + axis_d2(1, y, p.y);
+                for (uint32_t x = lo[0]; more && x <= hi[0] && dyz2 <= thr; ++x) {
+                    const bool skip = dyz2 + axis_d2(0, x, p.x) > thr;
                     const uint32_t h = zhash(x, y, z, g.res);
-                    const uint32_t s = cs[h], e = ce[h];
+                    const uint32_t s = skip ? 0u : cs[h], e = skip ? 0u : ce[h];
                     for (uint32_t b = s; more && b < e; b += L) {
                         const uint32_t j = b + gl;
                         bool green = false, keep = false;
