@@ -209,3 +209,26 @@ def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: in
         p["settings"]["mSmallestTargetRadius"] = 0.5
         p["settings"]["mTargetRadiusScaleFactor"] = 0.05
     return p, st
+
+
+def batch_scenes(count: int = 64, n_min: int = 10_000, n_max: int = 500_000, seed: int = 0):
+    """Independent scenes for batched stepping: particle counts drawn
+    log-uniformly from [n_min, n_max] (seeded), alternating a plain dam break
+    (basic PBF lattice) and an adaptive water drop whose pool and drop are
+    scaled to the drawn count. Returns a list of (params, state)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(n_min), math.log(n_max), count)).round().astype(int)
+    out = []
+    for k, n in enumerate(counts):
+        if k % 2 == 0:
+            side = max(4, int(round(n ** (1.0 / 3.0))))
+            p, st = dam_break(n_side=side, seed=seed + k)
+        else:
+            # pool of a x b x a coarse particles with b = 0.27 a, plus a drop of n / 10
+            a = max(6, int(round((0.9 * n / 0.27) ** (1.0 / 3.0))))
+            b = max(3, int(round(0.27 * a)))
+            p, st = water_drop(pool_counts=(a, b, a), drop_particles=max(100, int(n) // 10), seed=seed + k,
+                               res=5 if n < 100_000 else 6, pairs_per_particle=300, cap_factor=1.5)
+        p["workload"] = f"batch_{k}_{p.get('workload', '')}"
+        out.append((p, st))
+    return out

@@ -163,11 +163,12 @@ __global__ __launch_bounds__(BLOCK) void k_remap_transfers(uint32_t* __restrict_
     }
 }
 
-// kernel_width_init.comp:17-31
+// kernel_width_init.comp:17-31 (when `gate` is set: only if *gate != 0)
 __global__ __launch_bounds__(BLOCK) void k_kw_init(const float* __restrict__ radius,
                                                    const float* __restrict__ trad,
                                                    const uint32_t* __restrict__ counts, int on_target,
-                                                   uint32_t* __restrict__ ukw) {
+                                                   uint32_t* __restrict__ ukw, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) {
         float tr = on_target ? trad[i] : 0.0f;
@@ -544,7 +545,8 @@ void Scene::particle_transfer(float dt) {
 void Scene::spread_kernel_width() {
     hipStream_t s = stream;
     int on_t = P.s.mBaseKernelWidthOnTargetRadius;
-    LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, on_t, ukw);
+    LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, on_t, ukw,
+           (const uint32_t*)nullptr);
     launch_set_n_plus_1(counts, s);
     bool tk = tbegin(K_SPREAD);
     launch_spread_rows(false, pos[cur], radius[cur], target_radius[cur], kernel_width[cur], counts, cap, row, col, pcap,
@@ -564,7 +566,12 @@ void Scene::spread_kernel_width() {
 void Scene::spread_begin() {
     hipStream_t s = stream;
     LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, P.s.mBaseKernelWidthOnTargetRadius,
-           ukw);
+           ukw, (const uint32_t*)nullptr);
+}
+void Scene::spread_reset_if_fallback() {  // the exact walk spreads again from the initial widths
+    hipStream_t s = stream;
+    LAUNCH(k_kw_init, stride_grid(cap), radius[cur], target_radius[cur], counts, P.s.mBaseKernelWidthOnTargetRadius,
+           ukw, counts + C_FALLBACK);
 }
 void Scene::spread_end() {
     hipStream_t s = stream;

@@ -170,11 +170,11 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
     hipMemcpyAsync(counts + C_N, d_len, 4, hipMemcpyDeviceToDevice, s);
     launch_set_n_plus_1(counts, s);
     const int4* p4 = reinterpret_cast<const int4*>(position);
-    launch_walk(false, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
+    launch_walk(0, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
                 pair_capacity, cnt, nullptr, nullptr, nullptr, nullptr, nullptr, s);
     scan_u32(cnt, row_offset, counts + C_N_PLUS_1, capacity + 1, false, nullptr, scan_ws, s);
     launch_pairs_total(counts, row_offset, pair_capacity, s);
-    launch_walk(true, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
+    launch_walk(1, false, p4, range, nullptr, nullptr, 0, cell_start, cell_end, counts, capacity, range_scale, g,
                 pair_capacity, nullptr, nullptr, row_offset, nullptr, neighbors, nullptr, s);
     hipMemcpyAsync(d_n_pairs, counts + C_PAIRS, 4, hipMemcpyDeviceToDevice, s);
     return last_status();

@@ -141,6 +141,7 @@ struct Scene {
     // helpers
     void build_pairs(float range_scale, bool spread);
     void spread_begin();
+    void spread_reset_if_fallback();
     void spread_end();
     void prepare_solver();
     SolverArgs solver_args() const;

@@ -205,21 +205,42 @@ void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;
     const int on_t = P.s.mBaseKernelWidthOnTargetRadius;
     uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list
+    const uint32_t stride = pcap / (cap ? cap : 1u);
     launch_set_n_plus_1(counts, s);
     if (spread) spread_begin();
-    bool tk = tbegin(K_WALK_COUNT);
-    launch_walk(false, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
-                cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s,
-                walk_lanes);
-    tend(tk);
-    scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    launch_pairs_total(counts, row_raw, pcap, s);
-    if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-    tk = tbegin(K_WALK_FILL);
-    launch_walk(true, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
-                cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
-                walk_lanes);
-    tend(tk);
+    if (stride == 0) {  // no room for slots: the two exact walks
+        bool tk = tbegin(K_WALK_COUNT);
+        launch_walk(0, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                    cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s,
+                    walk_lanes);
+        tend(tk);
+        scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+        launch_pairs_total(counts, row_raw, pcap, s);
+        if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+        tk = tbegin(K_WALK_FILL);
+        launch_walk(1, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                    cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
+                    walk_lanes);
+        tend(tk);
+    } else {
+        // one walk into fixed-stride slots (col_tmp), counts of both lists and
+        // the spreading; then either packing, or on overflow the exact walk
+        bool tk = tbegin(K_WALK_FILL);
+        launch_walk(2, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                    cell_end, counts, cap, range_scale, grid, pcap, cnt, spread ? u1 : nullptr, nullptr, nullptr,
+                    col_tmp, ukw, s, walk_lanes, stride);
+        tend(tk);
+        scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+        launch_pairs_total(counts, row_raw, pcap, s);
+        if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+        launch_walk_decide(counts, pcap, stride, s);
+        launch_pack_rows(counts, cap, row, col_tmp, stride, col, s);
+        // exact path (kernels return at once unless counts[C_FALLBACK] is set)
+        if (spread) spread_reset_if_fallback();
+        launch_walk(1, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                    cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
+                    walk_lanes, 0, counts + C_FALLBACK);
+    }
     if (spread) spread_end();
     pairs_valid = true;
 }

@@ -6,10 +6,16 @@
 // consecutive output slots, so every row keeps the sequential order and the
 // writes of a step are contiguous.
 //
-// Two passes: COUNT writes each row's length, FILL writes the rows at the
-// offsets of the scanned counts. Rows are clamped to the pair capacity the way
-// the sequential schedule fills a bounded list: entries whose slot in the full
-// list is at or beyond the capacity are dropped (neighbor_add.glsl:22-23).
+// Modes: COUNT writes each row's length, FILL writes the rows at the offsets
+// of the scanned counts, clamped to the pair capacity the way the sequential
+// schedule fills a bounded list: entries whose slot in the full list is at or
+// beyond the capacity are dropped (neighbor_add.glsl:22-23). PAD does both in
+// one walk: it writes every row into a fixed-stride slot of its own (stride =
+// capacity / particles) and counts it; when no row outgrew its slot and the
+// list fits the capacity (the normal case), the rows are then packed into the
+// list, and otherwise the exact COUNT-less FILL walk runs (the counts of PAD
+// are the counts of COUNT). The choice is made on the device, so the substep
+// needs no host synchronisation.
 //
 // With SPREAD the walk also carries kernel_width.comp:22-56: every entry of
 // the (clamped) list spreads the particle's original kernel width to the
@@ -28,7 +34,11 @@ struct WalkArgs {
     GridDesc g;
     int on_target;  // spreading: kernel width from max(radius, target radius)
     uint32_t pcap;
+    uint32_t stride;        // PAD: slot entries per particle
+    const uint32_t* gate;   // FILL: run only when *gate != 0 (the PAD fallback); may be null
 };
+
+enum WalkMode { WALK_COUNT = 0, WALK_FILL = 1, WALK_PAD = 2 };
 
 template <int L>
 __device__ __forceinline__ uint64_t walk_group_mask() {
@@ -37,7 +47,7 @@ __device__ __forceinline__ uint64_t walk_group_mask() {
     return m << (lane & ~(L - 1));
 }
 
-template <int L, bool FILL, bool SPREAD>
+template <int L, int MODE, bool SPREAD>
 __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ pos,
                                                      const float* __restrict__ kw,
                                                      const float* __restrict__ radius,
@@ -49,6 +59,9 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                                                      const uint32_t* __restrict__ row_raw,
                                                      const uint32_t* __restrict__ row_f, uint32_t* __restrict__ col,
                                                      uint32_t* __restrict__ ukw) {
+    constexpr bool FILL = MODE == WALK_FILL;
+    constexpr bool PAD = MODE == WALK_PAD;
+    if (FILL && wa.gate && *wa.gate == 0u) return;
     const int lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t gm = walk_group_mask<L>();
@@ -113,6 +126,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             bf = SPREAD ? row_f[i] : braw;
             more = braw < wa.pcap;
         }
+        const size_t slot = PAD ? (size_t)i * wa.stride : 0;
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
             const float dz2 = axis_d2(2, z, p.z);
             for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
@@ -133,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                                 const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
                                 const float d2 = dot(diff, diff);
                                 keep = d2 <= t_cut;
-                                if (FILL) {
+                                if (FILL || PAD) {
                                     if (d2 <= t_okw) {  // distance <= okw: influence exactly 1
                                         uval = to_u32(okw * KW_RES);
                                     } else {
@@ -146,7 +160,15 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                             }
                         }
                         const uint64_t rbal = __ballot(green) & gm;
-                        if (!FILL) {
+                        if (PAD) {
+                            if (SPREAD && green && uval) atomicMax(&ukw[j], uval);
+                            const bool kept = SPREAD ? (green && keep) : green;
+                            const uint64_t kbal = SPREAD ? (__ballot(kept) & gm) : rbal;
+                            const uint32_t k = cf + (uint32_t)__popcll(kbal & below);
+                            if (kept && k < wa.stride) col[slot + k] = j;
+                            cf += (uint32_t)__popcll(kbal);
+                            craw += (uint32_t)__popcll(rbal);
+                        } else if (!FILL) {
                             craw += (uint32_t)__popcll(rbal);
                             if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep) & gm);
                         } else {
@@ -172,7 +194,11 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             if (z == 0xFFFFFFFFu) break;
         }
         if (gl == 0) {
-            if (!FILL) {
+            if (PAD) {
+                cnt_raw[i] = craw;
+                if (SPREAD) cnt_f[i] = cf;
+                if (cf > wa.stride) atomicMax(&counts[C_MAXROW], cf);
+            } else if (!FILL) {
                 cnt_raw[i] = craw;
                 if (SPREAD) cnt_f[i] = cf;
             } else if (SPREAD && braw <= wa.pcap && row_raw[i + 1] > wa.pcap) {
@@ -185,6 +211,33 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
     if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) {
         cnt_raw[n] = 0;
         if (SPREAD) cnt_f[n] = 0;
+    }
+}
+
+// After a PAD walk and the scan of its raw counts: take the exact path when a
+// row outgrew its slot or the list exceeds the capacity.
+__global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap, uint32_t stride) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    counts[C_FALLBACK] = (counts[C_PAIRS_RAW] > pcap || counts[C_MAXROW] > stride) ? 1u : 0u;
+    counts[C_MAXROW] = 0u;
+}
+
+// Packs the PAD slots into the list at the scanned offsets (16 lanes per row,
+// coalesced); skipped when the exact path runs.
+__global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ row,
+                                                     const uint32_t* __restrict__ slots, uint32_t stride,
+                                                     uint32_t* __restrict__ col) {
+    if (counts[C_FALLBACK]) return;
+    constexpr int L = 16;
+    const uint32_t n = counts[C_N];
+    const int gl = threadIdx.x & (L - 1);
+    for (uint32_t base = blockIdx.x * (BLOCK / L); base < n; base += gridDim.x * (BLOCK / L)) {
+        const uint32_t i = base + threadIdx.x / L;
+        if (i >= n) continue;
+        const uint32_t b = row[i], len = row[i + 1] - b;
+        const uint32_t* src = slots + (size_t)i * stride;
+        for (uint32_t k = gl; k < len; k += L) col[b + k] = src[k];
     }
 }
 
@@ -206,32 +259,39 @@ static uint32_t walk_grid(uint32_t cap, uint32_t lanes) {
     return g < 16384u ? g : 16384u;
 }
 
-void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
+void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
-                 int lanes) {
+                 int lanes, uint32_t stride, const uint32_t* gate) {
     WalkArgs wa;
     wa.range_scale = range_scale;
     wa.g = g;
     wa.on_target = on_target;
     wa.pcap = pcap;
+    wa.stride = stride;
+    wa.gate = gate;
     if (lanes != 4 && lanes != 16) lanes = WALK_LANES;
     const dim3 grid(walk_grid(cap, (uint32_t)lanes)), block(BLOCK);
 #define WALK(L, F, S)                                                                                                \
     hipLaunchKernelGGL((k_walk_rows<L, F, S>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
                        cnt_f, row_raw, row_f, col, ukw)
-#define WALK_L(L)                  \
-    if (fill) {                    \
-        if (spread)                \
-            WALK(L, true, true);   \
-        else                       \
-            WALK(L, true, false);  \
-    } else {                       \
-        if (spread)                \
-            WALK(L, false, true);  \
-        else                       \
-            WALK(L, false, false); \
+#define WALK_L(L)                       \
+    if (mode == WALK_PAD) {             \
+        if (spread)                     \
+            WALK(L, WALK_PAD, true);    \
+        else                            \
+            WALK(L, WALK_PAD, false);   \
+    } else if (mode == WALK_FILL) {     \
+        if (spread)                     \
+            WALK(L, WALK_FILL, true);   \
+        else                            \
+            WALK(L, WALK_FILL, false);  \
+    } else {                            \
+        if (spread)                     \
+            WALK(L, WALK_COUNT, true);  \
+        else                            \
+            WALK(L, WALK_COUNT, false); \
     }
     if (lanes == 4) {
         WALK_L(4)
@@ -242,6 +302,15 @@ void launch_walk(bool fill, bool spread, const int4* pos, const float* kw, const
     }
 #undef WALK_L
 #undef WALK
+}
+
+void launch_walk_decide(uint32_t* counts, uint32_t pcap, uint32_t stride, hipStream_t s) {
+    hipLaunchKernelGGL(k_walk_decide, dim3(1), dim3(64), 0, s, counts, pcap, stride);
+}
+void launch_pack_rows(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* slots,
+                      uint32_t stride, uint32_t* col, hipStream_t s) {
+    uint32_t g = div_up(cap ? cap : 1, BLOCK / 16);
+    hipLaunchKernelGGL(k_pack_rows, dim3(g < 8192u ? g : 8192u), dim3(BLOCK), 0, s, counts, row, slots, stride, col);
 }
 
 void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s) {

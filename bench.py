@@ -127,6 +127,62 @@ def learned_comparison(scene, params, args, log):
             "tolerance": 1e-3}
 
 
+def run_batch(args, log):
+    """64 independent scenes (10k-500k particles, log-uniform) stepped as one
+    batch: eager (every scene's substep enqueued on its own stream) and as a
+    captured two-substep graph of all scenes."""
+    import torch
+    from apbf_amd import config as cfg
+    from apbf_amd.scene import Batch, Scene
+    cases = cfg.batch_scenes(64)
+    dt = cases[0][0]["dt"]
+    scenes = []
+    for p, st in cases:
+        s = Scene(p)
+        s.set_state(st)
+        scenes.append(s)
+    n_total = sum(len(st["radius"]) for _, st in cases)
+    log(f"{len(scenes)} scenes, {n_total} particles")
+    batch = Batch(scenes)
+    for _ in range(args.warmup):
+        batch.update(dt)
+    batch.synchronize()
+    steps = max(2, args.steps - args.steps % 2)
+    for s in scenes:
+        s.particle_substeps(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.update(dt)
+    batch.synchronize()
+    eager_s = time.perf_counter() - t0
+    eager_ps = sum(s.particle_substeps(reset=True)[0] for s in scenes)
+    batch.capture(dt)
+    batch.replay(1)  # first launch of the graph, untimed
+    batch.synchronize()
+    for s in scenes:
+        s.particle_substeps(reset=True)
+    t0 = time.perf_counter()
+    batch.replay(steps // 2)
+    batch.synchronize()
+    graph_s = time.perf_counter() - t0
+    graph_ps = sum(s.particle_substeps(reset=True)[0] for s in scenes)
+    log(f"eager {eager_s / steps * 1e3:.2f} ms/substep, graph {graph_s / steps * 1e3:.2f} ms/substep")
+    out = {
+        "metric": "particle-substeps/s, 64 independent scenes of 10k-500k particles stepped as one batch",
+        "value": graph_ps / graph_s, "unit": "particle-substeps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": graph_s / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic",
+        "config": {"workload": "batch64", "scenes": len(scenes), "particles": n_total,
+                   "kinds": "alternating plain dam break / adaptive water drop", "seed": 0},
+        "eager": {"value": eager_ps / eager_s, "ms_per_step": eager_s / steps * 1e3},
+    }
+    batch.close()
+    for s in scenes:
+        s.close()
+    print(json.dumps(out), flush=True)
+
+
 def workload(name: str):
     from apbf_amd import config as cfg
     if name == "water_drop":
@@ -191,6 +247,9 @@ def main():
     import torch
     from apbf_amd.scene import Scene
 
+    if args.workload == "batch64":
+        run_batch(args, lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
+        return
     params, state = workload(args.workload)
     if args.iterations is not None:
         params["solver_iterations"] = args.iterations
