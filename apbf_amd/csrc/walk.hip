@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
         const uint32_t last_cell = (1u << g.res) - 1u;
         const bool cull = t_range < __builtin_inff() && range >= 0.0f && hi[0] <= last_cell && hi[1] <= last_cell &&
                           hi[2] <= last_cell;
-        const float thr = range * range * 1.001f + 1e-30f;
+        const float thr = cull ? range * range * 1.001f + 1e-30f : __builtin_inff();  // NaN range: no cull
         auto axis_d2 = [&](int d, uint32_t c, float pc) -> float {
             if (!cull || c == 0u || c >= last_cell) return 0.0f;
             const float w = (g.mx[d] - g.mn[d]) / (float)(1u << g.res);
