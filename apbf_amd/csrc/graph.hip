@@ -8,11 +8,11 @@
 // substeps always return them to their starting roles: the captured graph
 // holds two substeps and every replay advances the scene by two.
 //
-// A batch captures the substeps of many scenes into one graph: a fork event
-// lets every scene's stream join the capture, each scene records its two
-// substeps on its own stream, and join events bring them back. One graph
-// launch then steps every scene, and scenes of uneven size run side by side
-// on the device instead of one after another.
+// A batch steps many scenes, each on a stream of its own so the device runs
+// scenes of uneven size side by side. Captured, every scene's two substeps
+// become a graph of their own, and a replay launches all of them on the
+// scenes' streams. (One graph holding every scene as a parallel branch was
+// measured slower than the eager streams: its branches did not overlap.)
 #include <vector>
 
 #include "scene.h"
@@ -61,12 +61,7 @@ void scene_release_graph(Scene* s) {
 
 struct apbf_batch {
     std::vector<Scene*> scenes;
-    std::vector<hipStream_t> own;
-    std::vector<hipEvent_t> joins;
-    hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    bool captured = false;
 };
 
 extern "C" {
@@ -123,17 +118,6 @@ apbf_status apbf_batch_create(void* const* scenes, uint32_t n, apbf_batch** out)
             }
         }
         b->scenes.push_back(s);
-        hipEvent_t e;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            apbf_batch_destroy(b);
-            return APBF_ERR_HIP;
-        }
-        b->joins.push_back(e);
-    }
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
-        apbf_batch_destroy(b);
-        return APBF_ERR_HIP;
     }
     *out = b;
     return APBF_OK;
@@ -141,11 +125,6 @@ apbf_status apbf_batch_create(void* const* scenes, uint32_t n, apbf_batch** out)
 
 apbf_status apbf_batch_destroy(apbf_batch* b) {
     if (!b) return APBF_ERR_INVALID_ARGUMENT;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return APBF_OK;
 }
@@ -159,51 +138,27 @@ apbf_status apbf_batch_update(apbf_batch* b, float dt) {
 
 apbf_status apbf_batch_capture(apbf_batch* b, float dt) {
     if (!b) return APBF_ERR_INVALID_ARGUMENT;
-    for (Scene* s : b->scenes)
-        if (s->timed_kernel >= 0 || hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_INVALID_ARGUMENT;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-    std::vector<HostRoles> before;
-    for (Scene* s : b->scenes) before.push_back(roles(s));
-    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess ||
-        hipEventRecord(b->fork, b->stream) != hipSuccess)
-        return APBF_ERR_HIP;
-    bool ok = true;
-    for (size_t k = 0; k < b->scenes.size() && ok; ++k) {
-        Scene* s = b->scenes[k];
-        ok = hipStreamWaitEvent(s->stream, b->fork, 0) == hipSuccess;
-        for (int i = 0; i < 2 && ok; ++i) s->update(dt);
-        ok = ok && hipEventRecord(b->joins[k], s->stream) == hipSuccess &&
-             hipStreamWaitEvent(b->stream, b->joins[k], 0) == hipSuccess;
+    for (Scene* s : b->scenes) {
+        apbf_status st = apbf_scene_capture(s, dt);
+        if (st != APBF_OK) return st;
     }
-    hipGraph_t g = nullptr;
-    if (hipStreamEndCapture(b->stream, &g) != hipSuccess || !g || !ok) {
-        if (g) (void)hipGraphDestroy(g);
-        return APBF_ERR_HIP;
-    }
-    for (size_t k = 0; k < b->scenes.size(); ++k)
-        if (!(roles(b->scenes[k]) == before[k])) {
-            (void)hipGraphDestroy(g);
-            return APBF_ERR_INVALID_ARGUMENT;
-        }
-    b->graph = g;
-    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return APBF_ERR_HIP;
+    b->captured = true;
     return APBF_OK;
 }
 
 apbf_status apbf_batch_replay(apbf_batch* b, uint32_t times) {
-    if (!b || !b->exec) return APBF_ERR_INVALID_ARGUMENT;
+    if (!b || !b->captured) return APBF_ERR_INVALID_ARGUMENT;
     for (uint32_t k = 0; k < times; ++k)
-        if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return APBF_ERR_HIP;
+        for (Scene* s : b->scenes) {
+            if (!s->graph_exec) return APBF_ERR_INVALID_ARGUMENT;  // a scene's graph was dropped since
+            if (hipGraphLaunch(s->graph_exec, s->stream) != hipSuccess) return APBF_ERR_HIP;
+        }
     for (Scene* s : b->scenes) s->pairs_valid = true;
     return APBF_OK;
 }
 
 apbf_status apbf_batch_synchronize(apbf_batch* b) {
     if (!b) return APBF_ERR_INVALID_ARGUMENT;
-    if (hipStreamSynchronize(b->stream) != hipSuccess) return APBF_ERR_HIP;
     for (Scene* s : b->scenes)
         if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
     return APBF_OK;

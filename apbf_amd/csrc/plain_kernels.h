@@ -34,8 +34,9 @@ void launch_ranges(const uint32_t* keys, const uint32_t* d_len, uint32_t cap, ui
                    uint32_t* end, hipStream_t s);
 // Row-group neighbour walk (COUNT / FILL passes), optionally fused with the
 // kernel-width spreading and its short-list filter.
-// mode 0 count, 1 fill (exact, clamped; runs only when *gate != 0 if gate is
-// set), 2 single pass into fixed-stride slots (see walk.hip).
+// mode 0 count, 1 fill (exact, clamped; after a mode-2 walk `gate` is the
+// count block, which selects all rows, the rows longer than `stride`, or
+// none), 2 single pass into fixed-stride slots (see walk.hip).
 void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const float* radius, const float* trad,
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,

@@ -235,11 +235,12 @@ void Scene::build_pairs(float range_scale, bool spread) {
         if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
         launch_walk_decide(counts, pcap, stride, s);
         launch_pack_rows(counts, cap, row, col_tmp, stride, col, s);
-        // exact path (kernels return at once unless counts[C_FALLBACK] is set)
+        // exact fill: every row when the list is clamped (after restarting the
+        // spreading), the rows longer than their slot otherwise, else nothing
         if (spread) spread_reset_if_fallback();
         launch_walk(1, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                     cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
-                    walk_lanes, 0, counts + C_FALLBACK);
+                    walk_lanes, stride, counts);
     }
     if (spread) spread_end();
     pairs_valid = true;

@@ -29,7 +29,8 @@ enum CountSlot : uint32_t {
     C_CLAMP_ROW = 18,  // row cut by the pair capacity in a spreading walk
     C_CLAMP_KEPT = 19, // entries of that row kept in the short list
     C_MAXROW = 20,     // single-pass walk: longest row that outgrew its slot (0: none)
-    C_FALLBACK = 21,   // single-pass walk: 1 when the exact two-pass fill must run
+    C_FALLBACK = 21,   // single-pass walk: 1 when the exact fill must rewrite every row (clamped list)
+    C_ROWFIX = 22,     // single-pass walk: 1 when some rows outgrew their slot (exact fill of those rows)
     C_COUNT = 32
 };
 

@@ -34,8 +34,8 @@ struct WalkArgs {
     GridDesc g;
     int on_target;  // spreading: kernel width from max(radius, target radius)
     uint32_t pcap;
-    uint32_t stride;        // PAD: slot entries per particle
-    const uint32_t* gate;   // FILL: run only when *gate != 0 (the PAD fallback); may be null
+    uint32_t stride;        // PAD: slot entries per particle; FILL after PAD: rows it left to fill
+    const uint32_t* gate;   // FILL after PAD: the count block (C_FALLBACK, C_ROWFIX); null otherwise
 };
 
 enum WalkMode { WALK_COUNT = 0, WALK_FILL = 1, WALK_PAD = 2 };
@@ -61,7 +61,16 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                                                      uint32_t* __restrict__ ukw) {
     constexpr bool FILL = MODE == WALK_FILL;
     constexpr bool PAD = MODE == WALK_PAD;
-    if (FILL && wa.gate && *wa.gate == 0u) return;
+    // After a PAD walk the exact fill runs for every row when the list exceeds
+    // the capacity, only for the rows that outgrew their slot otherwise, and
+    // not at all in the normal case.
+    bool only_long_rows = false;
+    if (FILL && wa.gate) {
+        if (wa.gate[C_FALLBACK] == 0u) {
+            if (wa.gate[C_ROWFIX] == 0u) return;
+            only_long_rows = true;
+        }
+    }
     const int lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t gm = walk_group_mask<L>();
@@ -72,6 +81,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
     for (uint32_t base = blockIdx.x * groups; base < n; base += gridDim.x * groups) {
         const uint32_t i = base + threadIdx.x / L;
         if (i >= n) continue;  // uniform within the group
+        if (only_long_rows && row_f[i + 1] - row_f[i] <= wa.stride) continue;
         const int4 pi = pos[i];
         const v3f p = pos_world(pi);
         const float kwi = kw[i];
@@ -218,12 +228,14 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
 // row outgrew its slot or the list exceeds the capacity.
 __global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap, uint32_t stride) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    counts[C_FALLBACK] = (counts[C_PAIRS_RAW] > pcap || counts[C_MAXROW] > stride) ? 1u : 0u;
+    counts[C_FALLBACK] = counts[C_PAIRS_RAW] > pcap ? 1u : 0u;  // every row exactly (clamped list)
+    counts[C_ROWFIX] = counts[C_MAXROW] > stride ? 1u : 0u;     // only rows longer than their slot
     counts[C_MAXROW] = 0u;
 }
 
 // Packs the PAD slots into the list at the scanned offsets (16 lanes per row,
-// coalesced); skipped when the exact path runs.
+// coalesced); rows longer than their slot are left to the exact fill, and
+// nothing is packed when the exact fill rewrites every row.
 __global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ row,
                                                      const uint32_t* __restrict__ slots, uint32_t stride,
@@ -236,6 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict_
         const uint32_t i = base + threadIdx.x / L;
         if (i >= n) continue;
         const uint32_t b = row[i], len = row[i + 1] - b;
+        if (len > stride) continue;
         const uint32_t* src = slots + (size_t)i * stride;
         for (uint32_t k = gl; k < len; k += L) col[b + k] = src[k];
     }

@@ -296,10 +296,10 @@ apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const 
  * Changing the launch shape drops the graph. */
 apbf_status apbf_scene_capture(void* scene, float dt);
 apbf_status apbf_scene_replay(void* scene, uint32_t times);
-/* Many independent scenes stepped together: apbf_batch_update enqueues one
- * substep of every scene, each on its own stream; apbf_batch_capture records
- * two substeps of every scene into one graph (fork/join across the scenes'
- * streams) that apbf_batch_replay launches. */
+/* Many independent scenes stepped together, each on a stream of its own:
+ * apbf_batch_update enqueues one substep of every scene; apbf_batch_capture
+ * captures two substeps of every scene (apbf_scene_capture) and
+ * apbf_batch_replay launches all of those graphs `times` times. */
 typedef struct apbf_batch apbf_batch;
 apbf_status apbf_batch_create(void* const* scenes, uint32_t n_scenes, apbf_batch** out_batch);
 apbf_status apbf_batch_destroy(apbf_batch* batch);
