@@ -211,6 +211,45 @@ def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: in
     return p, st
 
 
+def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: float = 0.4675,
+               kernel_width: float = 1.0, impulse_width: float = 20.0, impulse_speed: float = 4.0,
+               seed: int = 0, iterations: int = 3, res: int = 7, pairs_per_particle: int = 64):
+    """Large uniform pool, plain PBF: particles of radius kernel_width/4 on a
+    lattice at `spacing` (jitter +-0.05h) filling the box floor to the depth
+    that holds ~n_target particles (427 x 88 x 213 = 8 003 688 at the defaults,
+    depth ~41h), so a kernel-width ball holds (4/3) pi (h/s)^3 ~ 41 lattice
+    sites. A wave impulse: the slab x < impulse_width moves at impulse_speed
+    along +x. The velocity is inferred from the displacement over the last
+    substep (velocity_handling.cpp:26), so it enters as pos_backup = pos - v dt
+    with last_delta_time = dt. The hash domain is the cube [0, max(box)]^3 at
+    2^res cells per axis (cells 1.5625h at the defaults)."""
+    rng = np.random.default_rng(seed)
+    r = kernel_width / KERNEL_SCALE
+    nx = int(box[0] / spacing)
+    nz = int(box[2] / spacing)
+    ny = max(1, int(round(n_target / (nx * nz))))
+    pos = lattice((0.0, 0.0, 0.0), (nx, ny, nz), spacing, 0.05, rng)
+    n = len(pos)
+    st = make_state(pos, np.full(n, r, np.float32))
+    dt = 1.0 / 120.0
+    moving = pos[:, 0] < impulse_width
+    shift = np.zeros((n, 4), np.int32)
+    shift[moving, 0] = np.int32(math.trunc(impulse_speed * dt * POS_RESOLUTION))
+    st["pos_backup"] = st["position"] - shift
+    st["last_delta_time"] = dt
+    side = float(max(box))
+    s = dict(REFERENCE_SETTINGS)
+    s["mSmallestTargetRadius"] = r
+    p = base_params(
+        settings=s, basic_pbf=1, solver_iterations=iterations, dt=dt,
+        hash_min=(0.0, 0.0, 0.0), hash_max=(side, side, side), hash_res_log2=res,
+        boxes=walls((0, 0, 0), box, r, ceiling=True),
+        cap_particles=n, cap_transfers=16, cap_pairs=n * pairs_per_particle,
+        workload=f"large_pool_{n}",
+    )
+    return p, st
+
+
 def batch_scenes(count: int = 64, n_min: int = 10_000, n_max: int = 500_000, seed: int = 0):
     """Independent scenes for batched stepping: particle counts drawn
     log-uniformly from [n_min, n_max] (seeded), alternating a plain dam break

@@ -165,7 +165,8 @@ apbf_status apbf_batch_synchronize(apbf_batch* b) {
 }
 
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes) {
-    if (!scene || (walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) || (solve_lanes != 8 && solve_lanes != 16))
+    if (!scene || (walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) ||
+        (solve_lanes != 8 && solve_lanes != 16 && solve_lanes != 32))
         return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     s->walk_lanes = walk_lanes;
@@ -175,14 +176,14 @@ apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t
 }
 
 // Times `substeps` substeps under every launch shape (the scene advances by
-// 6 * substeps) and keeps the fastest. All shapes compute identical results.
+// 9 * substeps) and keeps the fastest. All shapes compute identical results.
 apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk, int32_t* best_solve,
                                 float* best_ms) {
     if (!scene || substeps == 0) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return APBF_ERR_HIP;
-    const int walks[3] = {4, 8, 16}, solves[2] = {8, 16};
+    const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
     float best = 1e30f;
     int bw = s->walk_lanes, bs = s->solve_lanes;
     apbf_status st = APBF_OK;

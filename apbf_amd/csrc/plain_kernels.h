@@ -42,9 +42,9 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
                  int lanes = 8, uint32_t stride = 0, const uint32_t* gate = nullptr);
-void launch_walk_decide(uint32_t* counts, uint32_t pcap, uint32_t stride, hipStream_t s);
+void launch_walk_decide(uint32_t* counts, uint32_t pcap, hipStream_t s);
 void launch_pack_rows(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* slots,
-                      uint32_t stride, uint32_t* col, hipStream_t s);
+                      uint32_t pcap, uint32_t* col, hipStream_t s);
 void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s);
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);

@@ -112,10 +112,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
     uint32_t cj[C];
     v3f cg[C];
-    auto pair1 = [&](uint32_t k, uint32_t& j_out, v3f& g_out) {
-        const uint32_t j = col[k];
-        const int4 pj = pos[j];
-        const float imN = inv_mass[j];
+    auto pair1 = [&](const int4 pj, const float imN, v3f& g_out) {
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         v3f grad;
@@ -151,20 +148,41 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
             m2 += (uint32_t)to_i32(wd.z);
             m3 += (uint32_t)to_i32(INCOMP_RES / imN);
         }
-        j_out = j;
         g_out = grad;
     };
+    // The cached part of the row goes in batches of B entries per lane: the
+    // batch's list entries, then its neighbours' positions and masses, are
+    // loaded before any arithmetic, so a lane waits for memory twice per
+    // batch instead of twice per entry. Lanes past the row end load the row's
+    // last entry and discard it.
+    constexpr int B = C < 8 ? C : 8;
 #pragma unroll
     for (int t = 0; t < C; ++t) {
-        const uint32_t k = rb + gl + (uint32_t)t * L;
         cj[t] = 0;
         cg[t] = mk(0, 0, 0);
-        if (k < re) pair1(k, cj[t], cg[t]);
+    }
+    if (rb < re) {
+        const uint32_t klast = re - 1u;
+#pragma unroll
+        for (int t0 = 0; t0 < C; t0 += B) {
+            int4 bp[B];
+            float bm[B];
+#pragma unroll
+            for (int t = 0; t < B; ++t) cj[t0 + t] = col[min(rb + gl + (uint32_t)(t0 + t) * L, klast)];
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                bp[t] = pos[cj[t0 + t]];
+                bm[t] = inv_mass[cj[t0 + t]];
+            }
+#pragma unroll
+            for (int t = 0; t < B; ++t)
+                if (rb + gl + (uint32_t)(t0 + t) * L < re) pair1(bp[t], bm[t], cg[t0 + t]);
+        }
     }
     for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
-        uint32_t j;
+        const uint32_t j = col[k];
         v3f gr;
-        pair1(k, j, gr);
+        pair1(pos[j], inv_mass[j], gr);
     }
     dens = gsum<L>(dens);
     sq = gsum<L>(sq);
@@ -226,7 +244,16 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
         const float sc = lam * POS_RES;
         const v3f empty = -normalize(vec_i(gs0, gs1, gs2));
         auto pair3 = [&](uint32_t j, v3f grad) {
-            if (do_filter && !hit && dot(empty, normalize(grad)) > 0.6f) hit = true;
+            if (do_filter && !hit) {
+                // cos(empty, grad) > 0.6 needs dot(empty, grad) > 0 and
+                // dot^2 > 0.36 |grad|^2; with a 3% margin over the rounding
+                // of either form, entries that fail the squared test cannot
+                // pass the normalised one, which then runs only for the rest
+                // (and for gradients too short for the squares to be exact).
+                const float dg = dot(empty, grad), gg = dot(grad, grad);
+                const bool maybe = !(gg >= 1e-30f) || (dg > 0.0f && dg * dg > 0.35f * gg);
+                if (maybe && dot(empty, normalize(grad)) > 0.6f) hit = true;
+            }
             if (do_shift) {
                 const v3f d = grad * sc;
                 const int32_t dx = to_i32(d.x), dy = to_i32(d.y), dz = to_i32(d.z);
@@ -589,6 +616,9 @@ void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radi
     int* ac = reinterpret_cast<int*>(acc);
     if (a.hid == 1 && a.gid == 1 && lanes == 8)  // Gaussian height and gradient, 8 lanes per row
         hipLaunchKernelGGL((k_solve_rows<8, 1, 1, 2 * SOLVE_CACHE>), dim3(row_grid(cap, 8)), dim3(BLOCK), 0, s, pos,
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
+    else if (a.hid == 1 && a.gid == 1 && lanes == 32)  // 32 lanes: rows of up to 256 entries stay cached
+        hipLaunchKernelGGL((k_solve_rows<32, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap, 32)), dim3(BLOCK), 0, s, pos,
                            inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
     else if (a.hid == 1 && a.gid == 1)  // Gaussian height and gradient (the reference's defaults)
         hipLaunchKernelGGL((k_solve_rows<ROW_LANES, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,

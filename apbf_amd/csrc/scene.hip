@@ -233,8 +233,8 @@ void Scene::build_pairs(float range_scale, bool spread) {
         scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
         launch_pairs_total(counts, row_raw, pcap, s);
         if (spread) scan_u32(u1, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
-        launch_walk_decide(counts, pcap, stride, s);
-        launch_pack_rows(counts, cap, row, col_tmp, stride, col, s);
+        launch_walk_decide(counts, pcap, s);
+        launch_pack_rows(counts, cap, row, col_tmp, pcap, col, s);
         // exact fill: every row when the list is clamped (after restarting the
         // spreading), the rows longer than their slot otherwise, else nothing
         if (spread) spread_reset_if_fallback();

@@ -11,15 +11,17 @@
 // schedule fills a bounded list: entries whose slot in the full list is at or
 // beyond the capacity are dropped (neighbor_add.glsl:22-23). PAD does both in
 // one walk: it writes every row into a fixed-stride slot of its own (stride =
-// capacity / particles) and counts it; when no row outgrew its slot and the
-// list fits the capacity (the normal case), the rows are then packed into the
-// list, and otherwise the exact COUNT-less FILL walk runs (the counts of PAD
-// are the counts of COUNT). The choice is made on the device, so the substep
+// pair capacity / current particles) and counts it; when no row outgrew its
+// slot and the list fits the capacity (the normal case), the rows are then
+// packed into the list, and otherwise the exact COUNT-less FILL walk runs
+// (the counts of PAD are the counts of COUNT). The choice is made on the device, so the substep
 // needs no host synchronisation.
 //
 // With SPREAD the walk also carries kernel_width.comp:22-56: every entry of
 // the (clamped) list spreads the particle's original kernel width to the
-// neighbour (atomicMax, order-free), and only the entries within the
+// neighbour (atomicMax, order-free; skipped when a plain load already shows a
+// value at least as large, which is exact because the stored values only grow
+// during the walk), and only the entries within the
 // particle's cutoff are written, which is the list the spreading pass of the
 // reference leaves behind (kernel_width.cpp:54-71). The full list is then
 // never materialised: COUNT counts both lists, FILL writes the short one.
@@ -34,11 +36,17 @@ struct WalkArgs {
     GridDesc g;
     int on_target;  // spreading: kernel width from max(radius, target radius)
     uint32_t pcap;
-    uint32_t stride;        // PAD: slot entries per particle; FILL after PAD: rows it left to fill
+    uint32_t stride;        // PAD, and FILL after PAD: nonzero (the slot stride is slot_stride())
     const uint32_t* gate;   // FILL after PAD: the count block (C_FALLBACK, C_ROWFIX); null otherwise
 };
 
 enum WalkMode { WALK_COUNT = 0, WALK_FILL = 1, WALK_PAD = 2 };
+
+// Entries per particle slot in the PAD layout: the pair capacity shared by the
+// current particles (not by the particle capacity), so slots are as wide as
+// the list allows. Every kernel of one pair build reads the same particle
+// count, so they agree on it.
+__device__ __forceinline__ uint32_t slot_stride(uint32_t pcap, uint32_t n) { return pcap / (n ? n : 1u); }
 
 template <int L>
 __device__ __forceinline__ uint64_t walk_group_mask() {
@@ -75,13 +83,14 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t gm = walk_group_mask<L>();
     const uint32_t n = counts[C_N];
+    const uint32_t stride = wa.stride ? slot_stride(wa.pcap, n) : 0u;
     const int gl = threadIdx.x & (L - 1);
     const uint32_t groups = BLOCK / L;
     const GridDesc& g = wa.g;
     for (uint32_t base = blockIdx.x * groups; base < n; base += gridDim.x * groups) {
         const uint32_t i = base + threadIdx.x / L;
         if (i >= n) continue;  // uniform within the group
-        if (only_long_rows && row_f[i + 1] - row_f[i] <= wa.stride) continue;
+        if (only_long_rows && row_f[i + 1] - row_f[i] <= stride) continue;
         const int4 pi = pos[i];
         const v3f p = pos_world(pi);
         const float kwi = kw[i];
@@ -103,12 +112,34 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
         // agree for every input). A NaN range keeps every candidate.
         const float t_range = (range != range) ? __builtin_inff() : sqrt_le_bound(range);
         float okw = 0.0f, t_cut = 0.0f, t_okw = 0.0f;
+        uint32_t umax = 0;
         if (SPREAD) {
             const float tr = wa.on_target ? trad[i] : 0.0f;
             okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
             t_cut = sqrt_le_bound(fmax_g(okw, kwi));
             t_okw = sqrt_le_bound(okw);
+            umax = to_u32(okw * KW_RES);
         }
+        // kernel_width.comp:41-53 for an entry at squared distance d2: the
+        // spread width is to_u32(okw * influence * 2^18) with the influence in
+        // [0, 1], so it never exceeds umax (rounding and to_u32 are monotone).
+        // The neighbour's stored value starts at its own width and only grows,
+        // so where it already reaches umax (the common case of neighbours no
+        // smaller than the particle) the entry changes nothing and the
+        // influence is not evaluated.
+        auto spread_to = [&](uint32_t j, float d2) {
+            const uint32_t cur = ukw[j];
+            if (umax <= cur) return;
+            uint32_t uval;
+            if (d2 <= t_okw) {  // distance <= okw: influence exactly 1
+                uval = umax;
+            } else {
+                const float dfk = __builtin_sqrtf(d2) - okw;
+                const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
+                uval = to_u32(okw * infl * KW_RES);
+            }
+            if (uval > cur) atomicMax(&ukw[j], uval);
+        };
         // Cell culling: a cell whose box lies farther than the range holds no
         // accepted candidate. The cell boxes are widened by margins far above
         // the rounding of the hash (grid_cell) and of the distance test, and
@@ -136,7 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             bf = SPREAD ? row_f[i] : braw;
             more = braw < wa.pcap;
         }
-        const size_t slot = PAD ? (size_t)i * wa.stride : 0;
+        const size_t slot = PAD ? (size_t)i * stride : 0;
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
             const float dz2 = axis_d2(2, z, p.z);
             for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
@@ -148,34 +179,24 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                     for (uint32_t b = s; more && b < e; b += L) {
                         const uint32_t j = b + gl;
                         bool green = false, keep = false;
-                        uint32_t uval = 0;
+                        float d2 = 0.0f;
                         if (j < e && j != i) {
                             const int4 pj = pos[j];
                             const v3f d = p - pos_world(pj);
                             green = !(dot(d, d) > t_range);
                             if (SPREAD && green) {
                                 const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-                                const float d2 = dot(diff, diff);
+                                d2 = dot(diff, diff);
                                 keep = d2 <= t_cut;
-                                if (FILL || PAD) {
-                                    if (d2 <= t_okw) {  // distance <= okw: influence exactly 1
-                                        uval = to_u32(okw * KW_RES);
-                                    } else {
-                                        const float dfk = __builtin_sqrtf(d2) - okw;
-                                        const float infl =
-                                            fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
-                                        uval = to_u32(okw * infl * KW_RES);
-                                    }
-                                }
                             }
                         }
                         const uint64_t rbal = __ballot(green) & gm;
                         if (PAD) {
-                            if (SPREAD && green && uval) atomicMax(&ukw[j], uval);
+                            if (SPREAD && green) spread_to(j, d2);
                             const bool kept = SPREAD ? (green && keep) : green;
                             const uint64_t kbal = SPREAD ? (__ballot(kept) & gm) : rbal;
                             const uint32_t k = cf + (uint32_t)__popcll(kbal & below);
-                            if (kept && k < wa.stride) col[slot + k] = j;
+                            if (kept && k < stride) col[slot + k] = j;
                             cf += (uint32_t)__popcll(kbal);
                             craw += (uint32_t)__popcll(rbal);
                         } else if (!FILL) {
@@ -185,7 +206,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                             const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
                             const bool incl = green && ridx < wa.pcap;
                             if (SPREAD) {
-                                if (incl && uval) atomicMax(&ukw[j], uval);
+                                if (incl) spread_to(j, d2);
                                 const bool kept = incl && keep;
                                 const uint64_t fbal = __ballot(kept) & gm;
                                 if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
@@ -207,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             if (PAD) {
                 cnt_raw[i] = craw;
                 if (SPREAD) cnt_f[i] = cf;
-                if (cf > wa.stride) atomicMax(&counts[C_MAXROW], cf);
+                if (cf > stride) atomicMax(&counts[C_MAXROW], cf);
             } else if (!FILL) {
                 cnt_raw[i] = craw;
                 if (SPREAD) cnt_f[i] = cf;
@@ -226,8 +247,9 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
 
 // After a PAD walk and the scan of its raw counts: take the exact path when a
 // row outgrew its slot or the list exceeds the capacity.
-__global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap, uint32_t stride) {
+__global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t stride = slot_stride(pcap, counts[C_N]);
     counts[C_FALLBACK] = counts[C_PAIRS_RAW] > pcap ? 1u : 0u;  // every row exactly (clamped list)
     counts[C_ROWFIX] = counts[C_MAXROW] > stride ? 1u : 0u;     // only rows longer than their slot
     counts[C_MAXROW] = 0u;
@@ -238,11 +260,12 @@ __global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap, uint
 // nothing is packed when the exact fill rewrites every row.
 __global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ row,
-                                                     const uint32_t* __restrict__ slots, uint32_t stride,
+                                                     const uint32_t* __restrict__ slots, uint32_t pcap,
                                                      uint32_t* __restrict__ col) {
     if (counts[C_FALLBACK]) return;
     constexpr int L = 16;
     const uint32_t n = counts[C_N];
+    const uint32_t stride = slot_stride(pcap, n);
     const int gl = threadIdx.x & (L - 1);
     for (uint32_t base = blockIdx.x * (BLOCK / L); base < n; base += gridDim.x * (BLOCK / L)) {
         const uint32_t i = base + threadIdx.x / L;
@@ -317,13 +340,13 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
 #undef WALK
 }
 
-void launch_walk_decide(uint32_t* counts, uint32_t pcap, uint32_t stride, hipStream_t s) {
-    hipLaunchKernelGGL(k_walk_decide, dim3(1), dim3(64), 0, s, counts, pcap, stride);
+void launch_walk_decide(uint32_t* counts, uint32_t pcap, hipStream_t s) {
+    hipLaunchKernelGGL(k_walk_decide, dim3(1), dim3(64), 0, s, counts, pcap);
 }
 void launch_pack_rows(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* slots,
-                      uint32_t stride, uint32_t* col, hipStream_t s) {
+                      uint32_t pcap, uint32_t* col, hipStream_t s) {
     uint32_t g = div_up(cap ? cap : 1, BLOCK / 16);
-    hipLaunchKernelGGL(k_pack_rows, dim3(g < 8192u ? g : 8192u), dim3(BLOCK), 0, s, counts, row, slots, stride, col);
+    hipLaunchKernelGGL(k_pack_rows, dim3(g < 8192u ? g : 8192u), dim3(BLOCK), 0, s, counts, row, slots, pcap, col);
 }
 
 void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s) {

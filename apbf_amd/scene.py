@@ -149,7 +149,7 @@ class Scene:
 
     def autotune(self, dt: float, substeps: int = 3):
         """Fastest (walk lanes, solver lanes, ms per substep); the scene advances
-        by 6 * substeps."""
+        by 9 * substeps."""
         w, s, ms = C.c_int32(), C.c_int32(), C.c_float()
         check(self._L.apbf_scene_autotune(self._h, float(dt), int(substeps), C.byref(w), C.byref(s), C.byref(ms)),
               "autotune")

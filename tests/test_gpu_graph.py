@@ -67,7 +67,7 @@ def test_batch_graph_matches_individual_scenes(device):
     batch.close()
 
 
-@pytest.mark.parametrize("walk,solve", [(4, 8), (4, 16), (8, 8), (16, 8), (16, 16)])
+@pytest.mark.parametrize("walk,solve", [(4, 8), (4, 16), (8, 8), (8, 32), (16, 8), (16, 16)])
 def test_launch_shapes_match_the_oracle(device, walk, solve):
     from oracle.oracle import OracleScene
     p, st = cfg.small_water_drop(seed=4, active=True)
@@ -87,8 +87,8 @@ def test_autotune_keeps_results(device):
     dt = p["dt"]
     tuned, plain = _scene(p, st), _scene(p, st)
     w, s, ms = tuned.autotune(dt, substeps=2)
-    assert w in (4, 8, 16) and s in (8, 16) and ms > 0
-    for _ in range(12):  # the autotuner advanced the scene by 6 x 2 substeps
+    assert w in (4, 8, 16) and s in (8, 16, 32) and ms > 0
+    for _ in range(18):  # the autotuner advanced the scene by 9 x 2 substeps
         plain.update(dt)
     _same(tuned, plain, "after autotune")
     tuned.update(dt)
