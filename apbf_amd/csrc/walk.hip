@@ -168,19 +168,50 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
             more = braw < wa.pcap;
         }
         const size_t slot = PAD ? (size_t)i * stride : 0;
+        // The cells of one x-line of the search box are taken L at a time:
+        // lane c of the group looks up cell lo.x + c0 + c, a scan over the
+        // group gives each cell's offset in the line's candidates taken in
+        // cell order, and every step then tests the next L of those
+        // candidates, so steps run full across cell ends. A lane finds the
+        // cell of its candidate by a binary search over the offsets.
+        const uint32_t hmask = (g.res >= 32u) ? 0xFFFFFFFFu : ((1u << g.res) - 1u);
+        const uint32_t nline = hi[0] >= lo[0] ? hi[0] - lo[0] + 1u : 0u;
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
             const float dz2 = axis_d2(2, z, p.z);
             for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
                 const float dyz2 = dz2 + axis_d2(1, y, p.y);
-                for (uint32_t x = lo[0]; more && x <= hi[0] && dyz2 <= thr; ++x) {
-                    const bool skip = dyz2 + axis_d2(0, x, p.x) > thr;
-                    const uint32_t h = zhash(x, y, z, g.res);
-                    const uint32_t s = skip ? 0u : cs[h], e = skip ? 0u : ce[h];
-                    for (uint32_t b = s; more && b < e; b += L) {
-                        const uint32_t j = b + gl;
+                const uint32_t hyz = (spread_bits3(y & hmask) << 1) | (spread_bits3(z & hmask) << 2);
+                for (uint32_t c0 = 0; more && c0 < nline && dyz2 <= thr; c0 += L) {
+                    const uint32_t nc = min(nline - c0, (uint32_t)L);
+                    uint32_t cstart = 0, clen = 0;
+                    if ((uint32_t)gl < nc) {
+                        const uint32_t x = lo[0] + c0 + (uint32_t)gl;
+                        if (!(dyz2 + axis_d2(0, x, p.x) > thr)) {
+                            const uint32_t h = spread_bits3(x & hmask) | hyz;
+                            cstart = cs[h];
+                            const uint32_t cend = ce[h];
+                            clen = cend > cstart ? cend - cstart : 0u;
+                        }
+                    }
+                    uint32_t incl = clen;
+#pragma unroll
+                    for (int m = 1; m < L; m <<= 1) {
+                        const uint32_t v = (uint32_t)__shfl_up((int)incl, m, L);
+                        if (gl >= m) incl += v;
+                    }
+                    const uint32_t total = (uint32_t)__shfl((int)incl, L - 1, L);
+                    const uint32_t off = incl - clen;  // lanes past the last cell hold total
+                    const uint32_t delta = cstart - off;
+                    for (uint32_t t0 = 0; more && t0 < total; t0 += L) {
+                        const uint32_t t = t0 + (uint32_t)gl;
+                        uint32_t c = 0;  // last cell whose offset is <= t
+#pragma unroll
+                        for (int stp = L / 2; stp >= 1; stp >>= 1)
+                            if ((uint32_t)__shfl((int)off, (int)c + stp, L) <= t) c += (uint32_t)stp;
+                        const uint32_t j = t + (uint32_t)__shfl((int)delta, (int)c, L);
                         bool green = false, keep = false;
                         float d2 = 0.0f;
-                        if (j < e && j != i) {
+                        if (t < total && j != i) {
                             const int4 pj = pos[j];
                             const v3f d = p - pos_world(pj);
                             green = !(dot(d, d) > t_range);
@@ -204,21 +235,20 @@ __global__ __launch_bounds__(BLOCK) void k_walk_rows(const int4* __restrict__ po
                             if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep) & gm);
                         } else {
                             const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
-                            const bool incl = green && ridx < wa.pcap;
+                            const bool incl_r = green && ridx < wa.pcap;
                             if (SPREAD) {
-                                if (incl) spread_to(j, d2);
-                                const bool kept = incl && keep;
+                                if (incl_r) spread_to(j, d2);
+                                const bool kept = incl_r && keep;
                                 const uint64_t fbal = __ballot(kept) & gm;
                                 if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
                                 cf += (uint32_t)__popcll(fbal);
-                            } else if (incl) {
+                            } else if (incl_r) {
                                 col[ridx] = j;
                             }
                             craw += (uint32_t)__popcll(rbal);
                             more = braw + craw < wa.pcap;  // later entries fall beyond the capacity
                         }
                     }
-                    if (x == 0xFFFFFFFFu) break;
                 }
                 if (y == 0xFFFFFFFFu) break;
             }
