@@ -150,6 +150,7 @@ PROTOTYPES = {
     "apbf_scene_harvest": (_i32, [_vp, _vp, _vp, _u32, _pu]),
     "apbf_version": (C.c_char_p, []),
     "apbf_device_count": (_i32, []),
+    "apbf_check_conversions": (_i32, [_u32, C.POINTER(C.c_uint64), _pu, _vp, _vp]),
 }
 
 

@@ -325,4 +325,61 @@ void copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s) {
     hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, s, dst, src);
 }
 
+// Every float bit pattern through the hardware conversions (to_i32 / to_u32)
+// and through their definitions (to_i32_ref / to_u32_ref): the number of
+// patterns where they differ, the smallest such pattern, and the hardware
+// results of every `stride`-th pattern.
+__global__ __launch_bounds__(BLOCK) void k_check_conversions(uint32_t stride, unsigned long long* mismatches,
+                                                             uint32_t* first_bad, int32_t* sample_i,
+                                                             uint32_t* sample_u) {
+    unsigned long long bad = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (1ull << 32);
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)k;
+        const float x = bits_f(b);
+        const int32_t hi = to_i32(x);
+        const uint32_t hu = to_u32(x);
+        if (hi != to_i32_ref(x) || hu != to_u32_ref(x)) {
+            ++bad;
+            atomicMin(first_bad, b);
+        }
+        if (b % stride == 0u) {
+            sample_i[b / stride] = hi;
+            sample_u[b / stride] = hu;
+        }
+    }
+    if (bad) atomicAdd(mismatches, bad);
+}
+
 }  // namespace apbf
+
+extern "C" apbf_status apbf_check_conversions(uint32_t stride, uint64_t* mismatches, uint32_t* first_bad,
+                                              int32_t* sample_i32, uint32_t* sample_u32) {
+    using namespace apbf;
+    if (!mismatches || !first_bad || !sample_i32 || !sample_u32 || stride < 1024u) return APBF_ERR_INVALID_ARGUMENT;
+    const uint64_t samples = ((1ull << 32) - 1) / stride + 1;
+    void* mem = nullptr;
+    const size_t bytes = 16 + samples * 8;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return APBF_ERR_HIP;
+    unsigned long long* d_bad = static_cast<unsigned long long*>(mem);
+    uint32_t* d_first = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + 8);
+    int32_t* d_si = reinterpret_cast<int32_t*>(static_cast<char*>(mem) + 16);
+    uint32_t* d_su = reinterpret_cast<uint32_t*>(d_si + samples);
+    const uint32_t init[4] = {0u, 0u, 0xFFFFFFFFu, 0u};
+    apbf_status st = APBF_OK;
+    if (hipMemcpy(mem, init, 16, hipMemcpyHostToDevice) != hipSuccess) st = APBF_ERR_HIP;
+    if (st == APBF_OK) {
+        hipLaunchKernelGGL(k_check_conversions, dim3(4096), dim3(BLOCK), 0, nullptr, stride, d_bad, d_first, d_si,
+                           d_su);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) st = APBF_ERR_HIP;
+    }
+    unsigned long long bad = 0;
+    if (st == APBF_OK && (hipMemcpy(&bad, d_bad, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                          hipMemcpy(first_bad, d_first, 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                          hipMemcpy(sample_i32, d_si, samples * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                          hipMemcpy(sample_u32, d_su, samples * 4, hipMemcpyDeviceToHost) != hipSuccess))
+        st = APBF_ERR_HIP;
+    *mismatches = bad;
+    (void)hipFree(mem);
+    return st;
+}

@@ -361,6 +361,16 @@ apbf_status apbf_scene_harvest(void* scene, float* features, float* targets, uin
 const char* apbf_version(void);
 /* Number of HIP devices visible (0 without a GPU); never fails. */
 int32_t apbf_device_count(void);
+/* Self-check of the device float->int conversions (truncation toward zero,
+ * saturating, NaN -> 0; the GLSL conversions the reference relies on, e.g.
+ * incompressibility_1.comp:52-58): runs all 2^32 float bit patterns through
+ * the hardware conversion the kernels use and through its definition.
+ * *mismatches = patterns where they differ (*first_bad the smallest);
+ * sample_i32 / sample_u32 (host, ceil(2^32 / stride) entries each) receive the
+ * device results of every stride-th pattern. stride >= 1024. No counterpart
+ * in the reference. */
+apbf_status apbf_check_conversions(uint32_t stride, uint64_t* mismatches, uint32_t* first_bad,
+                                   int32_t* sample_i32, uint32_t* sample_u32);
 
 #ifdef __cplusplus
 }
