@@ -444,13 +444,10 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
     }
     uint32_t dens = 0;
     v3f unused;
-    for (uint32_t k = rb + gl; k < re; k += L) {
-        const uint32_t j = col[k];
-        const int4 pj = pos[j];
+    auto pair = [&](const int4 pj, const float imN) {
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
-        const float imN = inv_mass[j];
         const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
         const uint32_t ie = (ib >> 23) & 0xFFu;
         float wq;
@@ -459,6 +456,26 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
         else
             wq = w / imN;
         dens += to_u32(wq * INCOMP_RES);
+    };
+    // batches of four entries per lane, loads first (as in k_solve_rows)
+    uint32_t k = rb + gl;
+    for (; k + 3u * L < re; k += 4u * L) {
+        uint32_t jb[4];
+        int4 pb[4];
+        float mb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) jb[t] = col[k + (uint32_t)t * L];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            pb[t] = pos[jb[t]];
+            mb[t] = inv_mass[jb[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) pair(pb[t], mb[t]);
+    }
+    for (; k < re; k += L) {
+        const uint32_t j = col[k];
+        pair(pos[j], inv_mass[j]);
     }
     dens = gsum<L>(dens);
     if (active && gl == 0) {
@@ -510,17 +527,34 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_rows(const int4* __restrict__ p
     }
     uint32_t bmin = 0xFFFFFFFFu, best = 0xFFFFFFFFu, best_k = 0u;
     bool found = false;
-    for (uint32_t k = rb + gl; k < re; k += L) {
-        const uint32_t j = col[k];
-        const int4 pj = pos[j];
+    auto entry = [&](uint32_t k, const int4 pj, uint32_t obd) {
         const uint32_t dist = to_u32(length(vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z))));
-        const uint32_t cand = old_bd[j] + dist;
+        const uint32_t cand = obd + dist;
         bmin = cand < bmin ? cand : bmin;
         if (!found || dist <= best) {  // later entries of the lane win ties
             best = dist;
             best_k = k;
             found = true;
         }
+    };
+    // batches of four entries per lane, loads first (as in k_solve_rows)
+    uint32_t k = rb + gl;
+    for (; k + 3u * L < re; k += 4u * L) {
+        uint32_t jb[4], ob[4];
+        int4 pb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) jb[t] = col[k + (uint32_t)t * L];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            pb[t] = pos[jb[t]];
+            ob[t] = old_bd[jb[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) entry(k + (uint32_t)t * L, pb[t], ob[t]);
+    }
+    for (; k < re; k += L) {
+        const uint32_t j = col[k];
+        entry(k, pos[j], old_bd[j]);
     }
     bmin = gmin<L>(bmin);
     // group arg-min of (distance, -position): smallest distance, then the last position
