@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--learned-iterations", type=int, default=3,
                     help="Jacobi iterations after the learned warm start")
+    ap.add_argument("--hash-res", type=int, default=None,
+                    help="override the workload's neighbour grid resolution (cells per axis = 2^res)")
     ap.add_argument("--launch-shape", default=None,
                     help="walk lanes,solver lanes (e.g. 8,16); every shape computes identical results")
     ap.add_argument("--autotune", action="store_true",
@@ -264,6 +266,8 @@ def main():
     params, state = workload(args.workload)
     if args.iterations is not None:
         params["solver_iterations"] = args.iterations
+    if args.hash_res is not None:
+        params["hash_res_log2"] = args.hash_res
     dt = params["dt"]
     torch.cuda.init()
     scene = Scene(params)
