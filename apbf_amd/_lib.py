@@ -120,6 +120,7 @@ PROTOTYPES = {
     "apbf_find_value_ranges": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "apbf_neighborhood_green": (_i32, [_vp, _vp, _vp, _u32, _f32, _pf, _pf, _u32, _vp, _vp, _vp, _vp,
                                        _u32, _vp, _vp, _sz, _vp]),
+    "apbf_neighborhood_brute_force": (_i32, [_vp, _vp, _vp, _u32, _f32, _vp, _vp, _u32, _vp, _vp, _sz, _vp]),
     "apbf_copy_scattered_read": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "apbf_hidden_edit_workspace_size": (_sz, [_u32, _u32, _u32]),
     "apbf_apply_hidden_edit": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
@@ -150,6 +151,7 @@ PROTOTYPES = {
     "apbf_scene_harvest": (_i32, [_vp, _vp, _vp, _u32, _pu]),
     "apbf_version": (C.c_char_p, []),
     "apbf_device_count": (_i32, []),
+    "apbf_scene_set_sphere": (_i32, [_vp, _vp, _f32, _i32, _i32]),
     "apbf_check_conversions": (_i32, [_u32, C.POINTER(C.c_uint64), _pu, _vp, _vp]),
 }
 

@@ -211,6 +211,124 @@ def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: in
     return p, st
 
 
+def box_shape(mn, mx, radius: float):
+    """initialize::add_box_shape (initialize.cpp:5-33, initialize_box.comp):
+    floor((max - min) / 2r) particles per axis, centred at min + r + 2r * index."""
+    mn = np.asarray(mn, np.float64)
+    counts = np.floor((np.asarray(mx, np.float64) - mn) / (2.0 * radius)).astype(int)
+    g = np.stack(np.meshgrid(*[np.arange(c) for c in counts], indexing="ij"), -1).reshape(-1, 3)
+    return mn + radius + 2.0 * radius * g
+
+
+def _sphere_shells(center, radius: float, particle_radius: float, dims: int = 3):
+    """initialize::append_sphere (initialize.cpp:71-100): the points of a sphere
+    surface, recursively as rings of circles of the next lower dimension."""
+    c = np.asarray(center, np.float64)
+    if dims <= 1:
+        if radius < particle_radius:
+            return [c.copy()]
+        off = np.array([radius, 0.0, 0.0])
+        return [c - off, c + off]
+    d = dims - 1
+    pts = _sphere_shells(c, radius, particle_radius, d)
+    count = math.floor(radius * math.pi / (2.0 * particle_radius))
+    if count <= 0:
+        return pts
+    da = math.pi / count
+    angle = 0.0
+    for _ in range(int(count) // 2):
+        angle += da
+        layer_off = math.sin(angle) * radius
+        layer_r = math.cos(angle) * radius
+        lc = c.copy()
+        lc[d] += layer_off
+        pts += _sphere_shells(lc, layer_r, particle_radius, d)
+        lc[d] -= 2 * layer_off
+        pts += _sphere_shells(lc, layer_r, particle_radius, d)
+    return pts
+
+
+def sphere_shape(center, outer_radius: float, particle_radius: float):
+    """initialize::add_sphere_shape(center, outer, 0, true, r) (initialize.cpp:35-69):
+    concentric shells from the outside in, each shell's radius chosen so the
+    particles placed so far fill the volume between it and the outer surface."""
+    a = outer_radius ** 3
+    b = -(particle_radius ** 3) * 2 * 3 / math.pi
+    shell = outer_radius - particle_radius
+    out = []
+    while 0.0 <= shell <= outer_radius - particle_radius:
+        out += _sphere_shells(center, shell, particle_radius)
+        if shell < particle_radius:
+            break  # the centre particle is placed
+        v = a + b * len(out)
+        shell = (v ** (1.0 / 3.0) if v >= 0 else math.nan) - particle_radius  # pow of a negative: NaN, the loop ends
+    return np.array(out, np.float64)
+
+
+def _reference_scene(pos, r, res, hash_min, hash_max, boxes, name, cap_pairs_per_particle=64):
+    n = len(pos)
+    st = make_state(pos, np.full(n, r, np.float32))
+    s = dict(REFERENCE_SETTINGS)
+    s["mSmallestTargetRadius"] = r
+    p = base_params(
+        settings=s, basic_pbf=0, solver_iterations=3, gravity=(0.0, -10.0, 0.0),
+        hash_min=tuple(float(x) for x in hash_min), hash_max=tuple(float(x) for x in hash_max), hash_res_log2=res,
+        boxes=boxes, cap_particles=max(2 * n, 16), cap_transfers=max(n, 16),
+        cap_pairs=max(2 * n, 16) * cap_pairs_per_particle, workload=f"{name}_{n}",
+    )
+    return p, st
+
+
+def pool_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4):
+    """The reference's pool (pool.cpp:6-56 as main.cpp:62 builds it): a box of
+    particles filling [mn, mx], six walls 4 units thick around it, gravity
+    (0, -10, 0), the hash domain [mn, mx] at 2^4 cells per axis, the reference
+    settings (adaptive, boundary-distance kernel widths)."""
+    pos = box_shape(mn, mx, radius)
+    return _reference_scene(pos, radius, res, mn, mx, walls(mn, mx, radius, ceiling=True), "pool")
+
+
+def waterfall_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4):
+    """The reference's waterfall (waterfall.cpp:6-67): the pool's particles and
+    an upper basin whose x-max side is open, over a lower basin shifted by
+    (min - max) in x and y; eleven wall boxes in 3D."""
+    pos = box_shape(mn, mx, radius)
+    t = 4.0
+    w0 = np.asarray(mn, np.float64) - t
+    w1 = np.asarray(mx, np.float64) + t + np.array([0.0, 2 * radius, 0.0])
+    shift = w0 - w1
+    shift[2] = 0.0
+    b = [(w0, np.array([w0[0] + t, w1[1], w1[2]])),
+         (w0, np.array([w1[0], w0[1] + t, w1[2]])),
+         (np.array([w1[0] - t, w0[1], w0[2]]), w1),
+         (np.array([w0[0], w1[1] - t, w0[2]]), w1),
+         (shift + w0, np.array([w0[0] + t, w1[1], w1[2]]) + shift),
+         (shift + w0, np.array([w1[0], w0[1] + t, w1[2]]) + shift),
+         (shift + np.array([w1[0] - t, w0[1], w0[2]]), w1 + shift),
+         (w0, np.array([w1[0], w1[1], w0[2] + t])),
+         (np.array([w0[0], w0[1], w1[2] - t]), w1),
+         (shift + w0, np.array([w1[0], w1[1], w0[2] + t]) + shift),
+         (shift + np.array([w0[0], w0[1], w1[2] - t]), w1 + shift)]
+    boxes = [(tuple(float(x) for x in lo), tuple(float(x) for x in hi)) for lo, hi in b]
+    return _reference_scene(pos, radius, res, mn, mx, boxes, "waterfall")
+
+
+def spherical_pool_scene(center=(0.0, 10.0, -60.0), pool_radius: float = 100.0, particle_radius: float = 1.0,
+                         res: int = 4, max_particles: int | None = None):
+    """The reference's spherical pool (spherical_pool.cpp:6-45): a ball of
+    particles on concentric shells inside a hollow sphere collider of the pool
+    radius, gravity (0, -10, 0), the hash domain center +- pool radius at 2^4
+    cells per axis. max_particles keeps the first particles (the reference's
+    100 000-particle capacity clamps a 3D ball of radius 100)."""
+    pos = sphere_shape(center, pool_radius, particle_radius)
+    if max_particles is not None:
+        pos = pos[:max_particles]
+    c = np.asarray(center, np.float64)
+    p, st = _reference_scene(pos, particle_radius, res, c - pool_radius, c + pool_radius, [], "spherical_pool")
+    p["sphere"] = (tuple(float(x) for x in center), float(pool_radius), True)
+    return p, st
+
+
 def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: float = 0.4675,
                kernel_width: float = 1.0, impulse_width: float = 20.0, impulse_speed: float = 4.0,
                seed: int = 0, iterations: int = 3, res: int = 7, pairs_per_particle: int = 64):

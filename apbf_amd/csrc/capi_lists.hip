@@ -180,6 +180,29 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
     return last_status();
 }
 
+apbf_status apbf_neighborhood_brute_force(const int32_t* position, const float* range, const uint32_t* d_len,
+                                          uint32_t capacity, float range_scale, uint32_t* row_offset,
+                                          uint32_t* neighbors, uint32_t pair_capacity, uint32_t* d_n_pairs, void* ws,
+                                          size_t ws_bytes, apbf_stream stream) {
+    if (((!position || !range) && capacity) || !d_len || !row_offset || !neighbors || !d_n_pairs || !ws)
+        return APBF_ERR_INVALID_ARGUMENT;  // an empty list (capacity 0) may come without particle arrays
+    if (ws_bytes < apbf_neighborhood_green_workspace_size(capacity)) return APBF_ERR_WORKSPACE;
+    hipStream_t s = S(stream);
+    uint32_t* counts = static_cast<uint32_t*>(ws);
+    uint32_t* cnt = counts + C_COUNT;
+    void* scan_ws = cnt + capacity + 1;
+    fill_u32(counts, 0u, C_COUNT, s);
+    hipMemcpyAsync(counts + C_N, d_len, 4, hipMemcpyDeviceToDevice, s);
+    launch_set_n_plus_1(counts, s);
+    const int4* p4 = reinterpret_cast<const int4*>(position);
+    launch_brute(false, p4, range, counts, capacity, range_scale, pair_capacity, cnt, nullptr, nullptr, s);
+    scan_u32(cnt, row_offset, counts + C_N_PLUS_1, capacity + 1, false, nullptr, scan_ws, s);
+    launch_pairs_total(counts, row_offset, pair_capacity, s);
+    launch_brute(true, p4, range, counts, capacity, range_scale, pair_capacity, nullptr, row_offset, neighbors, s);
+    hipMemcpyAsync(d_n_pairs, counts + C_PAIRS, 4, hipMemcpyDeviceToDevice, s);
+    return last_status();
+}
+
 apbf_status apbf_copy_scattered_read(const void* src, void* dst, const uint32_t* edit,
                                      const uint32_t* d_edit_len, uint32_t capacity, uint32_t stride,
                                      apbf_stream stream) {

@@ -43,6 +43,9 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
                  int lanes = 8, uint32_t stride = 0, const uint32_t* gate = nullptr);
 void launch_walk_decide(uint32_t* counts, uint32_t pcap, hipStream_t s);
+void launch_brute(bool fill, const int4* pos, const float* kw, const uint32_t* counts, uint32_t cap,
+                  float range_scale, uint32_t pcap, uint32_t* cnt, const uint32_t* row, uint32_t* col,
+                  hipStream_t s);
 void launch_pack_rows(const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* slots,
                       uint32_t pcap, uint32_t* col, hipStream_t s);
 void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap, uint32_t* row_f, hipStream_t s);
@@ -51,6 +54,8 @@ void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
 void launch_kernel_consts(const float* kw, const float* radius, const float* inv_mass, const uint32_t* counts,
                           uint32_t cap, int hid, int gid, float4* gc,
                           hipStream_t s);
+void launch_sphere(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap, const BoxSet& c,
+                   hipStream_t s);
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s);
 void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s);

@@ -182,6 +182,12 @@ __global__ __launch_bounds__(BLOCK) void k_box(int4* __restrict__ pos, const flo
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) pos[i] = box_collide(i, pos[i], radius[i], boxes);
 }
+// sphere_collision::apply (sphere_collision.cpp): one pass over the particles.
+__global__ __launch_bounds__(BLOCK) void k_sphere(int4* __restrict__ pos, const float* __restrict__ radius,
+                                                  const uint32_t* __restrict__ counts, BoxSet c) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) pos[i] = sphere_collide(i, pos[i], radius[i], c);
+}
 
 // Adds one substep's mean and max compression error to the running statistics
 // (stats[0] += mean, stats[1] = max(stats[1], max)); stats[2] keeps the last mean.
@@ -274,6 +280,10 @@ void launch_kernel_consts(const float* kw, const float* radius, const float* inv
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s) {
     LAUNCH(k_box, stride_grid(cap), pos, radius, counts, boxes);
+}
+void launch_sphere(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap, const BoxSet& c,
+                   hipStream_t s) {
+    LAUNCH(k_sphere, stride_grid(cap), pos, radius, counts, c);
 }
 void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s) {
     hipLaunchKernelGGL(k_count_particle_substeps, dim3(1), dim3(64), 0, s, counts, acc);

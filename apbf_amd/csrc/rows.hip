@@ -386,7 +386,8 @@ __global__ __launch_bounds__(BLOCK) void k_features_rows(const int4* __restrict_
 }
 
 // Applies the accumulated shifts, clears the accumulator, and (when `boxes`
-// holds walls) runs the box collision of the next solver iteration.
+// holds colliders) runs the box and sphere collision of the next solver
+// iteration.
 __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, int4* __restrict__ acc,
                                                        const float* __restrict__ radius,
                                                        const uint32_t* __restrict__ counts, BoxSet boxes) {
@@ -399,6 +400,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, i
         p.z = iadd(p.z, d.z);
         acc[j] = make_int4(0, 0, 0, 0);
         if (boxes.n) p = box_collide(j, p, radius[j], boxes);
+        if (boxes.sphere) p = sphere_collide(j, p, radius[j], boxes);
         pos[j] = p;
     }
 }

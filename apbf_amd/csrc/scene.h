@@ -129,7 +129,9 @@ struct Scene {
     void kernel_width_from_bd();
     void neighborhood_green(float range_scale, bool spread);
     void neighborhood_bs(float range_scale);  // neighborhood_binary_search (bsearch.hip)
-    int nb_method = 0;                        // 0: neighborhood_green, 1: neighborhood_binary_search
+    void neighborhood_bf(float range_scale);  // neighborhood_brute_force
+    // 0: neighborhood_green, 1: neighborhood_binary_search, 2: neighborhood_brute_force
+    int nb_method = 0;
     void spread_kernel_width();
     void box_collision();
     void incompressibility();
@@ -146,6 +148,8 @@ struct Scene {
     void prepare_solver();
     SolverArgs solver_args() const;
     void incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override = nullptr);
+    void sphere_collision();
+    void colliders();
     void solver_loop();
     void remap_transfers_after_permute();
     void accumulate_density_error();

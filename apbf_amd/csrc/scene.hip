@@ -246,8 +246,35 @@ void Scene::build_pairs(float range_scale, bool spread) {
     pairs_valid = true;
 }
 
+// neighborhood_brute_force.cpp:19-42: every particle against every other, in
+// the current particle order (no sort, no permutation).
+void Scene::neighborhood_bf(float range_scale) {
+    hipStream_t s = stream;
+    launch_set_n_plus_1(counts, s);
+    bool tk = tbegin(K_WALK_COUNT);
+    launch_brute(false, pos[cur], kernel_width[cur], counts, cap, range_scale, pcap, cnt, nullptr, nullptr, s);
+    tend(tk);
+    scan_u32(cnt, row, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
+    launch_pairs_total(counts, row, pcap, s);
+    tk = tbegin(K_WALK_FILL);
+    launch_brute(true, pos[cur], kernel_width[cur], counts, cap, range_scale, pcap, nullptr, row, col, s);
+    tend(tk);
+    pairs_valid = true;
+}
+
 // --------------------------------------------------------- box_collision
 void Scene::box_collision() { launch_box(pos[cur], radius[cur], counts, cap, boxes, stream); }
+// sphere_collision::apply (spherical_pool.cpp:86-89 runs it in place of the
+// box collision)
+void Scene::sphere_collision() {
+    if (boxes.sphere) launch_sphere(pos[cur], radius[cur], counts, cap, boxes, stream);
+}
+// The colliders that open a solver iteration: the boxes (when the scene has
+// any), then the sphere (when set).
+void Scene::colliders() {
+    if (boxes.n) box_collision();
+    sphere_collision();
+}
 
 // ----------------------------------------------------- incompressibility
 // incompressibility.cpp:12-45: one k_solve_rows pass (sums, lambda,
@@ -300,7 +327,7 @@ void Scene::incompressibility() {
 void Scene::solver_loop() {
     if (P.solver_iterations <= 0) return;
     prepare_solver();
-    box_collision();
+    colliders();
     if (ws_mode != 0) {
         // learned warm start: features of the state the solver starts from, then
         // (mode 2) one shift pass with the predicted lambdas before the iterations
@@ -330,8 +357,11 @@ void Scene::update(float dt) {
     if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_bd();
     const bool spread = !P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance;
     const float scale = range_scale_override >= 0.0f ? range_scale_override : (spread ? 1.5f : 1.0f);
-    if (nb_method == 1) {
-        neighborhood_bs(scale);
+    if (nb_method == 1 || nb_method == 2) {
+        if (nb_method == 1)
+            neighborhood_bs(scale);
+        else
+            neighborhood_bf(scale);
         if (spread) spread_kernel_width();
     } else {
         neighborhood_green(scale, spread);  // spreading fused into the walk
@@ -363,12 +393,15 @@ apbf_status Scene::stage(int st, float dt) {
                                                                                         : 1.5f;
             if (nb_method == 1)
                 neighborhood_bs(scale);
+            else if (nb_method == 2)
+                neighborhood_bf(scale);
             else
                 neighborhood_green(scale, false);
             break;
         }
         case APBF_STAGE_SPREAD: spread_kernel_width(); break;
         case APBF_STAGE_BOX: box_collision(); break;
+        case APBF_STAGE_SPHERE: sphere_collision(); break;
         case APBF_STAGE_INCOMPRESSIBILITY: incompressibility(); break;
         case APBF_STAGE_UPDATE_TRANSFERS: update_transfers(); break;
         default: return APBF_ERR_INVALID_ARGUMENT;
@@ -542,9 +575,20 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
 }
 
 apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method) {
-    if (!scene || method < 0 || method > 1) return APBF_ERR_INVALID_ARGUMENT;
+    if (!scene || method < 0 || method > 2) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     s->nb_method = method;
+    scene_drop_graph(s);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_sphere(void* scene, const float* center3, float radius, int32_t hollow, int32_t enable) {
+    if (!scene || (enable && !center3)) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->boxes.sphere = enable ? 1u : 0u;
+    s->boxes.hollow = hollow ? 1 : 0;
+    for (int d = 0; d < 3; ++d) s->boxes.sc[d] = enable ? center3[d] : 0.0f;
+    s->boxes.sr = enable ? radius : 0.0f;
     scene_drop_graph(s);
     return APBF_OK;
 }

@@ -40,10 +40,16 @@ struct GridDesc {
     uint32_t res;
 };
 
+// The scene's colliders: up to APBF_MAX_BOXES boxes (box_collision) and one
+// sphere (sphere_collision); `sphere` = 0 when the scene has none.
 struct BoxSet {
     uint32_t n;
     float mn[APBF_MAX_BOXES][3];
     float mx[APBF_MAX_BOXES][3];
+    uint32_t sphere;
+    int32_t hollow;
+    float sc[3];
+    float sr;
 };
 
 struct SolverArgs {
@@ -100,6 +106,29 @@ APBF_HD int4 box_collide(uint32_t id, int4 p, float r, const BoxSet& boxes) {
         if (dMin <= 0.0f || dMax <= 0.0f) continue;
         pos = pos + (dMin < dMax ? toMin : toMax);
     }
+    return make_int4(to_i32(pos.x * POS_RES), to_i32(pos.y * POS_RES), to_i32(pos.z * POS_RES), p.w);
+}
+
+// sphere_collision.comp:24-30 ("Hash without Sine", MIT licence)
+APBF_HD float hash11(float p) {
+    p = fract(p * 0.1031f);
+    p *= p + 33.33f;
+    p *= p + p;
+    return fract(p);
+}
+// sphere_collision.comp:42-60 for the particle with list position `id`: a
+// hollow sphere keeps particles inside (radius - r - jitter), a solid one
+// outside; a particle already on its side is left untouched.
+APBF_HD int4 sphere_collide(uint32_t id, int4 p, float r, const BoxSet& c) {
+    v3f pos = pos_world(p);
+    const float dir = c.hollow ? -1.0f : 1.0f;
+    const float offset = dir * (r + hash11((float)id * pos.y + pos.x + pos.z) * 0.05f);
+    const v3f center = mk(c.sc[0], c.sc[1], c.sc[2]);
+    const v3f rel = pos - center;
+    const float d = length(rel);
+    const float target = c.sr + offset;
+    if ((d < target) == (c.hollow != 0)) return p;
+    pos = center + rel * (target / d);
     return make_int4(to_i32(pos.x * POS_RES), to_i32(pos.y * POS_RES), to_i32(pos.z * POS_RES), p.w);
 }
 

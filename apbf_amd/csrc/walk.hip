@@ -319,6 +319,61 @@ __global__ __launch_bounds__(BLOCK) void k_fix_clamped_rows(const uint32_t* __re
         row_f[k] = end;
 }
 
+// neighborhood_brute_force.comp:31-50: particle i against every particle in
+// index order, a whole wavefront per particle (64 candidates per step); the
+// distance test and the capacity clamp are those of the cell walk, so the
+// rows come out in the order a sequential run appends them. COUNT writes the
+// row lengths, FILL the rows at the scanned offsets.
+template <bool FILL>
+__global__ __launch_bounds__(BLOCK) void k_brute_rows(const int4* __restrict__ pos, const float* __restrict__ kw,
+                                                      const uint32_t* __restrict__ counts, float range_scale,
+                                                      uint32_t pcap, uint32_t* __restrict__ cnt,
+                                                      const uint32_t* __restrict__ row, uint32_t* __restrict__ col) {
+    constexpr uint32_t L = 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t n = counts[C_N];
+    const uint32_t waves = BLOCK / 64;
+    for (uint32_t i = blockIdx.x * waves + threadIdx.x / 64; i < n; i += gridDim.x * waves) {
+        const v3f p = pos_world(pos[i]);
+        const float range = kw[i] * range_scale;
+        const float t_range = (range != range) ? __builtin_inff() : sqrt_le_bound(range);
+        const uint32_t b0 = FILL ? row[i] : 0u;
+        uint32_t c = 0;
+        bool more = !FILL || b0 < pcap;
+        for (uint32_t j0 = 0; more && j0 < n; j0 += L) {
+            const uint32_t j = j0 + lane;
+            bool green = false;
+            if (j < n && j != i) {
+                const v3f d = p - pos_world(pos[j]);
+                green = !(dot(d, d) > t_range);
+            }
+            const uint64_t bal = __ballot(green);
+            if (FILL) {
+                const uint32_t ridx = b0 + c + (uint32_t)__popcll(bal & below);
+                if (green && ridx < pcap) col[ridx] = j;
+            }
+            c += (uint32_t)__popcll(bal);
+            if (FILL) more = b0 + c < pcap;
+        }
+        if (!FILL && lane == 0) cnt[i] = c;
+    }
+    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cnt[n] = 0;
+}
+
+void launch_brute(bool fill, const int4* pos, const float* kw, const uint32_t* counts, uint32_t cap,
+                  float range_scale, uint32_t pcap, uint32_t* cnt, const uint32_t* row, uint32_t* col,
+                  hipStream_t s) {
+    uint32_t g = div_up(cap ? cap : 1, BLOCK / 64);
+    g = g < 16384u ? g : 16384u;
+    if (fill)
+        hipLaunchKernelGGL(k_brute_rows<true>, dim3(g), dim3(BLOCK), 0, s, pos, kw, counts, range_scale, pcap, cnt,
+                           row, col);
+    else
+        hipLaunchKernelGGL(k_brute_rows<false>, dim3(g), dim3(BLOCK), 0, s, pos, kw, counts, range_scale, pcap, cnt,
+                           row, col);
+}
+
 static uint32_t walk_grid(uint32_t cap, uint32_t lanes) {
     const uint32_t groups = BLOCK / lanes;
     uint32_t g = div_up(cap ? cap : 1, groups);

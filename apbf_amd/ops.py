@@ -131,6 +131,22 @@ def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx,
     return row, col, npairs
 
 
+def neighborhood_brute_force(position, rng, range_scale, pair_cap):
+    """CSR neighbour lists of the brute-force search (row offsets, neighbour ids, pair count)."""
+    L = _lib.lib()
+    n = position.shape[0]
+    dev = position.device
+    row = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    col = torch.empty(max(pair_cap, 1), dtype=torch.int32, device=dev)
+    npairs = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = _ws(L.apbf_neighborhood_green_workspace_size(n), dev)
+    check(L.apbf_neighborhood_brute_force(_p(position), _p(rng), _p(_len(n, dev)), n, float(range_scale), _p(row),
+                                          _p(col), pair_cap, _p(npairs), _p(ws), ws.numel(), _stream()),
+          "apbf_neighborhood_brute_force")
+    _sync()
+    return row, col, npairs
+
+
 def copy_scattered_read(src: torch.Tensor, edit: torch.Tensor, stride: int, out_len: int | None = None):
     L = _lib.lib()
     n = edit.numel() if out_len is None else out_len

@@ -15,7 +15,7 @@ from ._lib import check
 
 STAGES = {
     "velocity": 0, "particle_transfer": 1, "kernel_width_bd": 2, "neighborhood": 3,
-    "spread": 4, "box": 5, "incompressibility": 6, "update_transfers": 7,
+    "spread": 4, "box": 5, "incompressibility": 6, "update_transfers": 7, "sphere": 8,
 }
 
 
@@ -33,6 +33,9 @@ class Scene:
         self._h = h
         if stream is not None:
             check(self._L.apbf_scene_set_stream(self._h, C.c_void_p(stream)), "apbf_scene_set_stream")
+        if params.get("sphere"):  # (center, radius, hollow) of the scene's sphere collider
+            c, r, hollow = params["sphere"]
+            self.set_sphere(c, r, hollow)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -132,10 +135,16 @@ class Scene:
                                              int(reset)), "error_stats")
         return m.value, x.value, l.value, a.value, n.value
 
+    def set_sphere(self, center, radius: float, hollow: bool = True, enable: bool = True) -> None:
+        """The scene's sphere collider (sphere_collision::set_sphere)."""
+        c = np.asarray(center, np.float32)
+        check(self._L.apbf_scene_set_sphere(self._h, _ptr(c), float(radius), int(hollow), int(enable)),
+              "set_sphere")
+
     def set_neighborhood_method(self, method: str) -> None:
-        """'green' (hash grid) or 'binary_search' (z-order codes)."""
-        check(self._L.apbf_scene_set_neighborhood_method(self._h, {"green": 0, "binary_search": 1}[method]),
-              "set_neighborhood_method")
+        """'green' (hash grid), 'binary_search' (z-order codes) or 'brute_force'."""
+        code = {"green": 0, "binary_search": 1, "brute_force": 2}[method]
+        check(self._L.apbf_scene_set_neighborhood_method(self._h, code), "set_neighborhood_method")
 
     def capture(self, dt: float) -> None:
         """Capture two substeps into a HIP graph (replay advances by two)."""

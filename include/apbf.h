@@ -132,6 +132,7 @@ apbf_status apbf_scene_get_lambda(void* scene, float* host_lambda);
 #define APBF_STAGE_BOX 5               /* box_collision::apply (box_collision.cpp:12-24) */
 #define APBF_STAGE_INCOMPRESSIBILITY 6 /* incompressibility::apply (incompressibility.cpp:12-45) */
 #define APBF_STAGE_UPDATE_TRANSFERS 7  /* update_transfers::apply (update_transfers.cpp:14-71) */
+#define APBF_STAGE_SPHERE 8            /* sphere_collision::apply (sphere_collision.comp:42-60) */
 apbf_status apbf_scene_stage(void* scene, int32_t stage, float dt);
 /* One substep (pool::update / waterdrop::update, pool.cpp:67-106). The solver
  * loop runs box collision + incompressibility solverIterations times. */
@@ -240,6 +241,17 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
                                     uint32_t* d_n_pairs, void* workspace, size_t workspace_bytes,
                                     apbf_stream stream);
 
+/* Brute-force neighbour search (shader_provider::neighborhood_brute_force;
+ * neighborhood_brute_force.comp:31-50): every particle against every other in
+ * index order, no sort. Same CSR output, clamp and workspace as
+ * apbf_neighborhood_green; with capacity 0 (an empty list) position and range
+ * may be null. */
+apbf_status apbf_neighborhood_brute_force(const int32_t* position, const float* range,
+                                          const uint32_t* d_len, uint32_t capacity, float range_scale,
+                                          uint32_t* row_offset, uint32_t* neighbors, uint32_t pair_capacity,
+                                          uint32_t* d_n_pairs, void* workspace, size_t workspace_bytes,
+                                          apbf_stream stream);
+
 /* Gather by edit list: dst[i] = src[edit[i]] for i < d_edit_len, element size
  * `stride` bytes (a multiple of 4) (gpu_list::apply_edit, gpu_list.h:126-135;
  * copy_scattered_read.comp:17-27). */
@@ -278,10 +290,22 @@ apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_del
  * and neighborhood_green::set_position_range (neighborhood_green.h:13). Call
  * them between substeps; they drop a captured graph. */
 apbf_status apbf_scene_set_gravity(void* scene, const float* gravity3);
+/* sphere_collision::set_sphere (spherical_pool.cpp:29): one sphere collider;
+ * hollow != 0 keeps the particles inside it, 0 outside. It runs after the
+ * boxes at the start of every solver iteration (alone in the reference's
+ * spherical_pool, spherical_pool.cpp:86-89) and as APBF_STAGE_SPHERE.
+ * enable = 0 removes it. Drops a captured graph. */
+apbf_status apbf_scene_set_sphere(void* scene, const float* center3, float radius, int32_t hollow, int32_t enable);
 /* The neighbour search of the scene's neighbourhood stage, the compile-time
  * NEIGHBORHOOD_TYPE of the reference (cpu_gpu_shared_config.h:54-60):
  * 0 neighborhood_green (hash grid, default), 1 neighborhood_binary_search
- * (z-order codes of the raw positions, neighborhood_binary_search.cpp:22-75). */
+ * (z-order codes of the raw positions, neighborhood_binary_search.cpp:22-75),
+ * 2 neighborhood_brute_force (every pair tested, no sort;
+ * neighborhood_brute_force.cpp:19-42). Every method writes each particle's
+ * neighbours as one row in walk order: the reference's list with
+ * mNeighborListSorted set (neighbor_add.glsl:5-18 and
+ * linked_list_to_neighbor_list.comp), and one ordering of its unsorted
+ * (atomic-order) list. */
 apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method);
 apbf_status apbf_scene_set_range_scale(void* scene, float scale);
 apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2);

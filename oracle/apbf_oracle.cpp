@@ -488,7 +488,27 @@ struct Scene {
     // 32-bit sections of the 96-bit z-order code of the raw positions
     // (lexicographic by section 2, 1, 0), the particles permuted into that
     // order, then neighborhood_binary_search.comp per particle.
-    int nb_method = 0;  // 0: neighborhood_green, 1: neighborhood_binary_search
+    int nb_method = 0;  // 0: neighborhood_green, 1: neighborhood_binary_search, 2: neighborhood_brute_force
+
+    // neighborhood_brute_force.cpp:19-42 with neighborhood_brute_force.comp:31-50
+    // and neighbor_add.glsl: every fluid particle against every other in fluid
+    // order; no sort, no permutation; the list clamps at the pair capacity.
+    void neighborhood_bf(float range_scale) {
+        const uint32_t cap = P.cap_pairs;
+        uint32_t len = 0;
+        pairs.clear();
+        for (uint32_t id = 0; id < F(); ++id) {
+            const float range = kernel_width[id] * range_scale;
+            const V3 pos = pos_world(position[fidx[id]]);
+            for (uint32_t idN = 0; idN < F(); ++idN) {
+                const V3 posN = pos_world(position[fidx[idN]]);
+                if (id == idN || length(sub(pos, posN)) > range) continue;
+                const uint32_t i = len++;
+                if (i < cap) pairs.push_back(U2{id, idN});
+                else len = cap;
+            }
+        }
+    }
 
     void neighborhood_bs(float range_scale) {
         uint32_t nh = H();
@@ -1009,16 +1029,49 @@ struct Scene {
         velocity_handling(dt);
         if (!P.basic_pbf && (s.mMerge || s.mSplit)) particle_transfer(dt);
         if (!P.basic_pbf && s.mBaseKernelWidthOnBoundaryDistance) kernel_width_from_boundary_distance();
+        const float scale = (P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f;
         if (nb_method == 1)
-            neighborhood_bs((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            neighborhood_bs(scale);
+        else if (nb_method == 2)
+            neighborhood_bf(scale);
         else
-            neighborhood((P.basic_pbf || s.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+            neighborhood(scale);
         if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
         for (int i = 0; i < P.solver_iterations; ++i) {
-            box_collision();
+            if (P.n_boxes) box_collision();  // scenes without boxes have no box stage
+            sphere_collision();
             incompressibility();
         }
         if (!P.basic_pbf) update_transfers();
+    }
+
+    // sphere_collision.comp:42-60 (the collider of spherical_pool.cpp:29)
+    bool sphere_on = false, sphere_hollow = false;
+    float sphere_c[3] = {0, 0, 0}, sphere_r = 0.0f;
+    static float hash11(float p) {  // sphere_collision.comp:24-30 ("Hash without Sine", MIT)
+        p = fract(p * 0.1031f);
+        p *= p + 33.33f;
+        p *= p + p;
+        return fract(p);
+    }
+    void sphere_collision() {
+        if (!sphere_on) return;
+        const V3 center = v3(sphere_c[0], sphere_c[1], sphere_c[2]);
+        for (uint32_t id = 0; id < F(); ++id) {
+            const uint32_t idx = fidx[id];
+            V3 pos = pos_world(position[idx]);
+            const float dir = sphere_hollow ? -1.0f : 1.0f;
+            const float offset = dir * (radius[idx] + hash11((float)id * pos.y + pos.x + pos.z) * 0.05f);
+            const V3 rel = sub(pos, center);
+            const float d = length(rel);
+            const float target = sphere_r + offset;
+            if ((d < target) == sphere_hollow) continue;
+            pos = add(center, muls(rel, target / d));
+            I4& p = position[idx];
+            p.x = to_i32(pos.x * POS_RES);
+            p.y = to_i32(pos.y * POS_RES);
+            p.z = to_i32(pos.z * POS_RES);
+        }
     }
 };
 
@@ -1109,6 +1162,13 @@ uint32_t orc_get_splits(void* h, uint32_t* split_list) {
 }
 
 void orc_set_neighborhood_method(void* h, int method) { static_cast<Scene*>(h)->nb_method = method; }
+void orc_set_sphere(void* h, const float* center3, float radius, int hollow, int enable) {
+    Scene* s = static_cast<Scene*>(h);
+    s->sphere_on = enable != 0;
+    s->sphere_hollow = hollow != 0;
+    for (int d = 0; d < 3; ++d) s->sphere_c[d] = enable ? center3[d] : 0.0f;
+    s->sphere_r = enable ? radius : 0.0f;
+}
 
 void orc_stage(void* h, int stage, float dt) {
     Scene* s = static_cast<Scene*>(h);
@@ -1118,13 +1178,19 @@ void orc_stage(void* h, int stage, float dt) {
         case ORC_STAGE_PARTICLE_TRANSFER: s->particle_transfer(dt); break;
         case ORC_STAGE_KERNEL_WIDTH_BD: s->kernel_width_from_boundary_distance(); break;
         case ORC_STAGE_NEIGHBORHOOD:
+        {
+            const float scale = (s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f;
             if (s->nb_method == 1)
-                s->neighborhood_bs((s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+                s->neighborhood_bs(scale);
+            else if (s->nb_method == 2)
+                s->neighborhood_bf(scale);
             else
-                s->neighborhood((s->P.basic_pbf || st.mBaseKernelWidthOnBoundaryDistance) ? 1.0f : 1.5f);
+                s->neighborhood(scale);
             break;
+        }
         case ORC_STAGE_SPREAD: s->spread_kernel_width(); break;
         case ORC_STAGE_BOX: s->box_collision(); break;
+        case ORC_STAGE_SPHERE: s->sphere_collision(); break;
         case ORC_STAGE_INCOMPRESSIBILITY: s->incompressibility(); break;
         case ORC_STAGE_UPDATE_TRANSFERS: s->update_transfers(); break;
         default: break;

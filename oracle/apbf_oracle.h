@@ -104,10 +104,14 @@ enum {
     ORC_STAGE_SPREAD = 4,           /* spread_kernel_width::apply */
     ORC_STAGE_BOX = 5,              /* box_collision::apply */
     ORC_STAGE_INCOMPRESSIBILITY = 6,/* incompressibility::apply */
-    ORC_STAGE_UPDATE_TRANSFERS = 7  /* update_transfers::apply */
+    ORC_STAGE_UPDATE_TRANSFERS = 7, /* update_transfers::apply */
+    ORC_STAGE_SPHERE = 8            /* sphere_collision::apply */
 };
-/* 0: neighborhood_green (default), 1: neighborhood_binary_search. */
+/* 0: neighborhood_green (default), 1: neighborhood_binary_search,
+ * 2: neighborhood_brute_force. */
 void orc_set_neighborhood_method(void* scene, int method);
+/* One sphere collider (sphere_collision::set_sphere); enable = 0 removes it. */
+void orc_set_sphere(void* scene, const float* center3, float radius, int hollow, int enable);
 void orc_stage(void* h, int stage, float dt);
 /* One full substep (pool::update / waterdrop::update with the time machine off). */
 void orc_update(void* h, float dt);

@@ -85,6 +85,7 @@ def _declare(L):
     L.orc_get_splits.argtypes = [vp, pu]
     L.orc_stage.argtypes = [vp, C.c_int, f32]
     L.orc_set_neighborhood_method.argtypes = [vp, C.c_int]
+    L.orc_set_sphere.argtypes = [vp, pf, f32, C.c_int, C.c_int]
     L.orc_update.argtypes = [vp, f32]
     L.orc_density_error.argtypes = [vp, _p(f64), _p(f64)]
     L.orc_stable_radix_sort.argtypes = [pu, pu, u32, u32, pu, pu]
@@ -155,7 +156,7 @@ def make_params(p: dict) -> OrcSceneParams:
 
 STAGE = {
     "velocity": 0, "particle_transfer": 1, "kernel_width_bd": 2, "neighborhood": 3,
-    "spread": 4, "box": 5, "incompressibility": 6, "update_transfers": 7,
+    "spread": 4, "box": 5, "incompressibility": 6, "update_transfers": 7, "sphere": 8,
 }
 
 
@@ -167,6 +168,9 @@ class OracleScene:
         self._L = lib()
         self._params = make_params(params)
         self._h = self._L.orc_create(C.byref(self._params))
+        if params.get("sphere"):  # (center, radius, hollow) of the scene's sphere collider
+            c, r, hollow = params["sphere"]
+            self.set_sphere(c, r, hollow)
 
     def close(self):
         if self._h:
@@ -248,7 +252,11 @@ class OracleScene:
         return out[:n]
 
     def set_neighborhood_method(self, method: str) -> None:
-        self._L.orc_set_neighborhood_method(self._h, {"green": 0, "binary_search": 1}[method])
+        self._L.orc_set_neighborhood_method(self._h, {"green": 0, "binary_search": 1, "brute_force": 2}[method])
+
+    def set_sphere(self, center, radius: float, hollow: bool = True, enable: bool = True) -> None:
+        c = np.asarray(center, np.float32)
+        self._L.orc_set_sphere(self._h, _ptr(c, C.c_float), float(radius), int(hollow), int(enable))
 
     def stage(self, name: str, dt: float = 0.0) -> None:
         self._L.orc_stage(self._h, STAGE[name], float(dt))

@@ -166,6 +166,25 @@ def test_neighborhood_green_matches_oracle(device, res, scale):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n,cap", [(1000, 10_000_000), (1331, 4000), (0, 100)])
+def test_neighborhood_brute_force_matches_oracle(device, n, cap):
+    """neighborhood_brute_force.comp:31-50 in index order, with the clamp."""
+    from apbf_amd import ops
+    import torch
+    rng = np.random.default_rng(n)
+    p = np.zeros((n, 4), np.int32)
+    p[:, :3] = np.trunc(rng.uniform(0.0, 6.0, (n, 3)) * 262144.0).astype(np.int32)
+    rad = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    want = O.neighborhood_brute_force(p, rad, 1.25, cap)
+    row, col, npairs = ops.neighborhood_brute_force(torch.from_numpy(p).to(device), torch.from_numpy(rad).to(device),
+                                                    1.25, cap)
+    npairs = int(N(npairs)[0])
+    assert npairs == len(want)
+    r = np.minimum(N(row), cap)
+    got = np.stack([np.repeat(np.arange(n), np.diff(r)), N(col)[:npairs]], 1) if n else np.zeros((0, 2))
+    assert np.array_equal(got, want)
+
+
 def test_neighborhood_green_capacity_clamp(device):
     from apbf_amd import ops
     import torch

@@ -209,3 +209,47 @@ def test_binary_search_neighborhood_matches_oracle(device, kind):
         g.update(p["dt"])
         r.update(p["dt"])
         _check(g, r, f"substep {step}")
+
+
+@pytest.mark.parametrize("kind", ["dam_break", "water_drop_active"])
+def test_brute_force_neighborhood_matches_oracle(device, kind):
+    """neighborhood_brute_force as the scene's search: stages and substeps bit
+    for bit (no sort, rows in index order)."""
+    if kind == "dam_break":
+        p, st = cfg.dam_break(n_side=12)
+    else:
+        p, st = cfg.small_water_drop(seed=9, active=True)
+    g, r = _pair(p, st)
+    g.set_neighborhood_method("brute_force")
+    r.set_neighborhood_method("brute_force")
+    g.stage("velocity", p["dt"])
+    r.stage("velocity", p["dt"])
+    g.stage("neighborhood", p["dt"])
+    r.stage("neighborhood", p["dt"])
+    _check(g, r, "after the neighbourhood stage")
+    for step in range(3):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step}")
+
+
+@pytest.mark.parametrize("kind", ["spherical_pool", "waterfall", "pool"])
+def test_reference_scenes_match_oracle(device, kind):
+    """The reference's other scenes at reduced size: the spherical pool's
+    sphere collider (sphere_collision.comp:42-60, no boxes), the waterfall's
+    eleven boxes and the pool, stages and substeps bit for bit."""
+    if kind == "spherical_pool":
+        p, st = cfg.spherical_pool_scene(pool_radius=12.0)
+    elif kind == "waterfall":
+        p, st = cfg.waterfall_scene(mn=(-10.0, -4.0, -20.0), mx=(10.0, 6.0, -10.0))
+    else:
+        p, st = cfg.pool_scene(mn=(-10.0, -4.0, -20.0), mx=(10.0, 6.0, -10.0))
+    g, r = _pair(p, st)
+    for name in adaptive_stages(p) + (["sphere"] if kind == "spherical_pool" else ["box"]):
+        g.stage(name, p["dt"])
+        r.stage(name, p["dt"])
+        _check(g, r, f"after {name}")
+    for step in range(4):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step}")

@@ -345,3 +345,92 @@ def test_binary_search_substeps_track_green():
     ia = np.lexsort((ka[:, 2], ka[:, 1], ka[:, 0]))
     ib = np.lexsort((kb[:, 2], kb[:, 1], kb[:, 0]))
     assert np.array_equal(sa["position"][ia], sb["position"][ib])
+
+
+def test_brute_force_neighborhood_order_and_set():
+    """neighborhood_brute_force (neighborhood_brute_force.cpp:19-42): the
+    particles keep their order, every row lists its neighbours in index order
+    (the sorted-list layout, neighbor_add.glsl:5-18), and the pair set is the
+    predicate d(i, j) <= range_i, i != j."""
+    for seed in (0, 3):
+        p, st = _bs_state(seed)
+        o = OracleScene(p)
+        o.set_state(st)
+        o.set_neighborhood_method("brute_force")
+        o.stage("neighborhood", 0.0)
+        a = o.get_state()
+        assert np.array_equal(a["position"], st["position"])  # no permutation
+        pos = a["position"][:, :3].astype(np.float32) / F(262144.0)
+        d = pos[:, None, :] - pos[None, :, :]
+        d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        dist = np.sqrt(d2.astype(F))
+        want = np.argwhere((dist <= a["kernel_width"][:, None]) & ~np.eye(len(pos), dtype=bool))
+        assert np.array_equal(a["pairs"].astype(np.int64), want)  # argwhere is row-major: rows, then index
+
+
+def test_brute_force_substeps_track_green():
+    """A plain dam break stepped with the brute-force search and with the hash
+    grid: the same pair sets, so equal states up to the particle order (no walls,
+    whose jitter depends on the slot)."""
+    p, st = cfg.dam_break(n_side=7, seed=6)
+    p["boxes"] = []
+    a, b = OracleScene(p), OracleScene(p)
+    a.set_state(st)
+    b.set_state(st)
+    b.set_neighborhood_method("brute_force")
+    for _ in range(3):
+        a.update(p["dt"])
+        b.update(p["dt"])
+    sa, sb = a.get_state(), b.get_state()
+    ka = sa["pos_backup"][:, :3].astype(np.int64)
+    kb = sb["pos_backup"][:, :3].astype(np.int64)
+    ia = np.lexsort((ka[:, 2], ka[:, 1], ka[:, 0]))
+    ib = np.lexsort((kb[:, 2], kb[:, 1], kb[:, 0]))
+    assert np.array_equal(sa["position"][ia], sb["position"][ib])
+
+
+def test_reference_scene_layouts():
+    """pool / waterfall / spherical_pool as main.cpp:62-65 builds them: box and
+    shell particle layouts (initialize.cpp:5-100) and their colliders."""
+    p, st = cfg.pool_scene()
+    assert len(st["radius"]) == 40 * 20 * 20 and len(p["boxes"]) == 6
+    p, st = cfg.waterfall_scene()
+    assert len(st["radius"]) == 40 * 20 * 20 and len(p["boxes"]) == 11
+    p, st = cfg.spherical_pool_scene(pool_radius=15.0)
+    pos = st["position"][:, :3].astype(np.float64) / 262144.0
+    d = np.linalg.norm(pos - np.array([0.0, 10.0, -60.0]), axis=1)
+    assert d.max() <= 15.0 - 1.0 + 1e-4 and p["sphere"] == ((0.0, 10.0, -60.0), 15.0, True)
+    # each shell's radius keeps one particle per (2r)^3 of the ball's volume
+    want = 4.0 / 3.0 * np.pi * 15.0 ** 3 / 8.0
+    assert abs(len(d) - want) < 0.05 * want
+
+
+def test_sphere_collision_invariants():
+    """sphere_collision.comp:42-60, hollow: particles outside the sphere minus
+    their radius and jitter move onto that surface along the centre ray; the
+    rest keep their positions bit for bit."""
+    p, st = cfg.spherical_pool_scene(pool_radius=10.0)
+    rng = np.random.default_rng(1)
+    st = dict(st)
+    n = len(st["radius"])
+    c = np.array([0.0, 10.0, -60.0])
+    pos = c + rng.normal(size=(n, 3)) * 6.0
+    st["position"] = st["position"].copy()
+    st["position"][:, :3] = np.trunc(pos * 262144.0).astype(np.int32)
+    o = OracleScene(p)
+    o.set_state(st)
+    o.stage("sphere", 0.0)
+    a = o.get_state()
+    before = st["position"][:, :3].astype(np.float64) / 262144.0
+    after = a["position"][:, :3].astype(np.float64) / 262144.0
+    db = np.linalg.norm(before - c, axis=1)
+    da = np.linalg.norm(after - c, axis=1)
+    inside = db < 10.0 - 1.05  # the jitter lowers the target radius by less than 0.05
+    assert np.array_equal(a["position"][inside], st["position"][inside])
+    moved = db > 10.0 - 1.0 + 1e-3
+    assert moved.any()
+    assert np.all(da[moved] <= 10.0 - 1.0 + 1e-4) and np.all(da[moved] >= 10.0 - 1.05 - 1e-4)
+    # moved along the ray from the centre
+    u = (before[moved] - c) / db[moved, None]
+    v = (after[moved] - c) / da[moved, None]
+    assert np.all(np.sum(u * v, axis=1) > 1 - 1e-6)
