@@ -152,6 +152,12 @@ PROTOTYPES = {
     "apbf_version": (C.c_char_p, []),
     "apbf_device_count": (_i32, []),
     "apbf_scene_set_sphere": (_i32, [_vp, _vp, _f32, _i32, _i32]),
+    "apbf_snapshot_create": (_i32, [_vp, C.POINTER(_vp)]),
+    "apbf_snapshot_save": (_i32, [_vp, _vp]),
+    "apbf_snapshot_restore": (_i32, [_vp, _vp]),
+    "apbf_snapshot_destroy": (_i32, [_vp]),
+    "apbf_scene_save_particle_info": (_i32, [_vp, C.c_char_p]),
+    "apbf_scene_save_measurements": (_i32, [_vp, C.c_char_p, _u32, _f32, _f32, _f32]),
     "apbf_check_conversions": (_i32, [_u32, C.POINTER(C.c_uint64), _pu, _vp, _vp]),
 }
 

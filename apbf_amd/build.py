@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libapbf_hip.so")
 SOURCES = ["primitives.hip", "plain_kernels.hip", "rows.hip", "walk.hip", "adaptive.hip", "scene.hip",
-           "capi_lists.hip", "mlp.hip", "graph.hip", "bsearch.hip"]
+           "capi_lists.hip", "mlp.hip", "graph.hip", "bsearch.hip", "particle_info.hip",
+           "snapshot.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -135,6 +135,16 @@ class Scene:
                                              int(reset)), "error_stats")
         return m.value, x.value, l.value, a.value, n.value
 
+    def save_particle_info(self, folder: str = "particle_data") -> None:
+        """save_particle_info::apply: the reference's text dumps of the fluid."""
+        check(self._L.apbf_scene_save_particle_info(self._h, folder.encode()), "save_particle_info")
+
+    def save_measurements(self, folder: str, svg_id: int, step_ms: float = 0.0, neighborhood_ms: float = 0.0,
+                          solver_ms: float = 0.0) -> None:
+        """save_particle_info::save_additional_info: measurements_<svg_id>.txt."""
+        check(self._L.apbf_scene_save_measurements(self._h, folder.encode(), int(svg_id), float(step_ms),
+                                                   float(neighborhood_ms), float(solver_ms)), "save_measurements")
+
     def set_sphere(self, center, radius: float, hollow: bool = True, enable: bool = True) -> None:
         """The scene's sphere collider (sphere_collision::set_sphere)."""
         c = np.asarray(center, np.float32)

@@ -380,6 +380,34 @@ apbf_status apbf_scene_set_warm_start(void* scene, apbf_mlp* model, int32_t mode
  * to device buffers of `capacity` rows; *count receives n. Synchronises. */
 apbf_status apbf_scene_harvest(void* scene, float* features, float* targets, uint32_t capacity, uint32_t* count);
 
+/* ------------------------------------------------------ state snapshots */
+/* The keyframes of the reference's time machine (time_machine.h:46-143): a
+ * device-resident copy of a scene's particle, fluid and transfer lists, count
+ * block and last time step (not the neighbour list, which every substep
+ * rebuilds; a restored scene reports no pairs until its next neighbourhood
+ * stage). create allocates and saves; save overwrites; restore copies back
+ * into the scene. Copies are stream-ordered on the scene's stream (no host
+ * synchronisation). A snapshot fits only the scene (capacities) it came from. */
+typedef struct apbf_snapshot apbf_snapshot;
+apbf_status apbf_snapshot_create(void* scene, apbf_snapshot** out_snapshot);
+apbf_status apbf_snapshot_save(apbf_snapshot* snapshot, void* scene);
+apbf_status apbf_snapshot_restore(apbf_snapshot* snapshot, void* scene);
+apbf_status apbf_snapshot_destroy(apbf_snapshot* snapshot);
+
+/* --------------------------------------------------------- particle dumps */
+/* save_particle_info::apply (save_particle_info.cpp:21-128): writes
+ * centerDist.txt, radius.txt, neighborCount.txt, kernelWidth.txt,
+ * targetRadius.txt, boundaryDistance.txt and data.csv of the scene's fluid into
+ * `folder` (created if missing; the reference uses "particle_data"), in the
+ * reference's formats. Synchronises. */
+apbf_status apbf_scene_save_particle_info(void* scene, const char* folder);
+/* save_particle_info::save_additional_info (save_particle_info.cpp:271-280):
+ * folder/measurements_<id>.txt with the particle and pair counts, the given
+ * timings and the scene's neighbour search (the reference's 3D SVG dump writes
+ * only this file). Synchronises. */
+apbf_status apbf_scene_save_measurements(void* scene, const char* folder, uint32_t id, float step_ms,
+                                         float neighborhood_ms, float solver_ms);
+
 /* ------------------------------------------------------------ utilities */
 /* Library version string and the device it was built for ("gfx950"). */
 const char* apbf_version(void);

@@ -8,6 +8,8 @@
 #ifndef APBF_HPP
 #define APBF_HPP
 
+#include <algorithm>
+#include <deque>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -150,6 +152,26 @@ private:
     scene* s_ = nullptr;
 };
 
+/* neighborhood_binary_search.h / neighborhood_brute_force.h: the scene's
+ * search is one of the three (NEIGHBORHOOD_TYPE, cpu_gpu_shared_config.h:54-60);
+ * these select it and run it, with the range scale of neighborhood_green. */
+class neighborhood_binary_search : public neighborhood_green {
+public:
+    neighborhood_binary_search& set_data(scene* s) {
+        neighborhood_green::set_data(s);
+        check(apbf_scene_set_neighborhood_method(s->handle(), 1), "set_neighborhood_method");
+        return *this;
+    }
+};
+class neighborhood_brute_force : public neighborhood_green {
+public:
+    neighborhood_brute_force& set_data(scene* s) {
+        neighborhood_green::set_data(s);
+        check(apbf_scene_set_neighborhood_method(s->handle(), 2), "set_neighborhood_method");
+        return *this;
+    }
+};
+
 /* spread_kernel_width.h:10-11 */
 class spread_kernel_width {
 public:
@@ -176,6 +198,24 @@ private:
     scene* s_ = nullptr;
 };
 
+/* sphere_collision.h: set_sphere(center, radius, hollow), apply() */
+class sphere_collision {
+public:
+    sphere_collision& set_data(scene* s) {
+        s_ = s;
+        return *this;
+    }
+    sphere_collision& set_sphere(const vec3& center, float radius, bool hollow) {
+        const float c[3] = {center.x, center.y, center.z};
+        check(apbf_scene_set_sphere(s_->handle(), c, radius, hollow ? 1 : 0, 1), "set_sphere");
+        return *this;
+    }
+    void apply() { s_->stage(APBF_STAGE_SPHERE, 0.0f); }
+
+private:
+    scene* s_ = nullptr;
+};
+
 /* incompressibility.h:9-18: one solver iteration per apply() */
 class incompressibility {
 public:
@@ -197,6 +237,134 @@ public:
         return *this;
     }
     void apply() { s_->stage(APBF_STAGE_UPDATE_TRANSFERS, 0.0f); }
+
+private:
+    scene* s_ = nullptr;
+};
+
+/* time_machine.h:9-143 for one scene: keyframes every keyframe_interval
+ * substeps as device snapshots (at most max_keyframes), the time step of every
+ * substep on the host. As in the scenes' update (pool.cpp:67-106):
+ *     if (!tm.step_forward(dt)) { s.update(dt); tm.save_state(dt); }
+ * step_forward replaces dt by the recorded step in the past and returns true
+ * when it loaded a keyframe (the substep is then skipped). */
+class time_machine {
+public:
+    explicit time_machine(scene& s) : s_(s) {}
+    ~time_machine() {
+        for (apbf_snapshot* k : keys_) apbf_snapshot_destroy(k);
+        for (apbf_snapshot* k : spare_) apbf_snapshot_destroy(k);
+    }
+    time_machine(const time_machine&) = delete;
+    time_machine& operator=(const time_machine&) = delete;
+
+    time_machine& set_max_keyframes(uint32_t k) {
+        max_keyframes_ = k;
+        return *this;
+    }
+    time_machine& set_keyframe_interval(uint32_t i) {
+        interval_ = i;
+        drop_front(0);
+        dts_.clear();
+        return *this;
+    }
+    time_machine& enable(float dt = 1.0f) {
+        if (!enabled_) {
+            keys_.push_back(take());
+            dts_.push_back(dt);
+        }
+        enabled_ = true;
+        return *this;
+    }
+    time_machine& disable() {
+        drop_front(0);
+        dts_.clear();
+        enabled_ = false;
+        current_ = present_ = 0;
+        return *this;
+    }
+    bool enabled() const { return enabled_; }
+    bool on_keyframe() const { return current_ % interval_ == 0u; }
+    bool in_past() const { return current_ != present_; }
+    uint32_t keyframe_step_for(uint32_t step) const { return step / interval_ * interval_; }
+    uint32_t oldest_history_keyframe_step() const {
+        return keyframe_step_for(present_) - (max_keyframes_ - 1u) * interval_;
+    }
+    uint32_t history_index() const { return std::min(current_, current_ - oldest_history_keyframe_step()); }
+    uint32_t sparse_history_index() const { return history_index() / interval_; }
+
+    bool step_forward(float& dt) {
+        if (!enabled_) return false;
+        const bool past = in_past();
+        ++current_;
+        present_ += past ? 0u : 1u;
+        const bool loaded = past && on_keyframe();
+        if (past) {
+            dt = dts_[history_index()];
+            if (loaded) load();
+        } else if (on_keyframe()) {
+            drop_front(max_keyframes_ - 1u);
+            const size_t keep = (size_t)(max_keyframes_ - 1u) * interval_;
+            while (dts_.size() > keep) dts_.pop_front();
+        }
+        return loaded;
+    }
+    void save_state(float dt) {
+        if (!enabled_) return;
+        if (keys_.size() <= sparse_history_index()) keys_.push_back(take());
+        if (dts_.size() <= history_index()) dts_.push_back(dt);
+    }
+    bool jump_back() {
+        if ((on_keyframe() && current_ + interval_ * (max_keyframes_ - 1u) <= present_) || current_ == 0u)
+            return false;
+        current_ = (current_ - 1u) / interval_ * interval_;
+        load();
+        return true;
+    }
+
+private:
+    apbf_snapshot* take() {
+        if (!spare_.empty()) {
+            apbf_snapshot* k = spare_.back();
+            spare_.pop_back();
+            check(apbf_snapshot_save(k, s_.handle()), "apbf_snapshot_save");
+            return k;
+        }
+        apbf_snapshot* k = nullptr;
+        check(apbf_snapshot_create(s_.handle(), &k), "apbf_snapshot_create");
+        return k;
+    }
+    void drop_front(size_t keep) {
+        while (keys_.size() > keep) {
+            spare_.push_back(keys_.front());
+            keys_.pop_front();
+        }
+    }
+    void load() { check(apbf_snapshot_restore(keys_[sparse_history_index()], s_.handle()), "apbf_snapshot_restore"); }
+
+    scene& s_;
+    uint32_t max_keyframes_ = 5u, interval_ = 120u, current_ = 0u, present_ = 0u;
+    bool enabled_ = false;
+    std::deque<apbf_snapshot*> keys_;
+    std::vector<apbf_snapshot*> spare_;
+    std::deque<float> dts_;
+};
+
+/* save_particle_info.h:8-23 (3D: the SVG dump writes measurements_<id>.txt only) */
+class save_particle_info {
+public:
+    save_particle_info& set_data(scene* s) {
+        s_ = s;
+        return *this;
+    }
+    void apply(const char* folder = "particle_data") {
+        check(apbf_scene_save_particle_info(s_->handle(), folder), "save_particle_info");
+    }
+    void save_as_svg(uint32_t svg_id, float step_ms = 0.0f, float neighborhood_ms = 0.0f, float solver_ms = 0.0f,
+                     const char* folder = "particle_svg") {
+        check(apbf_scene_save_measurements(s_->handle(), folder, svg_id, step_ms, neighborhood_ms, solver_ms),
+              "save_measurements");
+    }
 
 private:
     scene* s_ = nullptr;

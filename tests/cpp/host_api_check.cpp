@@ -156,13 +156,103 @@ static int run_case(const char* name, bool basic) {
     return 0;
 }
 
+// spherical_pool::update (spherical_pool.cpp:67-106): the sphere collider in
+// place of the boxes, here with the brute-force neighbour search.
+static int run_sphere_case() {
+    const state st = make_state(8, 0.5f, true);
+    apbf_scene_params p = make_params(st, false, 16.0f);
+    p.n_boxes = 0;
+    const float dt = 1.0f / 120.0f;
+    const vec3 center{5.0f, 6.0f, 5.0f};
+    scene whole(p), parts(p);
+    whole.set_state(st);
+    parts.set_state(st);
+    const float c[3] = {center.x, center.y, center.z};
+    check(apbf_scene_set_sphere(whole.handle(), c, 7.0f, 1, 1), "set_sphere");
+    check(apbf_scene_set_neighborhood_method(whole.handle(), 2), "set_neighborhood_method");
+    velocity_handling velocity;
+    particle_transfer transfer;
+    neighborhood_brute_force neighborhood;
+    spread_kernel_width spread;
+    sphere_collision sphere;
+    incompressibility solve;
+    update_transfers transfers;
+    velocity.set_data(&parts).set_acceleration(vec3{0.0f, -9.81f, 0.0f});
+    transfer.set_data(&parts);
+    neighborhood.set_data(&parts);
+    spread.set_data(&parts);
+    sphere.set_data(&parts).set_sphere(center, 7.0f, true);
+    solve.set_data(&parts);
+    transfers.set_data(&parts);
+    for (int step = 0; step < 4; ++step) {
+        whole.update(dt);
+        velocity.apply(dt);
+        transfer.apply(dt);
+        neighborhood.set_range_scale(1.5f).apply();
+        spread.apply();
+        for (int it = 0; it < p.solver_iterations; ++it) {
+            sphere.apply();
+            solve.apply();
+        }
+        transfers.apply();
+        if (!same_state(whole.get_state(), parts.get_state())) {
+            std::printf("spherical pool: substep %d differs\n", step);
+            return 1;
+        }
+    }
+    std::printf("spherical pool: ok, %u particles\n", whole.get_state().size());
+    return 0;
+}
+
+// time_machine: ten substeps in the present, back to the oldest keyframe,
+// forward again through the past; every substep must equal its recording.
+static int run_time_machine_case() {
+    const state st = make_state(8, 0.5f, true);
+    const apbf_scene_params p = make_params(st, false, 16.0f);
+    scene s(p);
+    s.set_state(st);
+    time_machine tm(s);
+    tm.set_max_keyframes(3).set_keyframe_interval(3).enable();
+    std::vector<state> rec(11);
+    rec[0] = s.get_state();
+    auto dt_of = [](int k) { return (1.0f + 0.25f * (float)(k % 3)) / 120.0f; };
+    for (int k = 1; k <= 10; ++k) {
+        float dt = dt_of(k);
+        if (tm.step_forward(dt)) return 1;
+        s.update(dt);
+        tm.save_state(dt);
+        rec[k] = s.get_state();
+    }
+    int back = 0;
+    while (tm.jump_back()) ++back;
+    if (back != 3 || !same_state(s.get_state(), rec[3])) {
+        std::printf("time machine: jump back %d\n", back);
+        return 1;
+    }
+    for (int k = 4; k <= 10; ++k) {
+        float dt = 1.0f;
+        if (!tm.step_forward(dt)) {
+            if (dt != dt_of(k)) return 1;
+            s.update(dt);
+            tm.save_state(dt);
+        }
+        if (!same_state(s.get_state(), rec[k])) {
+            std::printf("time machine: replayed substep %d differs\n", k);
+            return 1;
+        }
+    }
+    std::printf("time machine: ok\n");
+    return 0;
+}
+
 int main() {
     if (apbf_device_count() <= 0) {
         std::printf("no device\n");
         return 2;
     }
     try {
-        return run_case("basic pbf", true) | run_case("adaptive", false);
+        return run_case("basic pbf", true) | run_case("adaptive", false) | run_sphere_case() |
+               run_time_machine_case();
     } catch (const error& e) {
         std::printf("error: %s\n", e.what());
         return 1;

@@ -27,4 +27,5 @@ def test_host_api_links_and_reports_missing_device():
 def test_host_api_subsystems_match_update(device):
     r = subprocess.run([_exe()], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "basic pbf: ok" in r.stdout and "adaptive: ok" in r.stdout
+    assert "basic pbf: ok" in r.stdout and "adaptive: ok" in r.stdout and "spherical pool: ok" in r.stdout
+    assert "time machine: ok" in r.stdout
