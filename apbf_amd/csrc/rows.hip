@@ -155,7 +155,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     // loaded before any arithmetic, so a lane waits for memory twice per
     // batch instead of twice per entry. Lanes past the row end load the row's
     // last entry and discard it.
-    constexpr int B = C < 8 ? C : 8;
+    constexpr int B = (C % 8 == 0) ? 8 : ((C % 4 == 0) ? 4 : C);
+    static_assert(C % B == 0, "the cached entries split into whole batches");
 #pragma unroll
     for (int t = 0; t < C; ++t) {
         cj[t] = 0;

@@ -253,3 +253,21 @@ def test_reference_scenes_match_oracle(device, kind):
         g.update(p["dt"])
         r.update(p["dt"])
         _check(g, r, f"substep {step}")
+
+
+@pytest.mark.parametrize("seed,res", [(3, 4), (11, 5)])
+def test_fused_walk_raw_count_matches_oracle(device, seed, res):
+    """The single-pass walk with spreading counts the full (range 1.5 kw) list
+    exactly, also where it only counts the entries of far cells: its raw total
+    equals the length of the oracle's unfiltered neighbour list."""
+    p, st = cfg.small_water_drop(seed=seed, res=res)
+    g, r = _pair(p, st)
+    for step in range(3):
+        for name in ("velocity", "particle_transfer", "neighborhood"):
+            r.stage(name, p["dt"])
+        raw = len(r.get_state()["pairs"])
+        g.update(p["dt"])
+        assert g.diagnostics()["pairs_before_clamp"] == raw, f"substep {step}"
+        for name in ["spread"] + ["box", "incompressibility"] * p["solver_iterations"] + ["update_transfers"]:
+            r.stage(name, p["dt"])
+        _check(g, r, f"substep {step}")
