@@ -100,9 +100,11 @@ class WarmStartModel:
         return p, n
 
     def set_output_damping(self, gamma: float) -> None:
-        """Scale the predicted lambdas by gamma (the one-shot warm shift uses the
-        total lambda of a whole solve with the gradients of its first
-        iteration, which overshoots where neighbours share constraints)."""
+        """Scale the predicted lambdas by gamma. The one-shot warm shift applies
+        the total lambda of a whole solve with the gradients of its first
+        iteration; gamma = 1 applies the prediction as it is, gamma < 1 damps
+        it and gamma > 1 pushes the particles further apart (less compression
+        error, slightly more expansion)."""
         p, n = self.get_params()
         if not hasattr(self, "_out_scale"):
             self._out_scale = float(n[32])
@@ -150,8 +152,8 @@ class WarmStartModel:
         return np.concatenate(losses)
 
 
-def fit_warm_start(scene, dt: float, harvest_substeps: int = 4, epochs: int = 8, batch: int = 4096,
-                   lr: float = 2e-3, damping: float = 0.6, seed: int = 0):
+def fit_warm_start(scene, dt: float, harvest_substeps: int = 6, epochs: int = 12, batch: int = 4096,
+                   lr: float = 2e-3, damping: float = 1.0, seed: int = 0):
     """Harvest (feature, total-lambda) pairs from plain substeps of `scene`,
     train a network on them on the GPU and return (model, report)."""
     import time

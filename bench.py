@@ -99,11 +99,11 @@ def timed_run(params, state, steps: int, mode: str, model=None, iterations=None)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     before = s.iteration_errors(max_iters=p["solver_iterations"]).tolist()
-    mean_err = s.error_stats()[0]
+    stats = s.error_stats()
     ps, _ = s.particle_substeps()
     s.close()
     return {"iterations": p["solver_iterations"], "warm_start": mode == "warm", "ms_per_step": el / steps * 1e3,
-            "particle_substeps_per_s": ps / el, "mean_density_error": mean_err,
+            "particle_substeps_per_s": ps / el, "mean_density_error": stats[0], "mean_abs_density_error": stats[3],
             "mean_error_before_iteration": before}
 
 
@@ -112,19 +112,39 @@ def learned_comparison(scene, params, args, log):
     warm-start network on the GPU, then run the plain solver and the warm
     start from the same state."""
     from apbf_amd import learned as Lm
-    model, rep = Lm.fit_warm_start(scene, params["dt"], harvest_substeps=3, epochs=6)
+    model, rep = Lm.fit_warm_start(scene, params["dt"], harvest_substeps=args.learned_harvest,
+                                   epochs=args.learned_epochs, damping=args.learned_damping)
     log(f"warm-start network trained on {rep['samples']} pairs in {rep['train_s']:.2f} s "
         f"(relative rms error {rep['relative_rms_error']:.3f})")
     st = scene.get_state(pairs=False)
     st["fluid_index"] = st["fluid_index"][: len(st["radius"])]
     steps = max(10, min(args.steps, 30))
+    iters = params["solver_iterations"]
     plain = timed_run(params, st, steps, "off")
     warm = timed_run(params, st, steps, "warm", model, iterations=args.learned_iterations)
     log(f"plain {plain['ms_per_step']:.2f} ms err {plain['mean_density_error']:.2e}; "
         f"learned {warm['ms_per_step']:.2f} ms err {warm['mean_density_error']:.2e}")
+    # iterations to the plain solver's error: every plain iteration count below
+    # the workload's and every warm-started one up to it, from the same state
+    table = {"plain": {iters: plain}, "learned": {args.learned_iterations: warm}}
+    for k in range(1, iters + 1):
+        if k not in table["plain"]:
+            table["plain"][k] = timed_run(params, st, steps, "off", iterations=k)
+        if k not in table["learned"]:
+            table["learned"][k] = timed_run(params, st, steps, "warm", model, iterations=k)
+    target = plain["mean_density_error"]
+    plain_k = min(k for k, r in table["plain"].items() if r["mean_density_error"] <= target)
+    warm_ks = [k for k, r in table["learned"].items() if r["mean_density_error"] <= target]
+    warm_k = min(warm_ks) if warm_ks else None
+    log("iterations to the plain error %.2e: plain %d, learned %s" % (target, plain_k, warm_k))
     model.close()
-    return {"steps": steps, "training": rep, "plain": plain, "learned": warm,
-            "tolerance": 1e-3}
+    brief = {side: {str(k): {"mean_density_error": r["mean_density_error"],
+                             "mean_abs_density_error": r["mean_abs_density_error"], "ms_per_step": r["ms_per_step"]}
+                    for k, r in sorted(rows.items())} for side, rows in table.items()}
+    return {"steps": steps, "training": rep, "plain": plain, "learned": warm, "tolerance": 1e-3,
+            "by_iterations": brief, "plain_error_target": target,
+            "plain_iterations_to_target": plain_k, "learned_iterations_to_target": warm_k,
+            "iteration_ratio": (plain_k / warm_k) if warm_k else None}
 
 
 def run_batch(args, log):
@@ -235,8 +255,11 @@ def main():
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's solver iterations (studies of iterations to tolerance)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
-    ap.add_argument("--learned-iterations", type=int, default=3,
+    ap.add_argument("--learned-iterations", type=int, default=1,
                     help="Jacobi iterations after the learned warm start")
+    ap.add_argument("--learned-damping", type=float, default=1.0, help="scale of the predicted lambdas")
+    ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--learned-harvest", type=int, default=6, help="plain substeps harvested for training")
     ap.add_argument("--hash-res", type=int, default=None,
                     help="override the workload's neighbour grid resolution (cells per axis = 2^res)")
     ap.add_argument("--launch-shape", default=None,
