@@ -87,10 +87,9 @@ APBF_HD float sqrt_le_bound(float r) {
     return t;
 }
 
-APBF_HD float exp_f(float x) {
-    if (x != x) return x;
-    if (x <= -86.0f) return 0.0f;
-    if (x > 88.0f) return __builtin_inff();
+// apbf_exp on an argument in (-86, 88]: Cody & Waite reduction and the
+// degree-7 polynomial.
+APBF_HD float exp_core(float x) {
     const float log2e = bits_f(0x3FB8AA3Bu);
     const float ln2_hi = bits_f(0x3F317200u);
     const float ln2_lo = bits_f(0x35BFBE8Eu);
@@ -107,6 +106,42 @@ APBF_HD float exp_f(float x) {
     p = __builtin_fmaf(p, r, 1.0f);
     int ki = (int)k;
     return p * bits_f((uint32_t)(ki + 127) << 23);
+}
+APBF_HD float exp_f(float x) {
+    if (x != x) return x;
+    if (x <= -86.0f) return 0.0f;
+    if (x > 88.0f) return __builtin_inff();
+    return exp_core(x);
+}
+
+// IEEE square root and reciprocal for arguments in a known range. On the
+// device the square root is v_sqrt_f32 with its one-ulp correction (the full
+// sequence without its scaling of tiny arguments and its special cases) and
+// the reciprocal v_rcp_f32 with one Newton step on fma; both give the IEEE
+// bits for x = 0 or 2^-96 <= x <= 2^120 (square root) and for
+// 2^-120 <= |x| <= 2^120 (reciprocal), checked on every such float by
+// apbf_check_math. Callers keep their arguments in those ranges.
+APBF_HD float sqrt_in_range(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = bits_f(__builtin_bit_cast(uint32_t, s) - 1u);
+    const float sp = bits_f(__builtin_bit_cast(uint32_t, s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x);
+    const float rp = __builtin_fmaf(-sp, s, x);
+    const float r = (rm <= 0.0f) ? sm : s;
+    return (rp > 0.0f) ? sp : r;
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+APBF_HD float recip_in_range(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+#else
+    return 1.0f / x;
+#endif
 }
 
 APBF_HD double log_d(double x) {

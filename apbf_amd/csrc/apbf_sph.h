@@ -24,11 +24,15 @@ APBF_HD GaussConst gauss_const(float kw) {
 APBF_HD float gauss_height(v3f r, GaussConst g) {  // kernels.glsl:84-89
     return exp_f(-dot(r, r) * g.ivd) * g.norm;
 }
-APBF_HD v3f gauss_gradient(v3f r, GaussConst g, float height_value) {  // kernels.glsl:91-97
-    float dist = length(r);
+// kernels.glsl:91-97. r is a difference of fixed-point positions over 2^18,
+// so dot(r, r) is 0 or lies in [2^-36, 2^31], and the length is >= 1e-4 when
+// it is inverted: the in-range square root and reciprocal give the IEEE
+// results of length(r) and normalize(r) = r * (1 / length(r)).
+APBF_HD v3f gauss_gradient(v3f r, GaussConst g, float height_value) {
+    const float dist = sqrt_in_range(dot(r, r));
     if (dist < 0.0001f) return mk(0, 0, 0);
-    float s = -height_value * 2.0f * dist * g.ivd;
-    return normalize(r) * s;
+    const float s = -height_value * 2.0f * dist * g.ivd;
+    return (r * recip_in_range(dist)) * s;
 }
 
 APBF_HD float poly6_height(v3f r, float h) {  // kernels.glsl:3-9

@@ -351,7 +351,59 @@ __global__ __launch_bounds__(BLOCK) void k_check_conversions(uint32_t stride, un
     if (bad) atomicAdd(mismatches, bad);
 }
 
+// Every float bit pattern in the stated ranges through sqrt_in_range /
+// recip_in_range and through the IEEE square root and 1/x: the number of
+// patterns where the bits differ and the smallest such pattern, per function
+// (slot 0 is unused).
+__global__ __launch_bounds__(BLOCK) void k_check_math(unsigned long long* mismatches, uint32_t* first_bad) {
+    unsigned long long bad[3] = {0, 0, 0};
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (1ull << 32);
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)k;
+        const float x = bits_f(b);
+        const float ax = __builtin_fabsf(x);
+        bool d[3] = {false, false, false};
+        if (x == 0.0f || (x >= 0x1p-96f && x <= 0x1p120f))
+            d[1] = __builtin_bit_cast(uint32_t, sqrt_in_range(x)) != __builtin_bit_cast(uint32_t, __builtin_sqrtf(x));
+        if (ax >= 0x1p-120f && ax <= 0x1p120f)
+            d[2] = __builtin_bit_cast(uint32_t, recip_in_range(x)) != __builtin_bit_cast(uint32_t, 1.0f / x);
+        for (int f = 1; f < 3; ++f)
+            if (d[f]) {
+                ++bad[f];
+                atomicMin(&first_bad[f], b);
+            }
+    }
+    for (int f = 0; f < 3; ++f)
+        if (bad[f]) atomicAdd(&mismatches[f], bad[f]);
+}
+
 }  // namespace apbf
+
+extern "C" apbf_status apbf_check_math(uint64_t* mismatches3, uint32_t* first_bad3) {
+    using namespace apbf;
+    if (!mismatches3 || !first_bad3) return APBF_ERR_INVALID_ARGUMENT;
+    void* mem = nullptr;
+    if (hipMalloc(&mem, 64) != hipSuccess) return APBF_ERR_HIP;
+    unsigned long long* d_bad = static_cast<unsigned long long*>(mem);
+    uint32_t* d_first = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + 32);
+    const uint64_t zero[4] = {0, 0, 0, 0};
+    const uint32_t ones[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    apbf_status st = APBF_OK;
+    if (hipMemcpy(d_bad, zero, 32, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_first, ones, 16, hipMemcpyHostToDevice) != hipSuccess)
+        st = APBF_ERR_HIP;
+    if (st == APBF_OK) {
+        hipLaunchKernelGGL(k_check_math, dim3(4096), dim3(BLOCK), 0, nullptr, d_bad, d_first);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) st = APBF_ERR_HIP;
+    }
+    unsigned long long bad[3] = {0, 0, 0};
+    if (st == APBF_OK && (hipMemcpy(bad, d_bad, 24, hipMemcpyDeviceToHost) != hipSuccess ||
+                          hipMemcpy(first_bad3, d_first, 12, hipMemcpyDeviceToHost) != hipSuccess))
+        st = APBF_ERR_HIP;
+    for (int f = 0; f < 3; ++f) mismatches3[f] = bad[f];
+    (void)hipFree(mem);
+    return st;
+}
 
 extern "C" apbf_status apbf_check_conversions(uint32_t stride, uint64_t* mismatches, uint32_t* first_bad,
                                               int32_t* sample_i32, uint32_t* sample_u32) {

@@ -423,6 +423,12 @@ int32_t apbf_device_count(void);
  * in the reference. */
 apbf_status apbf_check_conversions(uint32_t stride, uint64_t* mismatches, uint32_t* first_bad,
                                    int32_t* sample_i32, uint32_t* sample_u32);
+/* Self-check of the device square root and reciprocal the Gaussian gradient
+ * uses (IEEE sqrt and 1/x, kernels.glsl:91-97): every float bit pattern of
+ * their argument ranges through the short device forms and through the full
+ * IEEE forms. mismatches3[f] / first_bad3[f] for f = 1 square root,
+ * 2 reciprocal (slot 0 unused). No counterpart in the reference. */
+apbf_status apbf_check_math(uint64_t* mismatches3, uint32_t* first_bad3);
 
 #ifdef __cplusplus
 }

@@ -34,3 +34,15 @@ def test_conversions_match_definition_on_every_input(device):
     assert np.array_equal(su.astype(np.int64), want_u)
     # the sample holds NaNs, values beyond both saturation limits and values below 1 in magnitude
     assert np.isnan(x).any() and (x > 2.0 ** 32).any() and (x < -2.0 ** 31).any() and (np.abs(x) < 1).any()
+
+
+def test_short_sqrt_and_reciprocal_equal_ieee(device):
+    """sqrt_in_range and recip_in_range (hardware approximation plus one
+    correction) give the IEEE bits on every float of their argument ranges."""
+    from apbf_amd import _lib
+    L = _lib.lib()
+    bad = np.full(3, 7, np.uint64)
+    first = np.zeros(3, np.uint32)
+    _lib.check(L.apbf_check_math(bad.ctypes.data_as(C.c_void_p), first.ctypes.data_as(C.c_void_p)), "apbf_check_math")
+    names = ("unused", "sqrt", "reciprocal")
+    assert not bad.any(), {names[f]: (int(bad[f]), hex(first[f])) for f in range(3) if bad[f]}
