@@ -581,13 +581,24 @@ void Scene::spread_end() {
 }
 
 // update_transfers.cpp:14-71
-void Scene::update_transfers() {
+// With `with_density_error` the substep's density-error measurement rides on
+// the first pass over the rows (k_fsm12_density_rows).
+void Scene::update_transfers(bool with_density_error) {
     hipStream_t s = stream;
     uint32_t* old_bd = u0;
     uint32_t* nearest = u1;
     LAUNCH(k_copy_u32_n, stride_grid(cap), old_bd, boundary_distance[cur], counts);
     bool tk = tbegin(K_FSM12);
-    launch_fsm12_rows(pos[cur], counts, cap, row, col, pcap, old_bd, boundary_distance[cur], nearest, s);
+    if (with_density_error) {
+        launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                             P.s.mGradientKernelId, gc, s);
+        launch_fsm12_density_rows(pos[cur], inv_mass[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
+                                  old_bd, boundary_distance[cur], nearest, P.s.mHeightKernelId,
+                                  P.s.mGradientKernelId, part_sum, part_max, s);
+        launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
+    } else {
+        launch_fsm12_rows(pos[cur], counts, cap, row, col, pcap, old_bd, boundary_distance[cur], nearest, s);
+    }
     tend(tk);
     Fsm3Args fa;
     fa.update_tr = P.s.mUpdateTargetRadius;

@@ -78,6 +78,10 @@ void launch_density_rows(const int4* pos, const float* inv_mass, const float* ra
                          const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                          const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
                          float* part_max, hipStream_t s);
+void launch_fsm12_density_rows(const int4* pos, const float* inv_mass, const float* kw, const float4* gc,
+                               const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                               uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest, int hid,
+                               int gid, double* part_sum, float* part_max, hipStream_t s);
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,
                        hipStream_t s);

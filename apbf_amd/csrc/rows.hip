@@ -579,6 +579,134 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_rows(const int4* __restrict__ p
     ROW_LOOP_END
 }
 
+// k_fsm12_rows and k_density_rows in one pass: after the solver loop the
+// substep's error measurement and the split/merge find read the same rows at
+// the same positions, so one walk over the rows loads them once. The grid is
+// that of k_density_rows (its per-block partial sums).
+template <int L, int KH>
+__global__ __launch_bounds__(BLOCK) void k_fsm12_density_rows(
+    const int4* __restrict__ pos, const float* __restrict__ inv_mass, const float* __restrict__ kw,
+    const float4* __restrict__ gc, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
+    const uint32_t* __restrict__ col, uint32_t pcap, const uint32_t* __restrict__ old_bd, uint32_t* __restrict__ bd,
+    uint32_t* __restrict__ nearest, int hid, int gid, double* __restrict__ part_sum, float* __restrict__ part_max) {
+    __shared__ double ssum[BLOCK];
+    __shared__ double sabs[BLOCK];
+    __shared__ float smax[BLOCK];
+    double acc_sum = 0.0, acc_abs = 0.0;
+    float acc_max = 0.0f;
+    const uint32_t rows = counts[C_ROWS];
+    ROW_LOOP_BEGIN(L)
+    int4 pi = make_int4(0, 0, 0, 0);
+    float kwi = 1.0f, ird = 1.0f;
+    uint32_t self = 0;
+    GaussConst g{1.0f, 1.0f};
+    uint32_t rb = 0, re = 0, rbd = 0, red = 0;
+    if (active) {
+        pi = pos[i];
+        kwi = kw[i];
+        const float4 g4 = gc[i];
+        g = GaussConst{g4.x, g4.y};
+        ird = g4.z;
+        self = __builtin_bit_cast(uint32_t, g4.w);
+        rb = row[i];
+        re = min(row[i + 1], pcap);
+        if (i < rows) {  // the density pass covers the rows of the current list
+            rbd = rb;
+            red = re;
+        }
+    }
+    uint32_t bmin = 0xFFFFFFFFu, best = 0xFFFFFFFFu, best_k = 0u, dens = 0;
+    bool found = false;
+    v3f unused;
+    auto entry = [&](uint32_t k, const int4 pj, uint32_t obd, float imN) {
+        const int32_t dx = isub(pj.x, pi.x), dy = isub(pj.y, pi.y), dz = isub(pj.z, pi.z);
+        // find_split_and_merge_1/2 (as k_fsm12_rows)
+        const uint32_t dist = to_u32(length(vec_i(dx, dy, dz)));
+        const uint32_t cand = obd + dist;
+        bmin = cand < bmin ? cand : bmin;
+        if (!found || dist <= best) {
+            best = dist;
+            best_k = k;
+            found = true;
+        }
+        // the density term (as k_density_rows)
+        if (k >= rbd && k < red) {
+            const v3f diff = vec_i(dx, dy, dz) / POS_RES;
+            float w;
+            kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
+            const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
+            const uint32_t ie = (ib >> 23) & 0xFFu;
+            float wq;
+            if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u))  // exact scaling, as in k_solve_rows
+                wq = w * __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
+            else
+                wq = w / imN;
+            dens += to_u32(wq * INCOMP_RES);
+        }
+    };
+    uint32_t k = rb + gl;
+    for (; k + 3u * L < re; k += 4u * L) {
+        uint32_t jb[4], ob[4];
+        int4 pb[4];
+        float mb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) jb[t] = col[k + (uint32_t)t * L];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            pb[t] = pos[jb[t]];
+            ob[t] = old_bd[jb[t]];
+            mb[t] = inv_mass[jb[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) entry(k + (uint32_t)t * L, pb[t], ob[t], mb[t]);
+    }
+    for (; k < re; k += L) {
+        const uint32_t j = col[k];
+        entry(k, pos[j], old_bd[j], inv_mass[j]);
+    }
+    bmin = gmin<L>(bmin);
+    dens = gsum<L>(dens);
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) {
+        const uint32_t ob = __shfl_xor(best, m, 64);
+        const uint32_t ok = __shfl_xor(best_k, m, 64);
+        const bool of = __shfl_xor((int)found, m, 64) != 0;
+        if (of && (!found || ob < best || (ob == best && ok > best_k))) {
+            best = ob;
+            best_k = ok;
+            found = true;
+        }
+    }
+    if (active && gl == 0) {
+        bd[i] = bmin;
+        nearest[i] = found ? col[best_k] : i;
+        dens += self;
+        const float dev = (float)dens / INCOMP_RES * ird - 1.0f;
+        const float e = fmax_g(0.0f, dev);
+        acc_sum += (double)e;
+        acc_abs += (double)__builtin_fabsf(dev);
+        acc_max = fmax_g(acc_max, e);
+    }
+    ROW_LOOP_END
+    ssum[threadIdx.x] = acc_sum;
+    sabs[threadIdx.x] = acc_abs;
+    smax[threadIdx.x] = acc_max;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            sabs[threadIdx.x] += sabs[threadIdx.x + s];
+            smax[threadIdx.x] = fmax_g(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_sum[blockIdx.x] = ssum[0];
+        part_sum[gridDim.x + blockIdx.x] = sabs[0];
+        part_max[blockIdx.x] = smax[0];
+    }
+}
+
 // kernel_width.comp:22-56 over row i. Pass FILL = false: spread the original
 // kernel width to every neighbour (atomicMax, order-free) and count the pairs
 // within the cutoff; FILL = true: write those pairs from row2[i] in row order.
@@ -691,6 +819,19 @@ void launch_density_rows(const int4* pos, const float* inv_mass, const float* ra
     else
         hipLaunchKernelGGL((k_density_rows<ROW_LANES, -1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos,
                            inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+}
+void launch_fsm12_density_rows(const int4* pos, const float* inv_mass, const float* kw, const float4* gc,
+                               const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
+                               uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest, int hid,
+                               int gid, double* part_sum, float* part_max, hipStream_t s) {
+    if (hid == 1)
+        hipLaunchKernelGGL((k_fsm12_density_rows<ROW_LANES, 1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s,
+                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, part_sum,
+                           part_max);
+    else
+        hipLaunchKernelGGL((k_fsm12_density_rows<ROW_LANES, -1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s,
+                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, part_sum,
+                           part_max);
 }
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,

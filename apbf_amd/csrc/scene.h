@@ -135,7 +135,7 @@ struct Scene {
     void spread_kernel_width();
     void box_collision();
     void incompressibility();
-    void update_transfers();
+    void update_transfers(bool with_density_error = false);
     void update(float dt);
     apbf_status stage(int st, float dt);
     apbf_status density_error(double* mean_err, double* max_err);

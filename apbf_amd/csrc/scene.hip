@@ -368,8 +368,12 @@ void Scene::update(float dt) {
     }
     solver_loop();
     launch_count_particle_substeps(counts, particle_substeps, stream);
-    if (measure_error) accumulate_density_error();
-    if (!P.basic_pbf) update_transfers();
+    if (measure_error && !P.basic_pbf) {
+        update_transfers(true);  // the error pass rides on the split/merge find
+    } else {
+        if (measure_error) accumulate_density_error();
+        if (!P.basic_pbf) update_transfers();
+    }
 }
 
 void Scene::accumulate_density_error() {

@@ -271,3 +271,25 @@ def test_fused_walk_raw_count_matches_oracle(device, seed, res):
         for name in ["spread"] + ["box", "incompressibility"] * p["solver_iterations"] + ["update_transfers"]:
             r.stage(name, p["dt"])
         _check(g, r, f"substep {step}")
+
+
+def test_adaptive_density_error_matches_oracle(device):
+    """The density error measured inside the adaptive substep (after the solver
+    loop, in the split/merge find's pass over the rows) equals the oracle's
+    density error of the same state."""
+    p, st = cfg.small_water_drop(seed=4, active=True)
+    g, r = _pair(p, st)
+    g.set_error_measurement(True)
+    run_max = 0.0
+    for step in range(4):
+        g.update(p["dt"])
+        for name in adaptive_stages(p)[:-1]:  # up to the end of the solver loop
+            r.stage(name, p["dt"])
+        m_r, x_r = r.density_error()
+        r.stage("update_transfers", p["dt"])
+        _check(g, r, f"substep {step}")
+        _, x_g, last_g, _, n_g = g.error_stats()
+        run_max = max(run_max, x_r)
+        assert n_g == step + 1
+        assert abs(last_g - m_r) <= 1e-12 + 1e-9 * abs(m_r), (step, last_g, m_r)
+        assert x_g == run_max
