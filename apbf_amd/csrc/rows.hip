@@ -12,9 +12,9 @@
 //   k_solve_rows: incompressibility_0/1 sums over row i, incompressibility_2
 //     (lambda, boundariness), then the position shifts of
 //     incompressibility_2.comp:106-109 (particle i) and _3.comp:62-66 (every
-//     neighbour j of row i, with the gradient of the first loop) added with
-//     int32 atomics into a per-particle accumulator, and the boundariness
-//     filter of _3.comp:41-46;
+//     neighbour j of row i, with the gradient of the first loop), summed per
+//     block in LDS and added with int32 atomics into a per-particle
+//     accumulator, and the boundariness filter of _3.comp:41-46;
 //   k_solve_apply: position += accumulator (and the box collision that opens
 //     the next solver iteration).
 // Positions are read-only during k_solve_rows, as in the reference, where
@@ -77,6 +77,70 @@ __device__ __forceinline__ void kernel_eval_k(int hid, int gid, v3f r, float kw,
 // recompute the rest.
 constexpr int SOLVE_CACHE = 8;
 
+// The position shifts of k_solve_rows are summed per block in a table in LDS
+// before they reach the per-particle accumulator. A block takes AGG_STEPS
+// groups of consecutive rows, and consecutive particles are neighbours (the
+// particles are sorted by cell), so the rows share most neighbours and one
+// global atomic per neighbour and component replaces one per pair; where
+// many rows push the same particles (a fluid at rest) this removes the
+// contention of the global atomics. Integer sums wrap modulo 2^32 in any
+// order, so the accumulator ends with the same bits. The table maps a
+// particle id to slot (id mod AGG_SLOTS) and probes the next AGG_PROBES - 1
+// slots; an id that finds no free slot goes straight to the accumulator.
+constexpr int AGG_SLOTS = 1024;
+constexpr int AGG_PROBES = 4;
+constexpr uint32_t AGG_STEPS = 4;  // row groups per table flush (64 rows at 16 lanes per row)
+constexpr uint32_t AGG_EMPTY = 0xFFFFFFFFu;
+
+struct ShiftTable {
+    uint32_t* key;  // AGG_SLOTS particle ids (AGG_EMPTY: free)
+    int32_t* sum;   // 3 x AGG_SLOTS: x sums, y sums, z sums
+    int* acc;       // per-particle accumulator (x, y, z, unused)
+
+    __device__ __forceinline__ void clear() {
+        for (int t = threadIdx.x; t < AGG_SLOTS; t += BLOCK) {
+            key[t] = AGG_EMPTY;
+            sum[t] = 0;
+            sum[AGG_SLOTS + t] = 0;
+            sum[2 * AGG_SLOTS + t] = 0;
+        }
+    }
+    __device__ __forceinline__ void add(uint32_t j, int32_t dx, int32_t dy, int32_t dz) {
+        if ((dx | dy | dz) == 0) return;
+#pragma unroll
+        for (int p = 0; p < AGG_PROBES; ++p) {
+            const uint32_t t = (j + (uint32_t)p) & (AGG_SLOTS - 1u);
+            const uint32_t k = atomicCAS(&key[t], AGG_EMPTY, j);
+            if (k == AGG_EMPTY || k == j) {
+                atomicAdd(&sum[t], dx);
+                atomicAdd(&sum[AGG_SLOTS + t], dy);
+                atomicAdd(&sum[2 * AGG_SLOTS + t], dz);
+                return;
+            }
+        }
+        if (dx) atomicAdd(&acc[4 * j + 0], dx);
+        if (dy) atomicAdd(&acc[4 * j + 1], dy);
+        if (dz) atomicAdd(&acc[4 * j + 2], dz);
+    }
+    // Adds the table to the accumulator and empties it; the block's threads
+    // must have finished their add() calls (a barrier before) and must not
+    // start new ones until every thread is done (a barrier after).
+    __device__ __forceinline__ void flush() {
+        for (int t = threadIdx.x; t < AGG_SLOTS; t += BLOCK) {
+            const uint32_t j = key[t];
+            if (j == AGG_EMPTY) continue;
+            const int32_t x = sum[t], y = sum[AGG_SLOTS + t], z = sum[2 * AGG_SLOTS + t];
+            if (x) atomicAdd(&acc[4 * j + 0], x);
+            if (y) atomicAdd(&acc[4 * j + 1], y);
+            if (z) atomicAdd(&acc[4 * j + 2], z);
+            key[t] = AGG_EMPTY;
+            sum[t] = 0;
+            sum[AGG_SLOTS + t] = 0;
+            sum[2 * AGG_SLOTS + t] = 0;
+        }
+    }
+};
+
 template <int L, int KH, int KG, int C>
 __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ pos,
                                                       const float* __restrict__ inv_mass,
@@ -89,200 +153,212 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                                                       SolverArgs a, float* __restrict__ boundariness,
                                                       float* __restrict__ lambda, int* __restrict__ acc) {
     __shared__ double s_err;
+    __shared__ uint32_t s_key[AGG_SLOTS];
+    __shared__ int32_t s_sum[3 * AGG_SLOTS];
+    ShiftTable tab{s_key, s_sum, acc};
     if (threadIdx.x == 0) s_err = 0.0;
+    tab.clear();
     __syncthreads();
-    ROW_LOOP_BEGIN(L)
-    int4 pi = make_int4(0, 0, 0, 0);
-    float im = 1.0f, kwi = 1.0f, ird = 1.0f;
-    uint32_t self = 0;
-    GaussConst g{1.0f, 1.0f};
-    uint32_t rb = 0, re = 0;
-    if (active) {
-        pi = pos[i];
-        im = inv_mass[i];
-        kwi = kw[i];
-        const float4 g4 = gc[i];
-        g = GaussConst{g4.x, g4.y};
-        ird = g4.z;
-        self = __builtin_bit_cast(uint32_t, g4.w);
-        rb = row[i];
-        re = min(row[i + 1], pcap);
-    }
-    // incompressibility_1.comp:43-68 over this lane's share of row i
-    uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    uint32_t cj[C];
-    v3f cg[C];
-    auto pair1 = [&](const int4 pj, const float imN, v3f& g_out) {
-        const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-        float w;
-        v3f grad;
-        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
-        const float gg = dot(grad, grad);
-        float wq, ggq;
-        v3f wg;
-        // Inverse masses are powers of two unless a merge mixed two masses.
-        // Dividing by a power of two whose reciprocal is a normal float is the
-        // exact scaling by that reciprocal (one rounding of the same real
-        // value), so the quotients below are the IEEE quotients either way.
-        const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
-        const uint32_t ie = (ib >> 23) & 0xFFu;
-        if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u)) {
-            const float rim = __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
-            wq = w * rim;
-            wg = grad * rim;
-            ggq = gg * rim;
-        } else {
-            wq = w / imN;
-            wg = grad / imN;
-            ggq = gg / imN;
-        }
-        dens += to_u32(wq * INCOMP_RES);
-        s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
-        s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
-        s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
-        sq += to_u32(ggq * INCOMP_RES);
-        if (a.method == 1) {
-            const v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / (imN * kwi);
-            m0 += (uint32_t)to_i32(wd.x);
-            m1 += (uint32_t)to_i32(wd.y);
-            m2 += (uint32_t)to_i32(wd.z);
-            m3 += (uint32_t)to_i32(INCOMP_RES / imN);
-        }
-        g_out = grad;
-    };
-    // The cached part of the row goes in batches of B entries per lane: the
-    // batch's list entries, then its neighbours' positions and masses, are
-    // loaded before any arithmetic, so a lane waits for memory twice per
-    // batch instead of twice per entry. Lanes past the row end load the row's
-    // last entry and discard it.
-    constexpr int B = (C % 8 == 0) ? 8 : ((C % 4 == 0) ? 4 : C);
-    static_assert(C % B == 0, "the cached entries split into whole batches");
-#pragma unroll
-    for (int t = 0; t < C; ++t) {
-        cj[t] = 0;
-        cg[t] = mk(0, 0, 0);
-    }
-    if (rb < re) {
-        const uint32_t klast = re - 1u;
-#pragma unroll
-        for (int t0 = 0; t0 < C; t0 += B) {
-            int4 bp[B];
-            float bm[B];
-#pragma unroll
-            for (int t = 0; t < B; ++t) cj[t0 + t] = col[min(rb + gl + (uint32_t)(t0 + t) * L, klast)];
-#pragma unroll
-            for (int t = 0; t < B; ++t) {
-                bp[t] = pos[cj[t0 + t]];
-                bm[t] = inv_mass[cj[t0 + t]];
+    // A block takes AGG_STEPS consecutive groups of BLOCK / L rows, then
+    // flushes the table; the trip counts are uniform per block, so every
+    // thread meets the barriers.
+    const uint32_t n = counts[C_N];
+    const int gl = threadIdx.x & (L - 1);
+    constexpr uint32_t groups = BLOCK / L;
+    constexpr uint32_t span = groups * AGG_STEPS;
+    for (uint32_t base = blockIdx.x * span; base < n; base += gridDim.x * span) {
+        for (uint32_t step = 0; step < AGG_STEPS; ++step) {
+            const uint32_t i = base + step * groups + threadIdx.x / L;
+            const bool active = i < n;
+            int4 pi = make_int4(0, 0, 0, 0);
+            float im = 1.0f, kwi = 1.0f, ird = 1.0f;
+            uint32_t self = 0;
+            GaussConst g{1.0f, 1.0f};
+            uint32_t rb = 0, re = 0;
+            if (active) {
+                pi = pos[i];
+                im = inv_mass[i];
+                kwi = kw[i];
+                const float4 g4 = gc[i];
+                g = GaussConst{g4.x, g4.y};
+                ird = g4.z;
+                self = __builtin_bit_cast(uint32_t, g4.w);
+                rb = row[i];
+                re = min(row[i + 1], pcap);
             }
-#pragma unroll
-            for (int t = 0; t < B; ++t)
-                if (rb + gl + (uint32_t)(t0 + t) * L < re) pair1(bp[t], bm[t], cg[t0 + t]);
-        }
-    }
-    for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
-        const uint32_t j = col[k];
-        v3f gr;
-        pair1(pos[j], inv_mass[j], gr);
-    }
-    dens = gsum<L>(dens);
-    sq = gsum<L>(sq);
-    s0 = gsum<L>(s0);
-    s1 = gsum<L>(s1);
-    s2 = gsum<L>(s2);
-    if (a.method == 1) {
-        m0 = gsum<L>(m0);
-        m1 = gsum<L>(m1);
-        m2 = gsum<L>(m2);
-        m3 = gsum<L>(m3);
-    }
-    dens += self;  // incompressibility_0.comp:43: the particle's own contribution
-    // incompressibility_2.comp:77-109 (every lane of the group computes the same values)
-    const int32_t gs0 = (int32_t)s0, gs1 = (int32_t)s1, gs2 = (int32_t)s2;
-    const float densf = (float)dens / INCOMP_RES;
-    const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
-    float sgs = (float)sq / INCOMP_RES;
-    const v3f selfGrad = (-wgs) * ird;
-    const float wgss = dot(wgs, wgs);
-    const float sgl = __builtin_sqrtf(wgss) * ird;
-    sgs += wgss * im;
-    const float under = 1.0f - densf * ird;
-    float b = active ? boundariness[i] : 0.0f;
-    if (a.update_b && !a.lam_override) {
-        float bb;
-        if (a.method == 1) {
-            const float totalMass = (float)(int32_t)m3 / INCOMP_RES + 1.0f / im;
-            const float dev = length(vec_i((int32_t)m0, (int32_t)m1, (int32_t)m2)) / (POS_RES * totalMass);
-            bb = dev * a.b_sgl_factor;
-        } else {
-            bb = sgl * kwi * a.b_sgl_factor + fmax_g(0.0f, under) * a.b_under_factor;
-        }
-        bb = bb >= 1.0f ? 1.0f : 0.0f;
-        b = fmin_g(1.0f, move_towards_add(b, bb, a.b_speed));
-    }
-    float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
-    lam /= a.lambda_scale_base / ird * im;
-    if (a.lam_override) lam = active ? a.lam_override[i] : 0.0f;
-    if (active && gl == 0) {
-        if (a.it_err) atomicAdd(&s_err, (double)fmax_g(0.0f, densf * ird - 1.0f));
-        if (a.lam_acc) a.lam_acc[i] += fminf(lam, 0.0f);
-        lambda[i] = lam;
-        if (!(lam >= 0.0f)) {  // own shift; a NaN lambda shifts by ivec3(NaN) = 0
-            const v3f d = selfGrad * (lam * im * POS_RES);
-            const int32_t dx = to_i32(d.x), dy = to_i32(d.y), dz = to_i32(d.z);
-            if (dx) atomicAdd(&acc[4 * i + 0], dx);
-            if (dy) atomicAdd(&acc[4 * i + 1], dy);
-            if (dz) atomicAdd(&acc[4 * i + 2], dz);
-        }
-    }
-    // neighbour shifts (incompressibility_3.comp:62-66) and the boundariness
-    // filter (:41-46), over this lane's share of row i, with the gradients of
-    // the first loop (positions are unchanged in between)
-    const bool do_shift = active && lam < 0.0f;
-    const bool do_filter = active && a.method == 2 && !a.lam_override && __builtin_bit_cast(uint32_t, b) != 0u;
-    bool hit = false;
-    if (do_shift || do_filter) {
-        const float sc = lam * POS_RES;
-        const v3f empty = -normalize(vec_i(gs0, gs1, gs2));
-        auto pair3 = [&](uint32_t j, v3f grad) {
-            if (do_filter && !hit) {
-                // cos(empty, grad) > 0.6 needs dot(empty, grad) > 0 and
-                // dot^2 > 0.36 |grad|^2; with a 3% margin over the rounding
-                // of either form, entries that fail the squared test cannot
-                // pass the normalised one, which then runs only for the rest
-                // (and for gradients too short for the squares to be exact).
-                const float dg = dot(empty, grad), gg = dot(grad, grad);
-                const bool maybe = !(gg >= 1e-30f) || (dg > 0.0f && dg * dg > 0.35f * gg);
-                if (maybe && dot(empty, normalize(grad)) > 0.6f) hit = true;
+            // incompressibility_1.comp:43-68 over this lane's share of row i
+            uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+            uint32_t cj[C];
+            v3f cg[C];
+            auto pair1 = [&](const int4 pj, const float imN, v3f& g_out) {
+                const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+                float w;
+                v3f grad;
+                kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+                const float gg = dot(grad, grad);
+                float wq, ggq;
+                v3f wg;
+                // Inverse masses are powers of two unless a merge mixed two masses.
+                // Dividing by a power of two whose reciprocal is a normal float is the
+                // exact scaling by that reciprocal (one rounding of the same real
+                // value), so the quotients below are the IEEE quotients either way.
+                const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
+                const uint32_t ie = (ib >> 23) & 0xFFu;
+                if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u)) {
+                    const float rim = __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
+                    wq = w * rim;
+                    wg = grad * rim;
+                    ggq = gg * rim;
+                } else {
+                    wq = w / imN;
+                    wg = grad / imN;
+                    ggq = gg / imN;
+                }
+                dens += to_u32(wq * INCOMP_RES);
+                s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
+                s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
+                s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
+                sq += to_u32(ggq * INCOMP_RES);
+                if (a.method == 1) {
+                    const v3f wd = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / (imN * kwi);
+                    m0 += (uint32_t)to_i32(wd.x);
+                    m1 += (uint32_t)to_i32(wd.y);
+                    m2 += (uint32_t)to_i32(wd.z);
+                    m3 += (uint32_t)to_i32(INCOMP_RES / imN);
+                }
+                g_out = grad;
+            };
+            // The cached part of the row goes in batches of B entries per lane: the
+            // batch's list entries, then its neighbours' positions and masses, are
+            // loaded before any arithmetic, so a lane waits for memory twice per
+            // batch instead of twice per entry. Lanes past the row end load the row's
+            // last entry and discard it.
+            constexpr int B = (C % 8 == 0) ? 8 : ((C % 4 == 0) ? 4 : C);
+            static_assert(C % B == 0, "the cached entries split into whole batches");
+        #pragma unroll
+            for (int t = 0; t < C; ++t) {
+                cj[t] = 0;
+                cg[t] = mk(0, 0, 0);
             }
-            if (do_shift) {
-                const v3f d = grad * sc;
-                const int32_t dx = to_i32(d.x), dy = to_i32(d.y), dz = to_i32(d.z);
-                if (dx) atomicAdd(&acc[4 * j + 0], dx);
-                if (dy) atomicAdd(&acc[4 * j + 1], dy);
-                if (dz) atomicAdd(&acc[4 * j + 2], dz);
+            if (rb < re) {
+                const uint32_t klast = re - 1u;
+        #pragma unroll
+                for (int t0 = 0; t0 < C; t0 += B) {
+                    int4 bp[B];
+                    float bm[B];
+        #pragma unroll
+                    for (int t = 0; t < B; ++t) cj[t0 + t] = col[min(rb + gl + (uint32_t)(t0 + t) * L, klast)];
+        #pragma unroll
+                    for (int t = 0; t < B; ++t) {
+                        bp[t] = pos[cj[t0 + t]];
+                        bm[t] = inv_mass[cj[t0 + t]];
+                    }
+        #pragma unroll
+                    for (int t = 0; t < B; ++t)
+                        if (rb + gl + (uint32_t)(t0 + t) * L < re) pair1(bp[t], bm[t], cg[t0 + t]);
+                }
             }
-        };
-#pragma unroll
-        for (int t = 0; t < C; ++t) {
-            const uint32_t k = rb + gl + (uint32_t)t * L;
-            if (k < re) pair3(cj[t], cg[t]);
-        }
-        for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
-            const uint32_t j = col[k];
-            const int4 pj = pos[j];
-            const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-            float w;
-            v3f grad;
-            kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
-            pair3(j, grad);
-        }
-    }
-    const uint64_t ballot = __ballot(hit);
-    if (ballot & group_mask<L>()) b = 0.0f;
-    if (active && gl == 0) boundariness[i] = b;
-    ROW_LOOP_END
-    if (a.it_err) {  // the grid-stride trip count is uniform per block
+            for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
+                const uint32_t j = col[k];
+                v3f gr;
+                pair1(pos[j], inv_mass[j], gr);
+            }
+            dens = gsum<L>(dens);
+            sq = gsum<L>(sq);
+            s0 = gsum<L>(s0);
+            s1 = gsum<L>(s1);
+            s2 = gsum<L>(s2);
+            if (a.method == 1) {
+                m0 = gsum<L>(m0);
+                m1 = gsum<L>(m1);
+                m2 = gsum<L>(m2);
+                m3 = gsum<L>(m3);
+            }
+            dens += self;  // incompressibility_0.comp:43: the particle's own contribution
+            // incompressibility_2.comp:77-109 (every lane of the group computes the same values)
+            const int32_t gs0 = (int32_t)s0, gs1 = (int32_t)s1, gs2 = (int32_t)s2;
+            const float densf = (float)dens / INCOMP_RES;
+            const v3f wgs = vec_i(gs0, gs1, gs2) / INCOMP_RES;
+            float sgs = (float)sq / INCOMP_RES;
+            const v3f selfGrad = (-wgs) * ird;
+            const float wgss = dot(wgs, wgs);
+            const float sgl = __builtin_sqrtf(wgss) * ird;
+            sgs += wgss * im;
+            const float under = 1.0f - densf * ird;
+            float b = active ? boundariness[i] : 0.0f;
+            if (a.update_b && !a.lam_override) {
+                float bb;
+                if (a.method == 1) {
+                    const float totalMass = (float)(int32_t)m3 / INCOMP_RES + 1.0f / im;
+                    const float dev = length(vec_i((int32_t)m0, (int32_t)m1, (int32_t)m2)) / (POS_RES * totalMass);
+                    bb = dev * a.b_sgl_factor;
+                } else {
+                    bb = sgl * kwi * a.b_sgl_factor + fmax_g(0.0f, under) * a.b_under_factor;
+                }
+                bb = bb >= 1.0f ? 1.0f : 0.0f;
+                b = fmin_g(1.0f, move_towards_add(b, bb, a.b_speed));
+            }
+            float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
+            lam /= a.lambda_scale_base / ird * im;
+            if (a.lam_override) lam = active ? a.lam_override[i] : 0.0f;
+            if (active && gl == 0) {
+                if (a.it_err) atomicAdd(&s_err, (double)fmax_g(0.0f, densf * ird - 1.0f));
+                if (a.lam_acc) a.lam_acc[i] += fminf(lam, 0.0f);
+                lambda[i] = lam;
+                if (!(lam >= 0.0f)) {  // own shift; a NaN lambda shifts by ivec3(NaN) = 0
+                    const v3f d = selfGrad * (lam * im * POS_RES);
+                    tab.add(i, to_i32(d.x), to_i32(d.y), to_i32(d.z));
+                }
+            }
+            // neighbour shifts (incompressibility_3.comp:62-66) and the boundariness
+            // filter (:41-46), over this lane's share of row i, with the gradients of
+            // the first loop (positions are unchanged in between)
+            const bool do_shift = active && lam < 0.0f;
+            const bool do_filter = active && a.method == 2 && !a.lam_override && __builtin_bit_cast(uint32_t, b) != 0u;
+            bool hit = false;
+            if (do_shift || do_filter) {
+                const float sc = lam * POS_RES;
+                const v3f empty = -normalize(vec_i(gs0, gs1, gs2));
+                auto pair3 = [&](uint32_t j, v3f grad) {
+                    if (do_filter && !hit) {
+                        // cos(empty, grad) > 0.6 needs dot(empty, grad) > 0 and
+                        // dot^2 > 0.36 |grad|^2; with a 3% margin over the rounding
+                        // of either form, entries that fail the squared test cannot
+                        // pass the normalised one, which then runs only for the rest
+                        // (and for gradients too short for the squares to be exact).
+                        const float dg = dot(empty, grad), gg = dot(grad, grad);
+                        const bool maybe = !(gg >= 1e-30f) || (dg > 0.0f && dg * dg > 0.35f * gg);
+                        if (maybe && dot(empty, normalize(grad)) > 0.6f) hit = true;
+                    }
+                    if (do_shift) {
+                        const v3f d = grad * sc;
+                        tab.add(j, to_i32(d.x), to_i32(d.y), to_i32(d.z));
+                    }
+                };
+        #pragma unroll
+                for (int t = 0; t < C; ++t) {
+                    const uint32_t k = rb + gl + (uint32_t)t * L;
+                    if (k < re) pair3(cj[t], cg[t]);
+                }
+                for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
+                    const uint32_t j = col[k];
+                    const int4 pj = pos[j];
+                    const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+                    float w;
+                    v3f grad;
+                    kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+                    pair3(j, grad);
+                }
+            }
+            const uint64_t ballot = __ballot(hit);
+            if (ballot & group_mask<L>()) b = 0.0f;
+            if (active && gl == 0) boundariness[i] = b;
+        }  // step
+        __syncthreads();
+        tab.flush();
+        __syncthreads();
+    }  // base
+    if (a.it_err) {
         __syncthreads();
         if (threadIdx.x == 0 && s_err != 0.0) atomicAdd(a.it_err, s_err / (double)counts[C_N]);
     }
