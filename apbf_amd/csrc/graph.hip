@@ -181,38 +181,84 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
                                 float* best_ms) {
     if (!scene || substeps == 0) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return APBF_ERR_HIP;
-    const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
-    float best = 1e30f;
-    int bw = s->walk_lanes, bs = s->solve_lanes;
-    apbf_status st = APBF_OK;
-    for (int w : walks)
-        for (int v : solves) {
-            s->walk_lanes = w;
-            s->solve_lanes = v;
-            float ms = 0.0f;
-            if (hipEventRecord(e0, s->stream) != hipSuccess) st = APBF_ERR_HIP;
-            for (uint32_t k = 0; k < substeps; ++k) s->update(dt);
-            if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    // Every shape runs the same substeps from the same state: the scene is
+    // saved first and restored before each trial and at the end, together
+    // with its statistics, so that shapes tried later do not time a scene
+    // that has moved on, and tuning leaves the scene as it found it (except
+    // the neighbour list, which a restore empties until the next substep).
+    // Each trial runs one untimed substep, then times `substeps`; every shape
+    // is tried twice and keeps its faster time.
+    apbf_snapshot* snap = nullptr;
+    apbf_status st = apbf_snapshot_create(scene, &snap);
+    if (st != APBF_OK) return st;
+    const size_t err_bytes = 6 * sizeof(double), iter_bytes = MAX_ITER_RECORD * sizeof(double);
+    const size_t ps_bytes = 2 * sizeof(unsigned long long);
+    char* stats = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMalloc(&stats, err_bytes + iter_bytes + ps_bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        st = APBF_ERR_HIP;
+    auto copy_stats = [&](bool save) {
+        auto cp = [&](void* scene_buf, char* keep, size_t bytes) {
+            if (hipMemcpyAsync(save ? keep : scene_buf, save ? scene_buf : keep, bytes, hipMemcpyDeviceToDevice,
+                               s->stream) != hipSuccess)
                 st = APBF_ERR_HIP;
-            if (st != APBF_OK) break;
-            if (ms < best) {
-                best = ms;
-                bw = w;
-                bs = v;
+        };
+        cp(s->err_stats, stats, err_bytes);
+        cp(s->iter_err, stats + err_bytes, iter_bytes);
+        cp(s->particle_substeps, stats + err_bytes + iter_bytes, ps_bytes);
+    };
+    const int timed_kernel = s->timed_kernel, walk0 = s->walk_lanes, solve0 = s->solve_lanes;
+    s->timed_kernel = -1;
+    if (st == APBF_OK) copy_stats(true);
+    const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
+    float trial_ms[3][3];
+    for (int r = 0; r < 2 && st == APBF_OK; ++r)
+        for (int a = 0; a < 3 && st == APBF_OK; ++a)
+            for (int b = 0; b < 3 && st == APBF_OK; ++b) {
+                s->walk_lanes = walks[a];
+                s->solve_lanes = solves[b];
+                st = apbf_snapshot_restore(snap, scene);
+                if (st != APBF_OK) break;
+                s->update(dt);
+                float ms = 0.0f;
+                if (hipEventRecord(e0, s->stream) != hipSuccess) st = APBF_ERR_HIP;
+                for (uint32_t k = 0; k < substeps; ++k) s->update(dt);
+                if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    st = APBF_ERR_HIP;
+                trial_ms[a][b] = r == 0 ? ms : fminf(trial_ms[a][b], ms);
             }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    s->walk_lanes = bw;
-    s->solve_lanes = bs;
+    float best = 1e30f;
+    int bw = walk0, bs = solve0;
+    if (st == APBF_OK)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (trial_ms[a][b] < best) {
+                    best = trial_ms[a][b];
+                    bw = walks[a];
+                    bs = solves[b];
+                }
+    // put the scene back even after a failed trial
+    const apbf_status back = apbf_snapshot_restore(snap, scene);
+    if (st == APBF_OK) st = back;
+    if (stats) {
+        copy_stats(false);
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(stats);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)apbf_snapshot_destroy(snap);
+    s->timed_kernel = timed_kernel;
+    s->walk_lanes = st == APBF_OK ? bw : walk0;
+    s->solve_lanes = st == APBF_OK ? bs : solve0;
     drop_graph(s);
+    if (st != APBF_OK) return st;
     if (best_walk) *best_walk = bw;
     if (best_solve) *best_solve = bs;
     if (best_ms) *best_ms = best / (float)substeps;
-    return st;
+    return APBF_OK;
 }
 
 }  // extern "C"

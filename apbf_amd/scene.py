@@ -167,8 +167,9 @@ class Scene:
         check(self._L.apbf_scene_set_launch_shape(self._h, int(walk_lanes), int(solve_lanes)), "set_launch_shape")
 
     def autotune(self, dt: float, substeps: int = 3):
-        """Fastest (walk lanes, solver lanes, ms per substep); the scene advances
-        by 9 * substeps."""
+        """Fastest (walk lanes, solver lanes, ms per substep), every shape timed
+        from the same saved state; the scene is put back afterwards (its
+        neighbour list stays empty until the next substep)."""
         w, s, ms = C.c_int32(), C.c_int32(), C.c_float()
         check(self._L.apbf_scene_autotune(self._h, float(dt), int(substeps), C.byref(w), C.byref(s), C.byref(ms)),
               "autotune")

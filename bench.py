@@ -306,7 +306,7 @@ def main():
         shape = f"walk {w}, solver {v} lanes"
     for _ in range(args.warmup):
         scene.update(dt)
-    if args.autotune:  # part of the warm-up: the tuner times real substeps of the scene
+    if args.autotune:  # before the timed region; the tuner times every shape from this state and puts it back
         w, v, ms = scene.autotune(dt, substeps=2)
         log(f"autotuned launch shape: walk {w} lanes, solver {v} lanes ({ms:.3f} ms/substep)")
         shape = f"walk {w}, solver {v} lanes (autotuned)"

@@ -333,8 +333,10 @@ apbf_status apbf_batch_replay(apbf_batch* batch, uint32_t times);
 apbf_status apbf_batch_synchronize(apbf_batch* batch);
 /* Launch shapes: lanes per particle in the neighbour walk (4, 8, 16) and per
  * row in the solver (8, 16, 32); every shape computes identical results.
- * apbf_scene_autotune times `substeps` substeps under each shape (advancing
- * the scene by 9 * substeps) and keeps the fastest. */
+ * apbf_scene_autotune times `substeps` substeps under each shape, every
+ * shape from the same saved state (twice, keeping the faster time), and keeps
+ * the fastest; the scene and its statistics are put back afterwards, except
+ * the neighbour list, which is empty until the next substep. */
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes);
 apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk_lanes,
                                 int32_t* best_solve_lanes, float* best_ms_per_substep);

@@ -83,14 +83,26 @@ def test_launch_shapes_match_the_oracle(device, walk, solve):
 
 
 def test_autotune_keeps_results(device):
+    """The autotuner times every shape from the same saved state and puts the
+    scene and its statistics back: afterwards the scene equals one that was
+    never tuned (the neighbour list aside, which the next substep rebuilds)."""
     p, st = cfg.small_water_drop(seed=5)
     dt = p["dt"]
     tuned, plain = _scene(p, st), _scene(p, st)
+    for s in (tuned, plain):
+        s.set_error_measurement(True)
+        for _ in range(3):
+            s.update(dt)
     w, s, ms = tuned.autotune(dt, substeps=2)
     assert w in (4, 8, 16) and s in (8, 16, 32) and ms > 0
-    for _ in range(18):  # the autotuner advanced the scene by 9 x 2 substeps
+    bad = compare(tuned.get_state(), plain.get_state(), check_pairs=False)
+    assert not bad, "after autotune: " + "; ".join(bad)
+    assert tuned.counts()[2] == 0
+    assert tuned.error_stats() == plain.error_stats()
+    assert tuned.particle_substeps() == plain.particle_substeps()
+    # per-iteration errors are double sums in atomic order: equal up to rounding
+    np.testing.assert_allclose(tuned.iteration_errors(), plain.iteration_errors(), rtol=1e-12, atol=0)
+    for k in range(2):
+        tuned.update(dt)
         plain.update(dt)
-    _same(tuned, plain, "after autotune")
-    tuned.update(dt)
-    plain.update(dt)
-    _same(tuned, plain, "tuned substep")
+        _same(tuned, plain, f"tuned substep {k}")
