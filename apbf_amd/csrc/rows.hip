@@ -25,26 +25,43 @@ namespace apbf {
 
 constexpr int ROW_LANES = 16;
 
+// Group reductions (every lane of the wave active). Inside a 16-lane row of
+// the wave they use DPP lane moves, which read a neighbour lane's register in
+// the same VALU instruction: swap of adjacent lanes, swap of adjacent lane
+// pairs, the mirror inside 8 lanes, the mirror inside 16 lanes. After each
+// step every lane holds the reduction over its 2, 4, 8, 16 lanes. Wider
+// groups continue with xor shuffles. Sums of integers, minima and maxima do
+// not depend on the order of the operands, so the results equal those of
+// any other reduction order.
+template <int L, typename Op>
+__device__ __forceinline__ uint32_t greduce(uint32_t v, Op op) {
+    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "group of a power of two lanes");
+    constexpr int QUAD_XOR1 = 0xB1, QUAD_XOR2 = 0x4E, ROW_HALF_MIRROR = 0x141, ROW_MIRROR = 0x140;
+    auto dpp = [](uint32_t x, auto ctrl) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, decltype(ctrl)::value, 0xF, 0xF, false);
+    };
+    if (L >= 2) v = op(v, dpp(v, std::integral_constant<int, QUAD_XOR1>{}));
+    if (L >= 4) v = op(v, dpp(v, std::integral_constant<int, QUAD_XOR2>{}));
+    if (L >= 8) v = op(v, dpp(v, std::integral_constant<int, ROW_HALF_MIRROR>{}));
+    if (L >= 16) v = op(v, dpp(v, std::integral_constant<int, ROW_MIRROR>{}));
+    if (L >= 32) v = op(v, (uint32_t)__shfl_xor((int)v, 16, 64));
+    if (L >= 64) v = op(v, (uint32_t)__shfl_xor((int)v, 32, 64));
+    return v;
+}
 template <int L>
 __device__ __forceinline__ uint32_t gsum(uint32_t v) {
-#pragma unroll
-    for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    return greduce<L>(v, [](uint32_t a, uint32_t b) { return a + b; });
 }
 template <int L>
 __device__ __forceinline__ uint32_t gmin(uint32_t v) {
-#pragma unroll
-    for (int m = L / 2; m > 0; m >>= 1) {
-        uint32_t o = __shfl_xor(v, m, 64);
-        v = o < v ? o : v;
-    }
-    return v;
+    return greduce<L>(v, [](uint32_t a, uint32_t b) { return b < a ? b : a; });
 }
 template <int L>
 __device__ __forceinline__ float gmaxf(float v) {
-#pragma unroll
-    for (int m = L / 2; m > 0; m >>= 1) v = fmax_g(v, __shfl_xor(v, m, 64));
-    return v;
+    return __builtin_bit_cast(float, greduce<L>(__builtin_bit_cast(uint32_t, v), [](uint32_t a, uint32_t b) {
+                                  return __builtin_bit_cast(uint32_t, fmax_g(__builtin_bit_cast(float, a),
+                                                                             __builtin_bit_cast(float, b)));
+                              }));
 }
 // Mask of this lane's group within the wave's 64-bit ballot.
 template <int L>
