@@ -296,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ slots, uint32_t pcap,
                                                      uint32_t* __restrict__ col) {
     if (counts[C_FALLBACK]) return;
-    constexpr int L = 16;
+    constexpr int L = 16, B = 8;  // a lane has B loads in flight before its stores
     const uint32_t n = counts[C_N];
     const uint32_t stride = slot_stride(pcap, n);
     const int gl = threadIdx.x & (L - 1);
@@ -306,7 +306,19 @@ __global__ __launch_bounds__(BLOCK) void k_pack_rows(const uint32_t* __restrict_
         const uint32_t b = row[i], len = row[i + 1] - b;
         if (len > stride) continue;
         const uint32_t* src = slots + (size_t)i * stride;
-        for (uint32_t k = gl; k < len; k += L) col[b + k] = src[k];
+        for (uint32_t k0 = 0; k0 < len; k0 += (uint32_t)(B * L)) {
+            uint32_t v[B];
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                const uint32_t k = k0 + (uint32_t)gl + (uint32_t)(t * L);
+                v[t] = k < len ? src[k] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                const uint32_t k = k0 + (uint32_t)gl + (uint32_t)(t * L);
+                if (k < len) col[b + k] = v[t];
+            }
+        }
     }
 }
 
