@@ -626,7 +626,10 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_rows(const int4* __restrict__ p
     uint32_t bmin = 0xFFFFFFFFu, best = 0xFFFFFFFFu, best_k = 0u;
     bool found = false;
     auto entry = [&](uint32_t k, const int4 pj, uint32_t obd) {
-        const uint32_t dist = to_u32(length(vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z))));
+        // squared distances of fixed-point differences: 0 or in [1, 2^64], where
+        // sqrt_in_range is the IEEE square root of length()
+        const v3f dv = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z));
+        const uint32_t dist = to_u32(sqrt_in_range(dot(dv, dv)));
         const uint32_t cand = obd + dist;
         bmin = cand < bmin ? cand : bmin;
         if (!found || dist <= best) {  // later entries of the lane win ties
@@ -716,7 +719,8 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_density_rows(
     auto entry = [&](uint32_t k, const int4 pj, uint32_t obd, float imN) {
         const int32_t dx = isub(pj.x, pi.x), dy = isub(pj.y, pi.y), dz = isub(pj.z, pi.z);
         // find_split_and_merge_1/2 (as k_fsm12_rows)
-        const uint32_t dist = to_u32(length(vec_i(dx, dy, dz)));
+        const v3f dv = vec_i(dx, dy, dz);
+        const uint32_t dist = to_u32(sqrt_in_range(dot(dv, dv)));  // as in k_fsm12_rows
         const uint32_t cand = obd + dist;
         bmin = cand < bmin ? cand : bmin;
         if (!found || dist <= best) {
