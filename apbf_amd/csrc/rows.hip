@@ -422,22 +422,56 @@ __global__ __launch_bounds__(BLOCK) void k_features_rows(const int4* __restrict_
     }
     uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0;
     float rsum = 0.0f, dsum = 0.0f;
-    for (uint32_t k = rb + gl; k < re; k += L) {
-        const uint32_t j = col[k];
-        const int4 pj = pos[j];
-        const float imN = inv_mass[j];
+    auto entry = [&](const int4 pj, const float imN, const float rj) {
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         v3f grad;
         kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
-        const v3f wg = grad / imN;
-        dens += to_u32(w / imN * INCOMP_RES);
+        const float gg = dot(grad, grad);
+        float wq, ggq;
+        v3f wg;
+        // the quotients by the neighbour's inverse mass, as in k_solve_rows
+        const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
+        const uint32_t ie = (ib >> 23) & 0xFFu;
+        if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u)) {
+            const float rim = __builtin_bit_cast(float, (ib & 0x80000000u) | ((254u - ie) << 23));
+            wq = w * rim;
+            wg = grad * rim;
+            ggq = gg * rim;
+        } else {
+            wq = w / imN;
+            wg = grad / imN;
+            ggq = gg / imN;
+        }
+        dens += to_u32(wq * INCOMP_RES);
         s0 += (uint32_t)to_i32(wg.x * INCOMP_RES);
         s1 += (uint32_t)to_i32(wg.y * INCOMP_RES);
         s2 += (uint32_t)to_i32(wg.z * INCOMP_RES);
-        sq += to_u32(dot(grad, grad) / imN * INCOMP_RES);
-        rsum += radius[j];
-        dsum += length(diff);
+        sq += to_u32(ggq * INCOMP_RES);
+        rsum += rj;
+        dsum += sqrt_in_range(dot(diff, diff));  // length(diff): 0 or in [2^-36, 2^34]
+    };
+    // batches of 4 entries per lane, loaded before the arithmetic; each lane
+    // still adds its entries in row order
+    uint32_t k = rb + gl;
+    for (; k + 3u * L < re; k += 4u * L) {
+        uint32_t jb[4];
+        int4 pb[4];
+        float mb[4], rbj[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) jb[t] = col[k + (uint32_t)t * L];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            pb[t] = pos[jb[t]];
+            mb[t] = inv_mass[jb[t]];
+            rbj[t] = radius[jb[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) entry(pb[t], mb[t], rbj[t]);
+    }
+    for (; k < re; k += L) {
+        const uint32_t j = col[k];
+        entry(pos[j], inv_mass[j], radius[j]);
     }
     dens = gsum<L>(dens) + self;
     sq = gsum<L>(sq);
