@@ -336,7 +336,7 @@ def spherical_pool_scene(center=(0.0, 10.0, -60.0), pool_radius: float = 100.0, 
 
 def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: float = 0.4675,
                kernel_width: float = 1.0, impulse_width: float = 20.0, impulse_speed: float = 4.0,
-               seed: int = 0, iterations: int = 3, res: int = 7, pairs_per_particle: int = 64):
+               seed: int = 0, iterations: int = 3, res: int = 7, pairs_per_particle: int = 128):
     """Large uniform pool, plain PBF: particles of radius kernel_width/4 on a
     lattice at `spacing` (jitter +-0.05h) filling the box floor to the depth
     that holds ~n_target particles (427 x 88 x 213 = 8 003 688 at the defaults,
@@ -345,7 +345,9 @@ def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: fl
     along +x. The velocity is inferred from the displacement over the last
     substep (velocity_handling.cpp:26), so it enters as pos_backup = pos - v dt
     with last_delta_time = dt. The hash domain is the cube [0, max(box)]^3 at
-    2^res cells per axis (cells 1.5625h at the defaults)."""
+    2^res cells per axis (cells 1.5625h at the defaults). 128 pairs per particle
+    of capacity keep almost every row inside its slot of the neighbour walk
+    (64: 23.03 against 22.82 ms per substep, same states)."""
     rng = np.random.default_rng(seed)
     r = kernel_width / KERNEL_SCALE
     nx = int(box[0] / spacing)
