@@ -113,14 +113,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         // Distance tests compare squared distances with exact bounds (the
         // reference compares rounded square roots; sqrt_le_bound makes the two
         // agree for every input). A NaN range keeps every candidate.
-        const float t_range = (range != range) ? __builtin_inff() : sqrt_le_bound(range);
+        //
+        // Both tests run on unscaled coordinates. The world distance is
+        // p - pos_world(pj) = (float(xi) - float(xj)) * 2^-18 component-wise,
+        // and the differences of integer-valued floats are 0 or at least 1 in
+        // magnitude, so every rounding of the difference and of its squared
+        // length commutes with the scaling by 2^-18 (no result is subnormal):
+        // the squared world distance is exactly dot(u, u) * 2^-36 with
+        // u = float(xi) - float(xj), and likewise for the fixed-point
+        // difference. Comparing dot(u, u) with the bound times 2^36 (an exact
+        // scaling; an overflow to infinity keeps the comparison's outcome, as
+        // dot(u, u) * 2^-36 < 2^30) is then the same test.
+        const float t_range = ((range != range) ? __builtin_inff() : sqrt_le_bound(range)) * 0x1p36f;
+        const v3f pr = vec_i(pi.x, pi.y, pi.z);
         float okw = 0.0f, t_cut = 0.0f, t_okw = 0.0f;
         uint32_t umax = 0;
         if (SPREAD) {
             const float tr = wa.on_target ? trad[i] : 0.0f;
             okw = fmax_g(radius[i], tr) * KERNEL_SCALE;
-            t_cut = sqrt_le_bound(fmax_g(okw, kwi));
-            t_okw = sqrt_le_bound(okw);
+            t_cut = sqrt_le_bound(fmax_g(okw, kwi)) * 0x1p36f;
+            t_okw = sqrt_le_bound(okw) * 0x1p36f;
             umax = to_u32(okw * KW_RES);
         }
         // kernel_width.comp:41-53 for an entry at squared distance d2: the
@@ -130,14 +142,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         // so where it already reaches umax (the common case of neighbours no
         // smaller than the particle) the entry changes nothing and the
         // influence is not evaluated.
-        auto spread_to = [&](uint32_t j, float d2) {
+        auto spread_to = [&](uint32_t j, float d2r) {  // d2r: squared distance * 2^36
             const uint32_t cur = ukw[j];
             if (umax <= cur) return;
             uint32_t uval;
-            if (d2 <= t_okw) {  // distance <= okw: influence exactly 1
+            if (d2r <= t_okw) {  // distance <= okw: influence exactly 1
                 uval = umax;
             } else {
-                const float dfk = __builtin_sqrtf(d2) - okw;
+                const float dfk = __builtin_sqrtf(d2r * 0x1p-36f) - okw;
                 const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
                 uval = to_u32(okw * infl * KW_RES);
             }
@@ -216,11 +228,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                         float d2 = 0.0f;
                         if (t < total && j != i) {
                             const int4 pj = pos[j];
-                            const v3f d = p - pos_world(pj);
-                            green = !(dot(d, d) > t_range);
+                            const v3f u = pr - vec_i(pj.x, pj.y, pj.z);
+                            green = !(dot(u, u) > t_range);
                             if (SPREAD && green) {
-                                const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-                                d2 = dot(diff, diff);
+                                const v3f v = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z));
+                                d2 = dot(v, v);
                                 keep = d2 <= t_cut;
                             }
                         }
