@@ -7,9 +7,10 @@
 // host never synchronises; grids are sized for the capacity and tiles past the
 // length exit or contribute zeros.
 //
-// Scan: reduce-then-scan over tiles of 2048 values (256 threads x 8 values).
-// Wave-level inclusive scans use __shfl_up over the 64 lanes; the four waves of
-// a block combine through LDS.
+// Scan: reduce-then-scan over tiles of 2048 values (256 threads x 8 values),
+// two launches: the tile sums, then per tile the sum of the tile sums before
+// it plus the scan of its own values. Wave-level inclusive scans use
+// __shfl_up over the 64 lanes; the four waves of a block combine through LDS.
 //
 // Radix sort: LSD on 8-bit digits (half the passes of the reference's 4-bit
 // digits). Per pass: (1) per-tile digit histograms written bucket-major
@@ -65,33 +66,30 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t* __restric
     if (threadIdx.x == 0) tile_sums[tile] = (base < n) ? total : 0u;
 }
 
-// Exclusive scan of the tile sums by one block, looping over chunks.
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t* __restrict__ tile_sums,
-                                                      uint32_t n_tiles,
-                                                      uint32_t* __restrict__ total_out) {
-    __shared__ uint32_t lds[4];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n_tiles; base += BLOCK) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < n_tiles ? tile_sums[i] : 0u;
-        uint32_t total;
-        uint32_t ex = block_excl_scan(v, lds, total);
-        if (i < n_tiles) tile_sums[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && total_out) *total_out = carry;
-}
-
+// Second pass: every tile adds the sum of the tile sums before it (at most a
+// few thousand values, read by the block from L2) to the scan of its own
+// values; block 0 also writes the grand total. Two launches per scan.
 __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t* __restrict__ in,
                                                       uint32_t* __restrict__ out,
                                                       const uint32_t* __restrict__ d_len, uint32_t n_host,
-                                                      const uint32_t* __restrict__ tile_offsets,
-                                                      int inclusive) {
+                                                      const uint32_t* __restrict__ tile_sums, uint32_t n_tiles,
+                                                      int inclusive, uint32_t* __restrict__ total_out) {
     __shared__ uint32_t lds[4];
     const uint32_t n = d_len ? *d_len : n_host;
     const uint32_t tile = blockIdx.x;
     const uint64_t base = (uint64_t)tile * SCAN_TILE;
+    if (tile == 0 && total_out) {  // uniform in the block
+        uint32_t acc = 0;
+        for (uint32_t k = threadIdx.x; k < n_tiles; k += BLOCK) acc += tile_sums[k];
+        uint32_t all;
+        block_excl_scan(acc, lds, all);
+        if (threadIdx.x == 0) *total_out = all;
+    }
     if (base >= n) return;
+    uint32_t acc = 0;
+    for (uint32_t k = threadIdx.x; k < tile; k += BLOCK) acc += tile_sums[k];
+    uint32_t before;
+    block_excl_scan(acc, lds, before);
     uint64_t i0 = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t sum = 0;
@@ -101,7 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const uint32_t* __restrict
         sum += v[k];
     }
     uint32_t total;
-    uint32_t run = block_excl_scan(sum, lds, total) + tile_offsets[tile];
+    uint32_t run = block_excl_scan(sum, lds, total) + before;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         uint32_t x = v[k];
@@ -119,9 +117,8 @@ static void scan_impl(const uint32_t* in, uint32_t* out, const uint32_t* d_len, 
     uint32_t tiles = div_up(capacity ? capacity : 1, SCAN_TILE);
     uint32_t* tile_sums = static_cast<uint32_t*>(ws);
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(BLOCK), 0, s, in, d_len, n_host, tile_sums);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(BLOCK), 0, s, tile_sums, tiles, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(BLOCK), 0, s, in, out, d_len, n_host,
-                       tile_sums, inclusive ? 1 : 0);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(BLOCK), 0, s, in, out, d_len, n_host, tile_sums, tiles,
+                       inclusive ? 1 : 0, total);
 }
 
 void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* d_len, uint32_t capacity,
