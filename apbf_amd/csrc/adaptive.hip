@@ -590,8 +590,11 @@ void Scene::update_transfers(bool with_density_error) {
     LAUNCH(k_copy_u32_n, stride_grid(cap), old_bd, boundary_distance[cur], counts);
     bool tk = tbegin(K_FSM12);
     if (with_density_error) {
-        launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                             P.s.mGradientKernelId, gc, s);
+        // the solver loop of this substep computed gc from the same widths,
+        // radii and masses (it moves positions only)
+        if (P.solver_iterations <= 0)
+            launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                                 P.s.mGradientKernelId, gc, s);
         launch_fsm12_density_rows(pos[cur], inv_mass[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
                                   old_bd, boundary_distance[cur], nearest, P.s.mHeightKernelId,
                                   P.s.mGradientKernelId, part_sum, part_max, s);

@@ -376,10 +376,11 @@ void Scene::update(float dt) {
     }
 }
 
-void Scene::accumulate_density_error() {
+void Scene::accumulate_density_error() {  // after the solver loop of update()
     hipStream_t s = stream;
-    launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                         P.s.mGradientKernelId, gc, s);
+    if (P.solver_iterations <= 0)  // otherwise the solver loop computed gc from the same state
+        launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
+                             P.s.mGradientKernelId, gc, s);
     launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
                         P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
     launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
