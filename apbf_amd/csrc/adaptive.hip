@@ -290,18 +290,25 @@ __global__ __launch_bounds__(BLOCK) void k_claim_owner(const uint32_t* __restric
         if (ptype[i] == P_MERGE) atomicMin(&owner[pt[i]], i);
     }
 }
+// owner_next (optional): the other owner buffer, reset here for the next
+// round as k_claim_reset would (no kernel of this round reads it).
 __global__ __launch_bounds__(BLOCK) void k_claim_decide(const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ ps,
                                                         const uint32_t* __restrict__ pt,
                                                         const uint32_t* __restrict__ ptype,
                                                         uint32_t* __restrict__ state,
-                                                        const uint32_t* __restrict__ owner) {
+                                                        const uint32_t* __restrict__ owner,
+                                                        uint32_t* __restrict__ owner_next) {
     const uint32_t np = counts[C_PROPOSALS];
     GRID_STRIDE(k, np) {
         uint32_t i = list[k];
-        if (state[i] != CL_UNDECIDED) continue;
         bool merge = ptype[i] == P_MERGE;
+        if (owner_next) {
+            owner_next[ps[i]] = 0xFFFFFFFFu;
+            if (merge) owner_next[pt[i]] = 0xFFFFFFFFu;
+        }
+        if (state[i] != CL_UNDECIDED) continue;
         uint32_t os = owner[ps[i]];
         uint32_t ot = merge ? owner[pt[i]] : i;
         if (os == i && ot == i) {
@@ -619,10 +626,14 @@ void Scene::update_transfers(bool with_density_error) {
     scan_u32(a_flag, a_scan, counts + C_N, cap, false, counts + C_PROPOSALS, ws_mem, s);
     LAUNCH(k_scatter_list, stride_grid(cap), a_flag, a_scan, counts, a_list);
     LAUNCH(k_copy_u32_n, stride_grid(cap), a_t0, transferring[cur], counts);
+    // owner buffers alternate between rounds, so each round's decide pass
+    // resets the next round's buffer
+    uint32_t* own[2] = {a_owner, a_owner2};
+    LAUNCH(k_claim_reset, stride_grid(cap), a_list, counts, a_s, a_t, a_type, own[0]);
     for (int r = 0; r < claim_rounds; ++r) {
-        LAUNCH(k_claim_reset, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_owner);
-        LAUNCH(k_claim_owner, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_state, a_owner);
-        LAUNCH(k_claim_decide, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_state, a_owner);
+        LAUNCH(k_claim_owner, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_state, own[r & 1]);
+        LAUNCH(k_claim_decide, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_state, own[r & 1],
+               r + 1 < claim_rounds ? own[(r + 1) & 1] : nullptr);
     }
     LAUNCH1(k_zero_claim_counts, counts);
     LAUNCH(k_claim_apply, stride_grid(cap), a_list, counts, a_s, a_t, a_type, a_state, transferring[cur]);

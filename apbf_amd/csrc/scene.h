@@ -90,6 +90,7 @@ struct Scene {
     uint32_t* small = nullptr;
     // split/merge claim resolution (per particle) and transfer compaction scratch
     uint32_t *a_type = nullptr, *a_s = nullptr, *a_t = nullptr, *a_state = nullptr, *a_owner = nullptr;
+    uint32_t* a_owner2 = nullptr;  // second owner buffer of the claim rounds
     uint32_t *a_flag = nullptr, *a_scan = nullptr, *a_list = nullptr, *a_t0 = nullptr;
     uint32_t *t_flag = nullptr, *t_scan = nullptr;
     int claim_rounds = 6;

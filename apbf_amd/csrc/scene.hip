@@ -121,6 +121,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(a_t, cap + 1);
     A(a_state, cap + 1);
     A(a_owner, cap + 1);
+    A(a_owner2, cap + 1);
     A(a_flag, cap + 1);
     A(a_scan, cap + 1);
     A(a_list, cap + 1);
@@ -158,8 +159,9 @@ Scene::~Scene() {
         hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]); hipFree(lam_prev[s]);
         hipFree(t_source[s]); hipFree(t_target[s]); hipFree(t_time_left[s]);
     }
-    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
-                  part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_flag, a_scan,
+    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift,
+                  ukw, u0, u1, u2, u3, u4, small, part_sum,
+                  part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_owner2, a_flag, a_scan,
                   a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps};
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
