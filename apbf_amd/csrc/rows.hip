@@ -97,7 +97,7 @@ __device__ __forceinline__ void kernel_eval_k(int hid, int gid, v3f r, float kw,
 constexpr int SOLVE_CACHE = 6;
 
 // The position shifts of k_solve_rows are summed per block in a table in LDS
-// before they reach the per-particle accumulator. A block takes AGG_STEPS
+// before they reach the per-particle accumulator. A block takes 2 or 4
 // groups of consecutive rows, and consecutive particles are neighbours (the
 // particles are sorted by cell), so the rows share most neighbours and one
 // global atomic per neighbour and component replaces one per pair; where
@@ -108,7 +108,12 @@ constexpr int SOLVE_CACHE = 6;
 // slots; an id that finds no free slot goes straight to the accumulator.
 constexpr int AGG_SLOTS = 1024;
 constexpr int AGG_PROBES = 4;
-constexpr uint32_t AGG_STEPS = 4;  // row groups per table flush (64 rows at 16 lanes per row)
+constexpr uint32_t AGG_STEPS = 4;  // row groups per table (64 rows at 16 lanes per row); long rows: 2
+
+// Gradients of a lane's row entries after the SOLVE_CACHE in registers: the
+// next SOLVE_SPILL go to LDS (one column per thread, so no thread reads
+// another's), the rest are recomputed in the shift loop.
+constexpr int SOLVE_SPILL = 3;
 constexpr uint32_t AGG_EMPTY = 0xFFFFFFFFu;
 
 struct ShiftTable {
@@ -160,8 +165,13 @@ struct ShiftTable {
     }
 };
 
-template <int L, int KH, int KG, int C>
-__global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ pos,
+// STEPS: row groups a block takes at a time and sums in its table before a
+// flush. Long rows bring more distinct neighbours per row group, and a table
+// flushed every 32 rows overflows less (water drop, 100-160 entries per row:
+// 0.75 against 0.80 ms per launch); short rows gain from fewer flushes
+// (large pool, 33 entries per row: 4.04 against 4.31 ms).
+template <int L, int KH, int KG, int C, uint32_t STEPS>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_solve_rows(const int4* __restrict__ pos,
                                                       const float* __restrict__ inv_mass,
                                                       const float* __restrict__ radius,
                                                       const float* __restrict__ kw,
@@ -174,19 +184,20 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
     __shared__ double s_err;
     __shared__ uint32_t s_key[AGG_SLOTS];
     __shared__ int32_t s_sum[3 * AGG_SLOTS];
+    __shared__ float s_g[3][SOLVE_SPILL][BLOCK];
     ShiftTable tab{s_key, s_sum, acc};
     if (threadIdx.x == 0) s_err = 0.0;
     tab.clear();
     __syncthreads();
-    // A block takes AGG_STEPS consecutive groups of BLOCK / L rows, then
+    // A block takes STEPS consecutive groups of BLOCK / L rows, then
     // flushes the table; the trip counts are uniform per block, so every
     // thread meets the barriers.
     const uint32_t n = counts[C_N];
     const int gl = threadIdx.x & (L - 1);
     constexpr uint32_t groups = BLOCK / L;
-    constexpr uint32_t span = groups * AGG_STEPS;
+    constexpr uint32_t span = groups * STEPS;
     for (uint32_t base = blockIdx.x * span; base < n; base += gridDim.x * span) {
-        for (uint32_t step = 0; step < AGG_STEPS; ++step) {
+        for (uint32_t step = 0; step < STEPS; ++step) {
             const uint32_t i = base + step * groups + threadIdx.x / L;
             const bool active = i < n;
             int4 pi = make_int4(0, 0, 0, 0);
@@ -277,10 +288,18 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                         if (rb + gl + (uint32_t)(t0 + t) * L < re) pair1(bp[t], bm[t], cg[t0 + t]);
                 }
             }
-            for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
-                const uint32_t j = col[k];
-                v3f gr;
-                pair1(pos[j], inv_mass[j], gr);
+            {
+                uint32_t idx = 0;
+                for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L, ++idx) {
+                    const uint32_t j = col[k];
+                    v3f gr;
+                    pair1(pos[j], inv_mass[j], gr);
+                    if (idx < (uint32_t)SOLVE_SPILL) {
+                        s_g[0][idx][threadIdx.x] = gr.x;
+                        s_g[1][idx][threadIdx.x] = gr.y;
+                        s_g[2][idx][threadIdx.x] = gr.z;
+                    }
+                }
             }
             dens = gsum<L>(dens);
             sq = gsum<L>(sq);
@@ -359,13 +378,18 @@ __global__ __launch_bounds__(BLOCK) void k_solve_rows(const int4* __restrict__ p
                     const uint32_t k = rb + gl + (uint32_t)t * L;
                     if (k < re) pair3(cj[t], cg[t]);
                 }
-                for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L) {
+                uint32_t idx = 0;
+                for (uint32_t k = rb + gl + (uint32_t)C * L; k < re; k += L, ++idx) {
                     const uint32_t j = col[k];
-                    const int4 pj = pos[j];
-                    const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
-                    float w;
                     v3f grad;
-                    kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+                    if (idx < (uint32_t)SOLVE_SPILL) {
+                        grad = mk(s_g[0][idx][threadIdx.x], s_g[1][idx][threadIdx.x], s_g[2][idx][threadIdx.x]);
+                    } else {
+                        const int4 pj = pos[j];
+                        const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
+                        float w;
+                        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+                    }
                     pair3(j, grad);
                 }
             }
@@ -910,20 +934,22 @@ static uint32_t row_grid(uint32_t cap, uint32_t lanes = ROW_LANES) {
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
-                       float* lambda, int4* acc, hipStream_t s, int lanes) {
+                       float* lambda, int4* acc, hipStream_t s, int lanes, bool long_rows) {
     int* ac = reinterpret_cast<int*>(acc);
+#define SOLVE(L, KH, KG, C, F)                                                                                       \
+    hipLaunchKernelGGL((k_solve_rows<L, KH, KG, C, F>), dim3(row_grid(cap, L)), dim3(BLOCK), 0, s, pos, inv_mass,    \
+                       radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac)
     if (a.hid == 1 && a.gid == 1 && lanes == 8)  // Gaussian height and gradient, 8 lanes per row
-        hipLaunchKernelGGL((k_solve_rows<8, 1, 1, 2 * SOLVE_CACHE>), dim3(row_grid(cap, 8)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
-    else if (a.hid == 1 && a.gid == 1 && lanes == 32)  // 32 lanes: rows of up to 256 entries stay cached
-        hipLaunchKernelGGL((k_solve_rows<32, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap, 32)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
-    else if (a.hid == 1 && a.gid == 1)  // Gaussian height and gradient (the reference's defaults)
-        hipLaunchKernelGGL((k_solve_rows<ROW_LANES, 1, 1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
+        SOLVE(8, 1, 1, SOLVE_CACHE, AGG_STEPS);
+    else if (a.hid == 1 && a.gid == 1 && lanes == 32)  // 32 lanes: rows of up to 192 entries stay in registers
+        SOLVE(32, 1, 1, SOLVE_CACHE, AGG_STEPS);
+    else if (a.hid == 1 && a.gid == 1 && long_rows)  // the reference's defaults (Gaussian), long rows
+        SOLVE(ROW_LANES, 1, 1, SOLVE_CACHE, AGG_STEPS / 2);
+    else if (a.hid == 1 && a.gid == 1)
+        SOLVE(ROW_LANES, 1, 1, SOLVE_CACHE, AGG_STEPS);
     else
-        hipLaunchKernelGGL((k_solve_rows<ROW_LANES, -1, -1, SOLVE_CACHE>), dim3(row_grid(cap)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac);
+        SOLVE(ROW_LANES, -1, -1, SOLVE_CACHE, AGG_STEPS);
+#undef SOLVE
 }
 void launch_features_rows(const int4* pos, const float4* vel, const float* inv_mass, const float* radius,
                           const float* kw, const float4* gc, const float* boundariness, const float* lam_prev,

@@ -75,6 +75,11 @@ struct Scene {
     float* t_time_left[2] = {};
 
     uint32_t* counts = nullptr;  // CountSlot block
+    // Pinned host copy of the count block, refreshed without synchronisation
+    // after every substep (so it lags by a substep or two). It only steers
+    // the choice between launch variants that compute the same results.
+    uint32_t* h_counts = nullptr;
+    bool long_rows = false;  // mean neighbour row of a recent substep above 62 entries
     // neighbourhood
     uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *perm = nullptr;
     uint32_t *cell_start = nullptr, *cell_end = nullptr;

@@ -147,6 +147,8 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     HIP_OK(hipMemset(err_stats, 0, 6 * sizeof(double)));
     HIP_OK(hipMemset(iter_err, 0, MAX_ITER_RECORD * sizeof(double)));
     HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_counts), C_COUNT * 4, hipHostMallocDefault));
+    std::memset(h_counts, 0, C_COUNT * 4);
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
 }
@@ -165,6 +167,7 @@ Scene::~Scene() {
                   a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps};
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
+    if (h_counts) (void)hipHostFree(h_counts);
 }
 
 // ------------------------------------------------------ velocity_handling
@@ -300,6 +303,10 @@ SolverArgs Scene::solver_args() const {
 }
 
 void Scene::prepare_solver() {
+    {
+        const volatile uint32_t* hc = h_counts;  // written by an earlier substep's copy
+        long_rows = (uint64_t)hc[C_PAIRS] > 62ull * (uint64_t)hc[C_N];
+    }
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
                          P.s.mGradientKernelId, gc, stream);
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
@@ -314,7 +321,7 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const 
     if (lam_override) a.lam_acc = lam_acc;
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
-                      boundariness[cur], lambda, shift, s, solve_lanes);
+                      boundariness[cur], lambda, shift, s, solve_lanes, long_rows);
     tend(tk);
     tk = tbegin(K_SOLVE_APPLY);
     launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s);
@@ -370,6 +377,11 @@ void Scene::update(float dt) {
     }
     solver_loop();
     launch_count_particle_substeps(counts, particle_substeps, stream);
+    // the count block to the host for the launch choices of later substeps;
+    // not recorded into captured graphs
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone)
+        (void)hipMemcpyAsync(h_counts, counts, C_COUNT * 4, hipMemcpyDeviceToHost, stream);
     if (measure_error && !P.basic_pbf) {
         update_transfers(true);  // the error pass rides on the split/merge find
     } else {
