@@ -175,6 +175,14 @@ apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t
     return APBF_OK;
 }
 
+apbf_status apbf_scene_set_long_rows(void* scene, int32_t mode) {
+    if (!scene || mode < -1 || mode > 1) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->long_rows_mode = mode;
+    drop_graph(s);  // a captured graph holds the old variant
+    return APBF_OK;
+}
+
 // Times `substeps` substeps under every launch shape (the scene advances by
 // 9 * substeps) and keeps the fastest. All shapes compute identical results.
 apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk, int32_t* best_solve,

@@ -80,6 +80,7 @@ struct Scene {
     // the choice between launch variants that compute the same results.
     uint32_t* h_counts = nullptr;
     bool long_rows = false;  // mean neighbour row of a recent substep above 62 entries
+    int long_rows_mode = -1;  // -1: long_rows from h_counts; 0 / 1: pinned
     // neighbourhood
     uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *perm = nullptr;
     uint32_t *cell_start = nullptr, *cell_end = nullptr;

@@ -305,7 +305,7 @@ SolverArgs Scene::solver_args() const {
 void Scene::prepare_solver() {
     {
         const volatile uint32_t* hc = h_counts;  // written by an earlier substep's copy
-        long_rows = (uint64_t)hc[C_PAIRS] > 62ull * (uint64_t)hc[C_N];
+        long_rows = long_rows_mode >= 0 ? long_rows_mode == 1 : (uint64_t)hc[C_PAIRS] > 62ull * (uint64_t)hc[C_N];
     }
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
                          P.s.mGradientKernelId, gc, stream);

@@ -166,6 +166,10 @@ class Scene:
     def set_launch_shape(self, walk_lanes: int, solve_lanes: int) -> None:
         check(self._L.apbf_scene_set_launch_shape(self._h, int(walk_lanes), int(solve_lanes)), "set_launch_shape")
 
+    def set_long_rows(self, mode: int) -> None:
+        """The solver's shift-table variant: -1 by row length (default), 0 short rows, 1 long rows."""
+        check(self._L.apbf_scene_set_long_rows(self._h, int(mode)), "set_long_rows")
+
     def autotune(self, dt: float, substeps: int = 3):
         """Fastest (walk lanes, solver lanes, ms per substep), every shape timed
         from the same saved state; the scene is put back afterwards (its

@@ -338,6 +338,12 @@ apbf_status apbf_batch_synchronize(apbf_batch* batch);
  * the fastest; the scene and its statistics are put back afterwards, except
  * the neighbour list, which is empty until the next substep. */
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes);
+/* The solver's row groups per shift-table flush in LDS: 2 for long rows, 4
+ * for short ones. mode -1 (the default) picks by the mean row length of a
+ * recent substep, read without synchronisation; 0 and 1 pin the short- and
+ * long-row variants. No counterpart in the reference; both variants compute
+ * identical results. */
+apbf_status apbf_scene_set_long_rows(void* scene, int32_t mode);
 apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk_lanes,
                                 int32_t* best_solve_lanes, float* best_ms_per_substep);
 

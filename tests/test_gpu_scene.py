@@ -318,3 +318,16 @@ def test_solver_long_rows_match_oracle(device):
         gs, _ = _check(g, r, f"substep {step}")
         if step == 0:
             assert len(gs["pairs"]) > 1500 * n, len(gs["pairs"]) / n
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_solver_table_variants_match_oracle(device, mode):
+    """The solver's two shift-table variants (flush every 64 or every 32 rows),
+    pinned instead of chosen by row length, give the oracle's substeps."""
+    p, st = cfg.small_water_drop(seed=6, active=True)
+    g, r = _pair(p, st)
+    g.set_long_rows(mode)
+    for step in range(6):
+        g.update(p["dt"])
+        r.update(p["dt"])
+    _check(g, r, f"variant {mode}")
