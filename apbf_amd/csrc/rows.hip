@@ -170,17 +170,14 @@ struct ShiftTable {
 // flushed every 32 rows overflows less (water drop, 100-160 entries per row:
 // 0.75 against 0.80 ms per launch); short rows gain from fewer flushes
 // (large pool, 33 entries per row: 4.04 against 4.31 ms).
+// Six waves per SIMD (80 VGPRs, 25.6 KB of LDS per block) hide the latency
+// of the row's dependent loads.
 template <int L, int KH, int KG, int C, uint32_t STEPS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_solve_rows(const int4* __restrict__ pos,
-                                                      const float* __restrict__ inv_mass,
-                                                      const float* __restrict__ radius,
-                                                      const float* __restrict__ kw,
-                                                      const float4* __restrict__ gc,
-                                                      const uint32_t* __restrict__ counts,
-                                                      const uint32_t* __restrict__ row,
-                                                      const uint32_t* __restrict__ col, uint32_t pcap,
-                                                      SolverArgs a, float* __restrict__ boundariness,
-                                                      float* __restrict__ lambda, int* __restrict__ acc) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_solve_rows(
+    const int4* __restrict__ pos, const float* __restrict__ inv_mass, const float* __restrict__ radius,
+    const float* __restrict__ kw, const float4* __restrict__ gc, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, uint32_t pcap, SolverArgs a,
+    float* __restrict__ boundariness, float* __restrict__ lambda, int* __restrict__ acc) {
     __shared__ double s_err;
     __shared__ uint32_t s_key[AGG_SLOTS];
     __shared__ int32_t s_sum[3 * AGG_SLOTS];

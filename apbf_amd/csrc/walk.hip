@@ -55,21 +55,16 @@ __device__ __forceinline__ uint64_t walk_group_mask() {
     return m << (lane & ~(L - 1));
 }
 
-// Eight waves per SIMD (64 VGPRs, a 16-byte spill) hide the latency of the
+// Eight waves per SIMD (64 VGPRs, a 28-byte spill) hide the latency of the
 // cell-table and position loads better than seven without the cap
 // (71 VGPRs): 8.04 against 8.24 ms per water-drop substep.
 template <int L, int MODE, bool SPREAD>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_walk_rows(const int4* __restrict__ pos,
-                                                     const float* __restrict__ kw,
-                                                     const float* __restrict__ radius,
-                                                     const float* __restrict__ trad,
-                                                     const uint32_t* __restrict__ cs,
-                                                     const uint32_t* __restrict__ ce, uint32_t* __restrict__ counts,
-                                                     WalkArgs wa, uint32_t* __restrict__ cnt_raw,
-                                                     uint32_t* __restrict__ cnt_f,
-                                                     const uint32_t* __restrict__ row_raw,
-                                                     const uint32_t* __restrict__ row_f, uint32_t* __restrict__ col,
-                                                     uint32_t* __restrict__ ukw) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_walk_rows(
+    const int4* __restrict__ pos, const float* __restrict__ kw, const float* __restrict__ radius,
+    const float* __restrict__ trad, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+    uint32_t* __restrict__ counts, WalkArgs wa, uint32_t* __restrict__ cnt_raw, uint32_t* __restrict__ cnt_f,
+    const uint32_t* __restrict__ row_raw, const uint32_t* __restrict__ row_f, uint32_t* __restrict__ col,
+    uint32_t* __restrict__ ukw) {
     constexpr bool FILL = MODE == WALK_FILL;
     constexpr bool PAD = MODE == WALK_PAD;
     // After a PAD walk the exact fill runs for every row when the list exceeds
