@@ -170,6 +170,7 @@ apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t
         return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     s->walk_lanes = walk_lanes;
+    s->walk_auto = false;
     s->solve_lanes = solve_lanes;
     drop_graph(s);  // a captured graph holds the old shapes
     return APBF_OK;
@@ -217,6 +218,8 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
         cp(s->particle_substeps, stats + err_bytes + iter_bytes, ps_bytes);
     };
     const int timed_kernel = s->timed_kernel, walk0 = s->walk_lanes, solve0 = s->solve_lanes;
+    const bool auto0 = s->walk_auto;
+    s->walk_auto = false;
     s->timed_kernel = -1;
     if (st == APBF_OK) copy_stats(true);
     const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
@@ -260,6 +263,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     (void)apbf_snapshot_destroy(snap);
     s->timed_kernel = timed_kernel;
     s->walk_lanes = st == APBF_OK ? bw : walk0;
+    s->walk_auto = st == APBF_OK ? false : auto0;
     s->solve_lanes = st == APBF_OK ? bs : solve0;
     drop_graph(s);
     if (st != APBF_OK) return st;

@@ -102,6 +102,7 @@ struct Scene {
     int claim_rounds = 6;
     // launch shapes picked by the autotuner (all compute identical results)
     int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8 or 16
+    bool walk_auto = true;  // walk lanes from a recent mean row length (4 below 62 entries, else 8)
     int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
     // captured substeps (two per graph: the buffer roles repeat after two)
     hipStream_t own_stream = nullptr;

@@ -299,7 +299,7 @@ def main():
 
     log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
     log(f"scene {params.get('workload')} with {len(state['radius'])} particles ready")
-    shape = "walk 8, solver 16 lanes (library default)"
+    shape = "walk 8 lanes (4 when rows average under 62 entries), solver 16 lanes (library default)"
     if args.launch_shape:
         w, v = (int(x) for x in args.launch_shape.split(","))
         scene.set_launch_shape(w, v)

@@ -333,6 +333,9 @@ apbf_status apbf_batch_replay(apbf_batch* batch, uint32_t times);
 apbf_status apbf_batch_synchronize(apbf_batch* batch);
 /* Launch shapes: lanes per particle in the neighbour walk (4, 8, 16) and per
  * row in the solver (8, 16, 32); every shape computes identical results.
+ * Until a shape is set or tuned, the walk takes 4 lanes when a recent
+ * substep's mean neighbour row is below 62 entries and 8 otherwise, and the
+ * solver 16.
  * apbf_scene_autotune times `substeps` substeps under each shape, every
  * shape from the same saved state (twice, keeping the faster time), and keeps
  * the fastest; the scene and its statistics are put back afterwards, except
