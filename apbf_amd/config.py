@@ -158,7 +158,7 @@ def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coar
     particles in a 300 x 80 x 300 h box). Split, merge and kernel-width spreading
     on (baseKernelWidthOnBoundaryDistance off, neighbour range scale 1.5),
     4 solver iterations. smallestTargetRadius = r_coarse (the reference scenes use
-    their base radius, pool.cpp:255 / waterdrop.cpp:54); target radius
+    their base radius, pool.cpp:52 / waterdrop.cpp:54); target radius
     r_coarse + s * boundary distance with targetRadiusOffset 0 and s chosen so it
     reaches 2 r_coarse at the pool's centre depth, so radii stay in [0.5h, 2h].
     Initial boundary distances are the geometric distances to the pool faces /

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void k_position_code(const int4* __restrict_
 }
 
 // Gather of every particle/fluid array by the sort permutation (the
-// hidden_list().apply_edit of neighborhood_green.cpp:78 plus the owner
+// hidden_list().apply_edit of neighborhood_green.cpp:54 plus the owner
 // rebuild of indexed_list.h:289-308, which with fluid id == hidden slot is the
 // same permutation for the fluid attributes).
 __global__ __launch_bounds__(BLOCK) void k_permute(const uint32_t* __restrict__ perm,

@@ -183,7 +183,7 @@ void Scene::velocity_handling(float dt) {
 }
 
 // ----------------------------------------------------- neighborhood_green
-// neighborhood_green.cpp:51-101: hash every particle, stable sort by hash,
+// neighborhood_green.cpp:27-77: hash every particle, stable sort by hash,
 // permute the particle SoA, cell tables, pair search.
 void Scene::neighborhood_green(float range_scale, bool spread) {
     hipStream_t s = stream;
@@ -204,7 +204,7 @@ void Scene::neighborhood_green(float range_scale, bool spread) {
 }
 
 // The pair build. With `spread` the walk also carries the kernel-width
-// spreading (kernel_width.cpp:22-71) and writes only its short list, so
+// spreading (spread_kernel_width.cpp:12-26, kernel_width.comp:32-60) and writes only its short list, so
 // neighborhood_green followed by spread_kernel_width costs two walks.
 void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;

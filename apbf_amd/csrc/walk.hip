@@ -23,7 +23,7 @@
 // value at least as large, which is exact because the stored values only grow
 // during the walk), and only the entries within the
 // particle's cutoff are written, which is the list the spreading pass of the
-// reference leaves behind (kernel_width.cpp:54-71). The full list is then
+// reference leaves behind (kernel_width.comp:54-60). The full list is then
 // never materialised: COUNT counts both lists, FILL writes the short one.
 #include "scene.h"
 

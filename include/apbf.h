@@ -127,7 +127,7 @@ apbf_status apbf_scene_get_lambda(void* scene, float* host_lambda);
 #define APBF_STAGE_VELOCITY 0          /* velocity_handling::apply (velocity_handling.cpp:15-31) */
 #define APBF_STAGE_PARTICLE_TRANSFER 1 /* particle_transfer::apply (particle_transfer.cpp:10-28) */
 #define APBF_STAGE_KERNEL_WIDTH_BD 2   /* uint_to_float_with_indexed_lower_bound (pool.cpp:77-80) */
-#define APBF_STAGE_NEIGHBORHOOD 3      /* neighborhood_green::apply (neighborhood_green.cpp:51-101) */
+#define APBF_STAGE_NEIGHBORHOOD 3      /* neighborhood_green::apply (neighborhood_green.cpp:27-77) */
 #define APBF_STAGE_SPREAD 4            /* spread_kernel_width::apply (spread_kernel_width.cpp:12-26) */
 #define APBF_STAGE_BOX 5               /* box_collision::apply (box_collision.cpp:12-24) */
 #define APBF_STAGE_INCOMPRESSIBILITY 6 /* incompressibility::apply (incompressibility.cpp:12-45) */

@@ -99,7 +99,7 @@ private:
     void* h_ = nullptr;
 };
 
-/* velocity_handling.h:9-20 */
+/* velocity_handling.h:7-18 */
 class velocity_handling {
 public:
     velocity_handling& set_data(scene* s) {

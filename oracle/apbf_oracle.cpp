@@ -9,6 +9,11 @@
 // here is stated at the use site.
 //
 // Build: g++ -O2 -ffp-contract=off -fPIC -shared (see oracle/Makefile).
+//
+// ORACLE_MUTATION (default 0) selects one deliberately wrong variant of a
+// single expression, numbered at its use site. Only tests/test_oracle_pins.py
+// builds such variants (into a temporary directory), to show that each pin of
+// the oracle fails on a plausible mistake. liboracle.so is always built with 0.
 #include "apbf_oracle.h"
 #include "oracle_math.h"
 
@@ -18,6 +23,11 @@
 #include <vector>
 
 using namespace om;
+
+#ifndef ORACLE_MUTATION
+#define ORACLE_MUTATION 0
+#endif
+#define MUT(k) (ORACLE_MUTATION == (k))
 
 namespace {
 
@@ -296,13 +306,17 @@ struct Scene {
     // temporaries kept for inspection
     std::vector<float> lambda;
     std::vector<uint32_t> split_list;
+    // intermediates of the last update_transfers: boundary distance after the
+    // flood fill (find_split_and_merge_1), shortest neighbour distance and the
+    // nearest neighbour (find_split_and_merge_2), per fluid id
+    std::vector<uint32_t> fsm_bd_fill, fsm_min_dist, fsm_nearest;
     float last_dt = 1.0f;  // velocity_handling.h:17
 
     KernelEval kern() const { return KernelEval{P.s.mHeightKernelId, P.s.mGradientKernelId}; }
     uint32_t H() const { return (uint32_t)position.size(); }
     uint32_t F() const { return (uint32_t)fidx.size(); }
 
-    // A hidden-particle edit (hidden_data::apply_edit, indexed_list.h:327-331):
+    // A hidden-particle edit (hidden_data::apply_edit, indexed_list.h:60-64 and :289-308):
     // gather the hidden SoA, rebuild every owner index list. Owners in
     // registration order: transfer source list, transfer target list, fluid
     // particle list (pool.cpp:6-24 / waterdrop.cpp:6-22 with the drop released,
@@ -419,6 +433,7 @@ struct Scene {
         float dir = sign(newv - oldv);
         float result = oldv * (1.0f + step * dir);
         bool reached = (oldv < newv) != (result < newv);
+        if (MUT(7)) reached = false;  // mutation 7: no snap at the crossing
         return reached ? newv : result;
     }
     void kernel_width_from_boundary_distance() {  // pool.cpp:77-80
@@ -426,7 +441,7 @@ struct Scene {
         for (uint32_t id = 0; id < F(); ++id) {
             uint32_t idx = fidx[id];
             float value = (float)boundary_distance[id] * factor;
-            float lower = radius[idx] * KERNEL_SCALE;
+            float lower = MUT(8) ? radius[idx] : radius[idx] * KERNEL_SCALE;  // mutation 8: bound without KERNEL_SCALE
             value = move_towards_mult(kernel_width[id], value, P.s.mKernelWidthAdaptionSpeed);
             kernel_width[id] = gmax(value, lower);
         }
@@ -437,7 +452,7 @@ struct Scene {
         return Grid{v3(P.hash_min[0], P.hash_min[1], P.hash_min[2]),
                     v3(P.hash_max[0], P.hash_max[1], P.hash_max[2]), P.hash_res_log2};
     }
-    void neighborhood(float range_scale) {  // neighborhood_green.cpp:51-101
+    void neighborhood(float range_scale) {  // neighborhood_green.cpp:27-77
         Grid g = grid();
         uint32_t maxHash = 1u << (g.res * 3u);
         uint32_t nh = H();
@@ -642,7 +657,7 @@ struct Scene {
         }
     }
 
-    // neighborhood_green.comp:49-87 with neighbor_add.glsl:230-244 (unsorted list)
+    // neighborhood_green.comp:49-87 with neighbor_add.glsl:11-25 (unsorted list)
     void green(const Grid& g, const uint32_t* cs, const uint32_t* ce, float range_scale) {
         uint32_t cap = P.cap_pairs;
         uint32_t len = 0;
@@ -675,7 +690,7 @@ struct Scene {
                 uint32_t idxN = fidx[idN];
                 V3 posN = pos_world(position[idxN]);
                 if (id == idN || length(sub(pos, posN)) > range) continue;
-                uint32_t i = len++;  // neighbor_add.glsl:231-243
+                uint32_t i = len++;  // neighbor_add.glsl:12-23
                 if (i < cap) pairs.push_back(U2{id, idN});
                 else len = cap;
             }
@@ -919,18 +934,21 @@ struct Scene {
         for (uint32_t id = 0; id < (uint32_t)pairs.size(); ++id) {  // find_split_and_merge_1.comp:17-32
             U2 n = pairs[id];
             uint32_t idx = fidx[n.a], idxN = fidx[n.b];
-            uint32_t bdN = oldBd[n.b];
+            uint32_t bdN = MUT(1) ? oldBd[n.a] : oldBd[n.b];  // mutation 1: own old distance
             uint32_t dist = to_u32(length(idiff(position[idxN], position[idx])));
             uint32_t cand = bdN + dist;
             if (cand < boundary_distance[n.a]) boundary_distance[n.a] = cand;
-            if (dist < minD[n.a]) minD[n.a] = dist;
+            if (MUT(2) ? (cand < minD[n.a]) : (dist < minD[n.a])) minD[n.a] = MUT(2) ? cand : dist;  // mutation 2
         }
+        fsm_bd_fill = boundary_distance;
+        fsm_min_dist = minD;
         for (uint32_t id = 0; id < (uint32_t)pairs.size(); ++id) {  // find_split_and_merge_2.comp:17-33
             U2 n = pairs[id];
             uint32_t idx = fidx[n.a], idxN = fidx[n.b];
             uint32_t dist = to_u32(length(idiff(position[idxN], position[idx])));
-            if (dist == minD[n.a]) nearest[n.a] = n.b;
+            if (dist == minD[MUT(3) ? n.b : n.a]) nearest[n.a] = n.b;  // mutation 3: neighbour's minimum
         }
+        fsm_nearest = nearest;
         uint32_t tlen = (uint32_t)t_source.size();
         uint32_t maxT = P.cap_transfers, maxS = P.cap_particles;
         split_list.clear();
@@ -955,17 +973,19 @@ struct Scene {
                     tr = gmax(tr, P.s.mSmallestTargetRadius);
                 } else {
                     tr = P.s.mSmallestTargetRadius +
-                         gmax(0.0f, (bdist - P.s.mTargetRadiusOffset) * P.s.mTargetRadiusScaleFactor);
+                         (MUT(4) ? (bdist - P.s.mTargetRadiusOffset) * P.s.mTargetRadiusScaleFactor  // mutation 4: no clamp
+                                 : gmax(0.0f, (bdist - P.s.mTargetRadiusOffset) * P.s.mTargetRadiusScaleFactor));
                 }
                 target_radius[id] = tr;
             } else {
                 tr = target_radius[id];
             }
             b = b >= 1.0f ? 1.0f : 0.0f;
-            boundary_distance[id] = to_u32((bdist * (1.0f - b) + r * b) * POS_RES);  // mix()
+            boundary_distance[id] = MUT(5) ? to_u32((bdist * b + r * (1.0f - b)) * POS_RES)  // mutation 5: mix reversed
+                                           : to_u32((bdist * (1.0f - b) + r * b) * POS_RES);  // mix()
             boundariness[id] = b;
             bool split = P.s.mSplit && (tr * split_factor <= r);
-            bool merge = P.s.mMerge && nnDist < r &&
+            bool merge = P.s.mMerge && (MUT(6) || nnDist < r) &&  // mutation 6: no distance condition
                          pow_f(r, DIM) + pow_f(nnR, DIM) <= pow_f(tr, DIM);
             if (!merge && !split) continue;
             uint32_t s = (merge && nnLarger) ? nnIdx : idx;
@@ -976,7 +996,7 @@ struct Scene {
             if (merge) {
                 uint32_t oldT = transferring[t];
                 transferring[t] = 1u;
-                if (oldT == 1u) { transferring[s] = 0u; continue; }
+                if (oldT == 1u && !MUT(9)) { transferring[s] = 0u; continue; }  // mutation 9: target not claimed
                 uint32_t tIdx = tlen++;
                 if (tIdx >= maxT) {
                     tlen = maxT;
@@ -1155,6 +1175,15 @@ void orc_get_lambda(void* h, float* lambda) {
     Scene* s = static_cast<Scene*>(h);
     std::memcpy(lambda, s->lambda.data(), 4 * s->lambda.size());
 }
+uint32_t orc_get_fsm(void* h, uint32_t* bd_fill, uint32_t* min_dist, uint32_t* nearest) {
+    Scene* s = static_cast<Scene*>(h);
+    const size_t n = s->fsm_nearest.size();
+    if (bd_fill) std::memcpy(bd_fill, s->fsm_bd_fill.data(), 4 * n);
+    if (min_dist) std::memcpy(min_dist, s->fsm_min_dist.data(), 4 * n);
+    if (nearest) std::memcpy(nearest, s->fsm_nearest.data(), 4 * n);
+    return (uint32_t)n;
+}
+
 uint32_t orc_get_splits(void* h, uint32_t* split_list) {
     Scene* s = static_cast<Scene*>(h);
     if (split_list) std::memcpy(split_list, s->split_list.data(), 4 * s->split_list.size());

@@ -15,10 +15,13 @@
  * Pinning: list algorithms against the reference's test.cpp vectors; hash,
  * codes, ranges and neighbour sets against closed forms and brute force;
  * exp/pow against libm; the substep stages by invariants and closed forms
- * (tests/test_oracle_invariants.py). Parity unpinned: the flood-fill distance
- * and nearest-neighbour choice inside find_split_and_merge_1/2 (checked only
- * through the legality of the resulting claims), and everything that depends
- * on GLSL built-in rounding of the real driver.
+ * (tests/test_oracle_invariants.py); the split/merge find (flood fill,
+ * nearest neighbour, target radius, decay, predicates, maximal claims) and
+ * the kernel width from boundary distance by brute force over the pair list
+ * and per-particle closed forms, each shown to fail on a planted mistake
+ * (tests/test_oracle_pins.py). Not pinnable without a reference run: the
+ * rounding of GLSL built-ins in the real driver and which contended
+ * candidate its scheduler lets win.
  */
 #ifndef APBF_ORACLE_H
 #define APBF_ORACLE_H
@@ -113,6 +116,11 @@ void orc_set_neighborhood_method(void* scene, int method);
 /* One sphere collider (sphere_collision::set_sphere); enable = 0 removes it. */
 void orc_set_sphere(void* scene, const float* center3, float radius, int hollow, int enable);
 void orc_stage(void* h, int stage, float dt);
+/* Intermediates of the last update_transfers stage, per fluid id: boundary
+ * distance after the flood fill (find_split_and_merge_1.comp:17-32), shortest
+ * neighbour distance and nearest neighbour (find_split_and_merge_2.comp:17-33).
+ * Any pointer may be NULL; returns the fluid count they cover. */
+uint32_t orc_get_fsm(void* h, uint32_t* bd_fill, uint32_t* min_dist, uint32_t* nearest);
 /* One full substep (pool::update / waterdrop::update with the time machine off). */
 void orc_update(void* h, float dt);
 /* Mean and maximum per-particle compression error max(0, rho/rho0 - 1) of the

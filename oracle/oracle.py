@@ -22,6 +22,19 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
 
 
+def build_variant(mutation: int, out_dir: str):
+    """Builds the oracle with the deliberately wrong variant `mutation` of one
+    expression (ORACLE_MUTATION in apbf_oracle.cpp) into `out_dir` and returns
+    the loaded library. Used only to show that a pin fails on a planted mistake."""
+    out = os.path.join(out_dir, f"liboracle_m{mutation}.so")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-mfma",
+                    f"-DORACLE_MUTATION={int(mutation)}", "-o", out, os.path.join(_HERE, "apbf_oracle.cpp")],
+                   check=True)
+    L = C.CDLL(out)
+    _declare(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -83,6 +96,8 @@ def _declare(L):
     L.orc_get_lambda.argtypes = [vp, pf]
     L.orc_get_splits.restype = u32
     L.orc_get_splits.argtypes = [vp, pu]
+    L.orc_get_fsm.restype = u32
+    L.orc_get_fsm.argtypes = [vp, pu, pu, pu]
     L.orc_stage.argtypes = [vp, C.c_int, f32]
     L.orc_set_neighborhood_method.argtypes = [vp, C.c_int]
     L.orc_set_sphere.argtypes = [vp, pf, f32, C.c_int, C.c_int]
@@ -164,8 +179,8 @@ class OracleScene:
     """One scene held by the oracle. State dicts use the keys of
     :data:`STATE_KEYS`."""
 
-    def __init__(self, params: dict):
-        self._L = lib()
+    def __init__(self, params: dict, library=None):
+        self._L = library if library is not None else lib()
         self._params = make_params(params)
         self._h = self._L.orc_create(C.byref(self._params))
         if params.get("sphere"):  # (center, radius, hollow) of the scene's sphere collider
@@ -244,6 +259,15 @@ class OracleScene:
         out = np.zeros(nf, np.float32)
         self._L.orc_get_lambda(self._h, _ptr(out, C.c_float))
         return out
+
+    def fsm(self):
+        """Intermediates of the last update_transfers stage per fluid id: the
+        boundary distance after the flood fill, the shortest neighbour distance
+        and the nearest neighbour (all u32)."""
+        n = self._L.orc_get_fsm(self._h, None, None, None)
+        bd, md, nn = (np.zeros(max(n, 1), np.uint32) for _ in range(3))
+        self._L.orc_get_fsm(self._h, _ptr(bd, C.c_uint32), _ptr(md, C.c_uint32), _ptr(nn, C.c_uint32))
+        return bd[:n], md[:n], nn[:n]
 
     def splits(self) -> np.ndarray:
         n = self._L.orc_get_splits(self._h, None)

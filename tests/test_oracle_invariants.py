@@ -267,9 +267,10 @@ def test_box_collision_pushes_out_of_walls():
 def test_update_transfers_is_a_legal_schedule():
     """find_split_and_merge_3.comp:96-121 claims source and target with
     atomicExchange(transferring): whatever the schedule, no particle takes part
-    in two transfers, exactly the claimed particles are flagged, merges have a
-    positive time left and splits a non-positive one, and every split particle
-    satisfies the split predicate r_t 2^(1/D) 0.99 <= r at decision time."""
+    in two transfers, exactly the claimed particles are flagged, and merges have
+    a positive time left and splits a non-positive one, over eight substeps.
+    The predicates at decision time and the maximality of the claims are
+    checked per find in tests/test_oracle_pins.py."""
     p, st = cfg.small_water_drop(seed=11, active=True)
     o = OracleScene(p)
     o.set_state(st)
