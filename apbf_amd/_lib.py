@@ -125,7 +125,30 @@ PROTOTYPES = {
     "apbf_hidden_edit_workspace_size": (_sz, [_u32, _u32, _u32]),
     "apbf_apply_hidden_edit": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
                                       _sz, _vp]),
+    "apbf_delete_edit_list_workspace_size": (_sz, [_u32]),
     "apbf_delete_edit_list": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "apbf_append_list": (_i32, [_vp, _u32, _vp, _vp, _vp, _u32, _vp, _u32, _vp]),
+    "apbf_write_increasing_sequence": (_i32, [_vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp]),
+    "apbf_write_increasing_sequence_from_to": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp]),
+    "apbf_write_sequence": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp]),
+    "apbf_device_alloc": (_i32, [_sz, C.POINTER(_vp)]),
+    "apbf_device_free": (_i32, [_vp]),
+    "apbf_copy": (_i32, [_vp, _vp, _sz, _vp]),
+    "apbf_stream_synchronize": (_i32, [_vp]),
+    "apbf_context_create": (_i32, [C.POINTER(SceneParams), C.POINTER(_vp)]),
+    "apbf_context_destroy": (_i32, [_vp]),
+    "apbf_op_velocity_handling": (_i32, [_vp, _vp, _pf, _f32, _pf, _vp]),
+    "apbf_op_particle_transfer": (_i32, [_vp, _vp, _vp, _f32, _vp]),
+    "apbf_op_kernel_width_from_boundary_distance": (_i32, [_vp, _vp, _vp]),
+    "apbf_op_neighborhood": (_i32, [_vp, _vp, _vp, _vp, _i32, _f32, _pf, _pf, _u32, _vp]),
+    "apbf_op_spread_kernel_width": (_i32, [_vp, _vp, _vp, _vp]),
+    "apbf_op_box_collision": (_i32, [_vp, _vp, _u32, _pf, _pf, _vp]),
+    "apbf_op_sphere_collision": (_i32, [_vp, _vp, _pf, _f32, _i32, _vp]),
+    "apbf_op_incompressibility": (_i32, [_vp, _vp, _vp, _vp]),
+    "apbf_op_update_transfers": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "apbf_neighbor_pairs": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
+    "apbf_neighbor_rows_from_pairs_workspace_size": (_sz, [_u32, _u32]),
+    "apbf_neighbor_rows_from_pairs": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
     "apbf_scene_set_gravity": (_i32, [_vp, _pf]),
     "apbf_scene_set_neighborhood_method": (_i32, [_vp, _i32]),
     "apbf_scene_set_range_scale": (_i32, [_vp, _f32]),

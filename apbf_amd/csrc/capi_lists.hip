@@ -77,6 +77,63 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_kept(const uint32_t* __restri
     GRID_STRIDE(i, n) if (flags[i]) edit[scan[i]] = i;
 }
 
+// append_list.comp:21-25: the new length is clamped to the target's capacity,
+// and words that would land beyond it are dropped.
+__global__ __launch_bounds__(BLOCK) void k_append(uint32_t* __restrict__ target, const uint32_t* __restrict__ app,
+                                                  const uint32_t* __restrict__ d_tlen,
+                                                  const uint32_t* __restrict__ d_alen, uint32_t target_cap,
+                                                  uint32_t words, uint32_t* __restrict__ d_new_len) {
+    const uint32_t tl = *d_tlen, al = *d_alen;
+    const uint64_t room = tl < target_cap ? (uint64_t)(target_cap - tl) * words : 0ull;
+    const uint64_t n = min((uint64_t)al * words, room);
+    const uint64_t base = (uint64_t)tl * words;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        target[base + i] = app[i];
+}
+__global__ void k_append_len(const uint32_t* __restrict__ d_tlen, const uint32_t* __restrict__ d_alen,
+                             uint32_t target_cap, uint32_t* __restrict__ d_new_len) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint64_t t = (uint64_t)*d_tlen + (uint64_t)*d_alen;
+        *d_new_len = (uint32_t)min(t, (uint64_t)target_cap);
+    }
+}
+// write_increasing_sequence.comp:23-33
+__global__ __launch_bounds__(BLOCK) void k_increasing_sequence(uint32_t* __restrict__ target, uint32_t target_cap,
+                                                               const uint32_t* __restrict__ d_min, uint32_t upper,
+                                                               uint32_t seq_len) {
+    const uint32_t mn = *d_min;
+    const uint32_t len = min(min(target_cap, seq_len), upper > mn ? upper - mn : 0u);
+    GRID_STRIDE(i, len) target[i] = mn + i;
+}
+__global__ void k_increasing_sequence_len(uint32_t target_cap, const uint32_t* __restrict__ d_min, uint32_t upper,
+                                          uint32_t seq_len, uint32_t* __restrict__ d_new_min,
+                                          uint32_t* __restrict__ d_new_len) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t mn = *d_min;
+        const uint32_t len = min(min(target_cap, seq_len), upper > mn ? upper - mn : 0u);
+        *d_new_min = mn + len;
+        *d_new_len = len;
+    }
+}
+// write_increasing_sequence_from_to.comp:19-27 (the output capacity bounds the writes)
+__global__ __launch_bounds__(BLOCK) void k_sequence_from_to(uint32_t* __restrict__ out, uint32_t out_cap,
+                                                            const uint32_t* __restrict__ d_from,
+                                                            const uint32_t* __restrict__ d_to) {
+    const uint32_t from = *d_from, to = *d_to;
+    const uint32_t len = min(to > from ? to - from : 0u, out_cap);
+    GRID_STRIDE(i, len) out[i] = from + i;
+}
+__global__ void k_sequence_from_to_len(const uint32_t* __restrict__ d_from, const uint32_t* __restrict__ d_to,
+                                       uint32_t* __restrict__ d_out_len) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d_out_len = *d_to - *d_from;
+}
+// write_sequence.comp:15-20: out[i] = start + i * step for i < *d_len
+__global__ __launch_bounds__(BLOCK) void k_write_sequence(uint32_t* __restrict__ out, const uint32_t* __restrict__ d_len,
+                                                          uint32_t start, uint32_t step) {
+    const uint32_t n = *d_len;
+    GRID_STRIDE(i, n) out[i] = start + i * step;
+}
+
 }  // namespace apbf
 
 using namespace apbf;
@@ -107,6 +164,82 @@ apbf_status apbf_radix_sort(const uint32_t* keys, const uint32_t* values, const 
     if (ws_bytes < sort_workspace_bytes(capacity)) return APBF_ERR_WORKSPACE;
     radix_sort_pairs(keys, values, d_len, capacity, upper_bound, out_keys, out_values, ws, S(stream));
     return last_status();
+}
+
+apbf_status apbf_append_list(void* target, uint32_t target_capacity, const uint32_t* d_target_len,
+                             const void* appending, const uint32_t* d_append_len, uint32_t stride,
+                             uint32_t* d_new_len, uint32_t append_capacity, apbf_stream stream) {
+    if (!target || !d_target_len || !d_append_len || !d_new_len || stride == 0 || (stride % 4) != 0 ||
+        (!appending && append_capacity))
+        return APBF_ERR_INVALID_ARGUMENT;
+    hipStream_t s = S(stream);
+    const uint32_t words = stride / 4;
+    if (appending && append_capacity)
+        hipLaunchKernelGGL(k_append, dim3(stride_grid((uint64_t)append_capacity * words)), dim3(BLOCK), 0, s,
+                           static_cast<uint32_t*>(target), static_cast<const uint32_t*>(appending), d_target_len,
+                           d_append_len, target_capacity, words, d_new_len);
+    hipLaunchKernelGGL(k_append_len, dim3(1), dim3(64), 0, s, d_target_len, d_append_len, target_capacity,
+                       d_new_len);
+    return last_status();
+}
+
+apbf_status apbf_write_increasing_sequence(uint32_t* target, uint32_t target_capacity,
+                                           const uint32_t* d_sequence_min, uint32_t value_upper_bound,
+                                           uint32_t sequence_length, uint32_t* d_new_sequence_min,
+                                           uint32_t* d_new_target_len, apbf_stream stream) {
+    if ((!target && target_capacity) || !d_sequence_min || !d_new_sequence_min || !d_new_target_len ||
+        d_new_sequence_min == d_sequence_min)
+        return APBF_ERR_INVALID_ARGUMENT;
+    hipStream_t s = S(stream);
+    const uint32_t grid_n = target_capacity < sequence_length ? target_capacity : sequence_length;
+    if (grid_n)
+        hipLaunchKernelGGL(k_increasing_sequence, dim3(stride_grid(grid_n)), dim3(BLOCK), 0, s, target,
+                           target_capacity, d_sequence_min, value_upper_bound, sequence_length);
+    hipLaunchKernelGGL(k_increasing_sequence_len, dim3(1), dim3(64), 0, s, target_capacity, d_sequence_min,
+                       value_upper_bound, sequence_length, d_new_sequence_min, d_new_target_len);
+    return last_status();
+}
+
+apbf_status apbf_write_increasing_sequence_from_to(uint32_t* out, uint32_t out_capacity, uint32_t* d_out_len,
+                                                   const uint32_t* d_from, const uint32_t* d_to, apbf_stream stream) {
+    if ((!out && out_capacity) || !d_out_len || !d_from || !d_to) return APBF_ERR_INVALID_ARGUMENT;
+    hipStream_t s = S(stream);
+    if (out_capacity)
+        hipLaunchKernelGGL(k_sequence_from_to, dim3(stride_grid(out_capacity)), dim3(BLOCK), 0, s, out, out_capacity,
+                           d_from, d_to);
+    hipLaunchKernelGGL(k_sequence_from_to_len, dim3(1), dim3(64), 0, s, d_from, d_to, d_out_len);
+    return last_status();
+}
+
+apbf_status apbf_write_sequence(uint32_t* out, const uint32_t* d_len, uint32_t capacity, uint32_t start, uint32_t step,
+                                apbf_stream stream) {
+    if ((!out && capacity) || !d_len) return APBF_ERR_INVALID_ARGUMENT;
+    if (capacity)
+        hipLaunchKernelGGL(k_write_sequence, dim3(stride_grid(capacity)), dim3(BLOCK), 0, S(stream), out, d_len, start,
+                           step);
+    return last_status();
+}
+
+apbf_status apbf_device_alloc(size_t bytes, void** out) {
+    if (!out) return APBF_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (hipMalloc(out, bytes ? bytes : 4) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_device_free(void* p) {
+    if (p && hipFree(p) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_copy(void* dst, const void* src, size_t bytes, apbf_stream stream) {
+    if ((!dst || !src) && bytes) return APBF_ERR_INVALID_ARGUMENT;
+    if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)) != hipSuccess) return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_stream_synchronize(apbf_stream stream) {
+    return hipStreamSynchronize(S(stream)) == hipSuccess ? APBF_OK : APBF_ERR_HIP;
 }
 
 apbf_status apbf_calculate_position_hash(const int32_t* position, const uint32_t* d_len, uint32_t capacity,
@@ -261,14 +394,17 @@ apbf_status apbf_apply_hidden_edit(const uint32_t* index_list, const uint32_t* d
     return last_status();
 }
 
+size_t apbf_delete_edit_list_workspace_size(uint32_t hidden_capacity) {
+    return scan_workspace_bytes(hidden_capacity) + (size_t)hidden_capacity * 8 + 64;  // flags, scan, scan tiles
+}
+
 apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_deleted_len,
                                   uint32_t deleted_capacity, const uint32_t* d_hidden_len,
                                   uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len, void* ws,
                                   size_t ws_bytes, apbf_stream stream) {
     if (!deleted || !d_deleted_len || !d_hidden_len || !edit || !d_edit_len || !ws)
         return APBF_ERR_INVALID_ARGUMENT;
-    if (ws_bytes < scan_workspace_bytes(hidden_capacity) + (size_t)hidden_capacity * 8 + 64)
-        return APBF_ERR_WORKSPACE;
+    if (ws_bytes < apbf_delete_edit_list_workspace_size(hidden_capacity)) return APBF_ERR_WORKSPACE;
     hipStream_t s = S(stream);
     uint32_t* flags = static_cast<uint32_t*>(ws);
     uint32_t* scan = flags + hidden_capacity;

@@ -34,8 +34,8 @@ apbf_status ensure_stream(Scene* s) {
     if (s->stream) return APBF_OK;
     if (!s->own_stream && hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess)
         return APBF_ERR_HIP;
-    s->stream = s->own_stream;
-    return APBF_OK;
+    // the new stream does not wait for the null stream by itself
+    return scene_switch_stream(s, s->own_stream);
 }
 
 void drop_graph(Scene* s) {
@@ -48,6 +48,20 @@ void drop_graph(Scene* s) {
 }  // namespace
 
 namespace apbf {
+// Work already queued on the scene's current stream finishes before anything
+// queued on `next` afterwards: an event recorded on the old stream, waited
+// for by the new one (no host synchronisation).
+apbf_status scene_switch_stream(Scene* s, hipStream_t next) {
+    if (next == s->stream) return APBF_OK;
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return APBF_ERR_HIP;
+    apbf_status st = APBF_OK;
+    if (hipEventRecord(ev, s->stream) != hipSuccess || hipStreamWaitEvent(next, ev, 0) != hipSuccess)
+        st = APBF_ERR_HIP;
+    (void)hipEventDestroy(ev);
+    if (st == APBF_OK) s->stream = next;
+    return st;
+}
 void scene_drop_graph(Scene* s) { drop_graph(s); }
 void scene_release_graph(Scene* s) {
     drop_graph(s);
@@ -165,12 +179,12 @@ apbf_status apbf_batch_synchronize(apbf_batch* b) {
 }
 
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes) {
-    if (!scene || (walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) ||
+    if (!scene || (walk_lanes != 0 && walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) ||
         (solve_lanes != 8 && solve_lanes != 16 && solve_lanes != 32))
         return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
-    s->walk_lanes = walk_lanes;
-    s->walk_auto = false;
+    s->walk_auto = walk_lanes == 0;  // 0: lanes from a recent mean row length again
+    if (walk_lanes != 0) s->walk_lanes = walk_lanes;
     s->solve_lanes = solve_lanes;
     drop_graph(s);  // a captured graph holds the old shapes
     return APBF_OK;
@@ -184,8 +198,9 @@ apbf_status apbf_scene_set_long_rows(void* scene, int32_t mode) {
     return APBF_OK;
 }
 
-// Times `substeps` substeps under every launch shape (the scene advances by
-// 9 * substeps) and keeps the fastest. All shapes compute identical results.
+// Times `substeps` substeps under every launch shape, each from the same saved
+// state, and keeps the fastest; the scene is put back afterwards. All shapes
+// compute identical results.
 apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_t* best_walk, int32_t* best_solve,
                                 float* best_ms) {
     if (!scene || substeps == 0) return APBF_ERR_INVALID_ARGUMENT;

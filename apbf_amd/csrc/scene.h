@@ -79,6 +79,9 @@ struct Scene {
     // after every substep (so it lags by a substep or two). It only steers
     // the choice between launch variants that compute the same results.
     uint32_t* h_counts = nullptr;
+    hipEvent_t h_counts_ready = nullptr;  // recorded after each copy into h_counts
+    uint32_t seen_n = 0, seen_pairs = 0;   // particles and pairs of the last completed copy
+    void refresh_seen_counts();
     bool long_rows = false;  // mean neighbour row of a recent substep above 62 entries
     int long_rows_mode = -1;  // -1: long_rows from h_counts; 0 / 1: pinned
     // neighbourhood
@@ -104,6 +107,7 @@ struct Scene {
     int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8 or 16
     bool walk_auto = true;  // walk lanes from a recent mean row length (4 below 62 entries, else 8)
     int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
+    bool exact_walks = false;  // pair build by the two exact walks only (no fixed-stride slots)
     // captured substeps (two per graph: the buffer roles repeat after two)
     hipStream_t own_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -163,6 +167,7 @@ struct Scene {
     void accumulate_density_error();
 };
 
+apbf_status scene_switch_stream(Scene* s, hipStream_t next);  // graph.hip: stream-ordered switch
 void scene_drop_graph(Scene* s);     // graph.hip: drops a captured graph
 void scene_release_graph(Scene* s);  // graph.hip: graph and the scene's own stream
 

@@ -149,6 +149,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_counts), C_COUNT * 4, hipHostMallocDefault));
     std::memset(h_counts, 0, C_COUNT * 4);
+    HIP_OK(hipEventCreateWithFlags(&h_counts_ready, hipEventDisableTiming));
     HIP_OK(hipDeviceSynchronize());
     return APBF_OK;
 }
@@ -168,6 +169,7 @@ Scene::~Scene() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
     if (h_counts) (void)hipHostFree(h_counts);
+    if (h_counts_ready) (void)hipEventDestroy(h_counts_ready);
 }
 
 // ------------------------------------------------------ velocity_handling
@@ -209,13 +211,12 @@ void Scene::neighborhood_green(float range_scale, bool spread) {
 void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;
     if (walk_auto) {  // short rows: fewer lanes per particle fill a wave better (large pool: 22.1 vs 22.6 ms)
-        const volatile uint32_t* hc = h_counts;
-        const uint32_t hn = hc[C_N];
-        walk_lanes = (hn != 0u && (uint64_t)hc[C_PAIRS] < 62ull * (uint64_t)hn) ? 4 : 8;
+        refresh_seen_counts();
+        walk_lanes = (seen_n != 0u && (uint64_t)seen_pairs < 62ull * (uint64_t)seen_n) ? 4 : 8;
     }
     const int on_t = P.s.mBaseKernelWidthOnTargetRadius;
     uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list
-    const uint32_t stride = pcap / (cap ? cap : 1u);
+    const uint32_t stride = exact_walks ? 0u : pcap / (cap ? cap : 1u);
     launch_set_n_plus_1(counts, s);
     if (spread) spread_begin();
     if (stride == 0) {  // no room for slots: the two exact walks
@@ -307,10 +308,24 @@ SolverArgs Scene::solver_args() const {
     return a;
 }
 
+// The counts of the last substep whose copy to the host has completed (the
+// copy is stream-ordered and never waited for; until one completes, the
+// previous values stay). Inside stream capture no copy is recorded, so a
+// captured graph keeps the launch choices of the state it was captured from.
+void Scene::refresh_seen_counts() {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    if (hipEventQuery(h_counts_ready) != hipSuccess) return;  // still in flight (or never recorded)
+    seen_n = h_counts[C_N];
+    seen_pairs = h_counts[C_PAIRS];
+}
+
 void Scene::prepare_solver() {
-    {
-        const volatile uint32_t* hc = h_counts;  // written by an earlier substep's copy
-        long_rows = long_rows_mode >= 0 ? long_rows_mode == 1 : (uint64_t)hc[C_PAIRS] > 62ull * (uint64_t)hc[C_N];
+    if (long_rows_mode >= 0) {
+        long_rows = long_rows_mode == 1;
+    } else {
+        refresh_seen_counts();
+        long_rows = (uint64_t)seen_pairs > 62ull * (uint64_t)seen_n;
     }
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
                          P.s.mGradientKernelId, gc, stream);
@@ -385,8 +400,11 @@ void Scene::update(float dt) {
     // the count block to the host for the launch choices of later substeps;
     // not recorded into captured graphs
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone)
-        (void)hipMemcpyAsync(h_counts, counts, C_COUNT * 4, hipMemcpyDeviceToHost, stream);
+    if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+        refresh_seen_counts();  // read a completed copy before the next one is queued
+        if (hipMemcpyAsync(h_counts, counts, C_COUNT * 4, hipMemcpyDeviceToHost, stream) == hipSuccess)
+            (void)hipEventRecord(h_counts_ready, stream);
+    }
     if (measure_error && !P.basic_pbf) {
         update_transfers(true);  // the error pass rides on the split/merge find
     } else {
@@ -490,8 +508,7 @@ apbf_status apbf_scene_destroy(void* scene) {
 
 apbf_status apbf_scene_set_stream(void* scene, apbf_stream stream) {
     if (!scene) return APBF_ERR_INVALID_ARGUMENT;
-    static_cast<Scene*>(scene)->stream = static_cast<hipStream_t>(stream);
-    return APBF_OK;
+    return apbf::scene_switch_stream(static_cast<Scene*>(scene), static_cast<hipStream_t>(stream));
 }
 
 apbf_status apbf_scene_set_state(void* scene, uint32_t n, const int32_t* h_pos, const float* h_vel,

@@ -183,9 +183,63 @@ def delete_edit_list(deleted: torch.Tensor, n_hidden: int):
     cap = max(n_hidden, 1)
     edit = torch.zeros(cap, dtype=torch.int32, device=dev)
     el = torch.zeros(1, dtype=torch.int32, device=dev)
-    ws = _ws(L.apbf_prefix_sum_workspace_size(cap) + cap * 8 + 64, dev)
+    ws = _ws(L.apbf_delete_edit_list_workspace_size(cap), dev)
     check(L.apbf_delete_edit_list(_p(d), _p(_len(deleted.numel(), dev)), max(d.numel(), 1),
                                   _p(_len(n_hidden, dev)), cap, _p(edit), _p(el), _p(ws), ws.numel(), _stream()),
           "apbf_delete_edit_list")
     _sync()
     return edit[: int(el.item())]
+
+
+def append_list(target: torch.Tensor, n_target: int, appending: torch.Tensor, n_append: int, stride: int):
+    """gpu_list::operator+= (shader_provider::append_list): `target` is the
+    whole buffer (its capacity is target.numel() * 4 / stride); returns the
+    buffer and the clamped new length."""
+    L = _lib.lib()
+    dev = target.device
+    cap = target.numel() * 4 // stride
+    a = appending if appending.numel() else torch.zeros(1, dtype=torch.int32, device=dev)
+    new_len = torch.zeros(1, dtype=torch.int32, device=dev)
+    check(L.apbf_append_list(_p(target), cap, _p(_len(n_target, dev)), _p(a), _p(_len(n_append, dev)), stride,
+                             _p(new_len), appending.numel() * 4 // stride, _stream()), "apbf_append_list")
+    _sync()
+    return target, int(new_len.item())
+
+
+def write_increasing_sequence(target_capacity: int, sequence_min: int, value_upper_bound: int, sequence_length: int,
+                              device):
+    """shader_provider::write_increasing_sequence: (list, new sequence min, new length)."""
+    L = _lib.lib()
+    out = torch.zeros(max(target_capacity, 1), dtype=torch.int32, device=device)
+    new_min = torch.zeros(1, dtype=torch.int32, device=device)
+    new_len = torch.zeros(1, dtype=torch.int32, device=device)
+    check(L.apbf_write_increasing_sequence(_p(out), target_capacity, _p(_len(sequence_min, device)),
+                                           value_upper_bound & 0xFFFFFFFF, sequence_length, _p(new_min),
+                                           _p(new_len), _stream()), "apbf_write_increasing_sequence")
+    _sync()
+    n = int(new_len.item())
+    return out[:n], int(new_min.item()), n
+
+
+def write_increasing_sequence_from_to(out_capacity: int, start: int, stop: int, device):
+    """shader_provider::write_increasing_sequence_from_to: [start, stop) and its length."""
+    L = _lib.lib()
+    out = torch.zeros(max(out_capacity, 1), dtype=torch.int32, device=device)
+    n = torch.zeros(1, dtype=torch.int32, device=device)
+    check(L.apbf_write_increasing_sequence_from_to(_p(out), out_capacity, _p(n), _p(_len(start, device)),
+                                                   _p(_len(stop, device)), _stream()),
+          "apbf_write_increasing_sequence_from_to")
+    _sync()
+    k = int(n.item())
+    return out[:k], k
+
+
+def write_sequence(n: int, start: int, step: int, device, capacity: int | None = None):
+    """shader_provider::write_sequence: start + i * step for i < n."""
+    L = _lib.lib()
+    cap = n if capacity is None else capacity
+    out = torch.zeros(max(cap, 1), dtype=torch.int32, device=device)
+    check(L.apbf_write_sequence(_p(out), _p(_len(n, device)), cap, start & 0xFFFFFFFF, step & 0xFFFFFFFF, _stream()),
+          "apbf_write_sequence")
+    _sync()
+    return out[:n]

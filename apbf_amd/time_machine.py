@@ -56,6 +56,11 @@ class TimeMachine:
     def sparse_history_index(self) -> int:
         return self.history_index() // self.keyframe_interval
 
+    def current_dt(self) -> float:
+        """The time step recorded for the current step: the per-substep
+        (non-sparse) history value that stepping through the past restores."""
+        return self._dts[self.history_index()]
+
     # ---- snapshots
     def _take(self):
         if self._spare:

@@ -91,6 +91,8 @@ typedef struct apbf_scene_params {
  * Replaces the scene classes pool / waterdrop (pool.h:14-56, waterdrop.h). */
 apbf_status apbf_scene_create(const apbf_scene_params* params, void** out_scene);
 apbf_status apbf_scene_destroy(void* scene);
+/* Later calls on the scene run on `stream`; work already queued on the old
+ * stream completes before them (an event, no host synchronisation). */
 apbf_status apbf_scene_set_stream(void* scene, apbf_stream stream);
 
 /* Upload a state from host arrays (layout as the reference lists: position,
@@ -276,12 +278,53 @@ apbf_status apbf_apply_hidden_edit(const uint32_t* index_list, const uint32_t* d
 
 /* Edit list of indexed_list::delete_these (indexed_list.h:126-138): the hidden
  * slots [0, d_hidden_len) not named in `deleted`, ascending; *d_edit_len receives
- * the count. `workspace` >= apbf_prefix_sum_workspace_size(hidden_capacity) +
- * 4 * hidden_capacity. */
+ * the count. `workspace` >= apbf_delete_edit_list_workspace_size(hidden_capacity). */
+size_t apbf_delete_edit_list_workspace_size(uint32_t hidden_capacity);
 apbf_status apbf_delete_edit_list(const uint32_t* deleted, const uint32_t* d_deleted_len,
                                   uint32_t deleted_capacity, const uint32_t* d_hidden_len,
                                   uint32_t hidden_capacity, uint32_t* edit, uint32_t* d_edit_len,
                                   void* workspace, size_t workspace_bytes, apbf_stream stream);
+
+/* gpu_list::operator+= (gpu_list.h:214-227; shader_provider::append_list,
+ * shader_provider.cpp:49-72, append_list.comp:21-25): appends the
+ * *d_append_len elements of `appending` (stride bytes each, a multiple of 4)
+ * after the *d_target_len elements of `target`; *d_new_len = min(target
+ * length + appended length, target_capacity), and elements that would land
+ * beyond the capacity are dropped. `append_capacity` bounds the launch (the
+ * appended list's allocated length); appending may be NULL when it is 0.
+ * d_new_len may equal d_target_len. */
+apbf_status apbf_append_list(void* target, uint32_t target_capacity, const uint32_t* d_target_len,
+                             const void* appending, const uint32_t* d_append_len, uint32_t stride,
+                             uint32_t* d_new_len, uint32_t append_capacity, apbf_stream stream);
+/* shader_provider::write_increasing_sequence (shader_provider.cpp:198-218,
+ * write_increasing_sequence.comp:23-33), the core of
+ * indexed_list::increase_length (indexed_list.h:194-202): writes
+ * min + 0, 1, ... into target for len = min(target_capacity, sequence_length,
+ * value_upper_bound - min) elements, with min = *d_sequence_min;
+ * *d_new_sequence_min = min + len and *d_new_target_len = len (the two length
+ * results the reference returns). */
+apbf_status apbf_write_increasing_sequence(uint32_t* target, uint32_t target_capacity,
+                                           const uint32_t* d_sequence_min, uint32_t value_upper_bound,
+                                           uint32_t sequence_length, uint32_t* d_new_sequence_min,
+                                           uint32_t* d_new_target_len, apbf_stream stream);
+/* shader_provider::write_increasing_sequence_from_to (shader_provider.cpp:178-196,
+ * write_increasing_sequence_from_to.comp:19-27), used by
+ * indexed_list::duplicate_these (indexed_list.h:141-152): out[i] = from + i
+ * for from + i < to (at most out_capacity values), *d_out_len = to - from. */
+apbf_status apbf_write_increasing_sequence_from_to(uint32_t* out, uint32_t out_capacity, uint32_t* d_out_len,
+                                                   const uint32_t* d_from, const uint32_t* d_to, apbf_stream stream);
+/* shader_provider::write_sequence (write_sequence.comp:15-20): out[i] =
+ * start + i * step for i < *d_len (fills with step 0, iota with step 1). */
+apbf_status apbf_write_sequence(uint32_t* out, const uint32_t* d_len, uint32_t capacity, uint32_t start, uint32_t step,
+                                apbf_stream stream);
+/* Device memory of the host-side lists (the buffers gpu_list_data draws from
+ * its pool, gpu_list_data.cpp:6-45), a stream-ordered copy in any direction
+ * (algorithms::copy_bytes, algorithms.cpp:4-36) and a stream
+ * synchronisation (the reference's end_recording + wait). */
+apbf_status apbf_device_alloc(size_t bytes, void** out);
+apbf_status apbf_device_free(void* ptr);
+apbf_status apbf_copy(void* dst, const void* src, size_t bytes, apbf_stream stream);
+apbf_status apbf_stream_synchronize(apbf_stream stream);
 
 /* Scene settings the reference's subsystems take through setters:
  * velocity_handling::set_acceleration (velocity_handling.h:12),
@@ -310,6 +353,120 @@ apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method);
 apbf_status apbf_scene_set_range_scale(void* scene, float scale);
 apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2);
 
+/* -------------------------------------------- stage operators over lists */
+/* The subsystem apply() calls of the reference run on the lists they were
+ * bound to with set_data (velocity_handling.h:10-12, particle_transfer.h:11-12,
+ * neighborhood_green.h:11-14, spread_kernel_width.h:10-11, box_collision.h:10-11,
+ * incompressibility.h:11-12, update_transfers.h:11-12). The operators below do
+ * the same on caller-owned device arrays: the particle lists of
+ * list_definitions.h:9-13 (hidden particles and fluid attributes, stored with
+ * fluid id == hidden slot, as the library's scenes keep them), the transfer
+ * lists (list_definitions.h:15-16, endpoints are particle slots) and the
+ * neighbour list (list_definitions.h:19) as CSR rows: row_offset[i] ..
+ * row_offset[i + 1] index the neighbours of particle i for i < *d_rows (the
+ * particles present when the list was built), *d_length pairs in all
+ * (apbf_neighbor_pairs gives the reference's uvec2 layout). Lengths are
+ * device words, read and written by the operators on the stream. */
+typedef struct apbf_particle_lists {
+    int32_t* position;           /* 4 x int32 per particle, fixed point 2^-18 */
+    float* velocity;             /* 4 x f32 */
+    float* inverse_mass;
+    float* radius;
+    int32_t* pos_backup;         /* 4 x int32 */
+    uint32_t* transferring;
+    float* target_radius;
+    float* kernel_width;
+    float* boundariness;
+    uint32_t* boundary_distance; /* fixed point 2^-18 */
+    uint32_t* d_length;          /* particles */
+    uint32_t capacity;           /* must equal the context's particle capacity */
+} apbf_particle_lists;
+typedef struct apbf_transfer_lists {
+    uint32_t* source;
+    uint32_t* target;
+    float* time_left;            /* > 0 merge, <= 0 split (particle_transfer.comp:37-84) */
+    uint32_t* d_length;
+    uint32_t capacity;           /* must equal the context's transfer capacity */
+} apbf_transfer_lists;
+typedef struct apbf_neighbor_lists {
+    uint32_t* row_offset;        /* particle capacity + 1 entries */
+    uint32_t* neighbors;         /* neighbour ids, `capacity` entries */
+    uint32_t* d_length;          /* pairs */
+    uint32_t* d_rows;            /* particles the rows cover */
+    uint32_t capacity;           /* must equal the context's pair capacity */
+} apbf_neighbor_lists;
+/* The temporaries of every stage for the capacities and settings of
+ * `params` (cap_particles, cap_transfers, cap_pairs; the apbf_settings
+ * uniform block; the hash cell count 2^(3 hash_res_log2)). Allocates. One
+ * context serves one stream at a time. */
+typedef struct apbf_context apbf_context;
+apbf_status apbf_context_create(const apbf_scene_params* params, apbf_context** out_context);
+apbf_status apbf_context_destroy(apbf_context* context);
+/* velocity_handling::apply (velocity_handling.cpp:15-31; shader_provider
+ * infer_velocity / apply_acceleration / apply_velocity,
+ * shader_provider.cpp:1288-1354): *last_delta_time is the member
+ * mLastDeltaTime (velocity_handling.h:17), read and updated. */
+apbf_status apbf_op_velocity_handling(apbf_context* context, const apbf_particle_lists* particles,
+                                      float* last_delta_time, float delta_time, const float* acceleration3,
+                                      apbf_stream stream);
+/* particle_transfer::apply (particle_transfer.cpp:10-28; shader_provider::
+ * particle_transfer, shader_provider.cpp:1247): mass and volume moves, then
+ * the finished transfers' particles are deleted (indexed_list::delete_these)
+ * and both lists compacted. */
+apbf_status apbf_op_particle_transfer(apbf_context* context, const apbf_particle_lists* particles,
+                                      const apbf_transfer_lists* transfers, float delta_time, apbf_stream stream);
+/* uint_to_float_with_indexed_lower_bound (pool.cpp:77-80,
+ * shader_provider.cpp:305-330; settings from the context). */
+apbf_status apbf_op_kernel_width_from_boundary_distance(apbf_context* context, const apbf_particle_lists* particles,
+                                                        apbf_stream stream);
+/* neighborhood_green::apply (neighborhood_green.cpp:27-77), or with method 1
+ * neighborhood_binary_search::apply (neighborhood_binary_search.cpp:22-75),
+ * method 2 neighborhood_brute_force::apply: the particles are reordered by
+ * cell (the hidden-list edit of neighborhood_green.cpp:50), the transfer
+ * endpoints follow them (indexed_list owner rebuild), and the neighbour list
+ * is rebuilt with range kernel_width * range_scale. res_log2 must give the
+ * context's cell count. transfers may be NULL (no transfers). */
+apbf_status apbf_op_neighborhood(apbf_context* context, const apbf_particle_lists* particles,
+                                 const apbf_transfer_lists* transfers, const apbf_neighbor_lists* neighbors,
+                                 int32_t method, float range_scale, const float* min3, const float* max3,
+                                 uint32_t res_log2, apbf_stream stream);
+/* spread_kernel_width::apply (spread_kernel_width.cpp:12-26; kernel_width_init,
+ * kernel_width, uint_to_float_but_gradual, shader_provider.cpp:826-875,284-304):
+ * the neighbour list is replaced by its kept pairs. */
+apbf_status apbf_op_spread_kernel_width(apbf_context* context, const apbf_particle_lists* particles,
+                                        const apbf_neighbor_lists* neighbors, apbf_stream stream);
+/* box_collision::apply (box_collision.cpp:12-24; shader_provider::box_collision,
+ * shader_provider.cpp:559) with boxes [box_min4[4i..], box_max4[4i..]]. */
+apbf_status apbf_op_box_collision(apbf_context* context, const apbf_particle_lists* particles, uint32_t n_boxes,
+                                  const float* host_box_min4, const float* host_box_max4, apbf_stream stream);
+/* sphere_collision::apply (sphere_collision.comp:42-60). */
+apbf_status apbf_op_sphere_collision(apbf_context* context, const apbf_particle_lists* particles,
+                                     const float* host_center3, float radius, int32_t hollow, apbf_stream stream);
+/* incompressibility::apply (incompressibility.cpp:12-45): one Jacobi
+ * iteration, incompressibility_0..3 (shader_provider.cpp:970-1093). */
+apbf_status apbf_op_incompressibility(apbf_context* context, const apbf_particle_lists* particles,
+                                      const apbf_neighbor_lists* neighbors, apbf_stream stream);
+/* update_transfers::apply (update_transfers.cpp:14-71): find_split_and_merge_1..3
+ * (shader_provider.cpp:1094-1192), remove_impossible_splits and
+ * initialize_split_particles (:1193-1246), duplicate_these, appends. */
+apbf_status apbf_op_update_transfers(apbf_context* context, const apbf_particle_lists* particles,
+                                     const apbf_transfer_lists* transfers, const apbf_neighbor_lists* neighbors,
+                                     apbf_stream stream);
+/* The neighbour list in the reference's layout: out_pairs[2k], out_pairs[2k+1]
+ * = (particle, neighbour) of pair k, in row order (list_definitions.h:19,
+ * neighbor_add.glsl:20-23). rows_capacity bounds the launch. */
+apbf_status apbf_neighbor_pairs(const uint32_t* row_offset, const uint32_t* neighbors, const uint32_t* d_rows,
+                                const uint32_t* d_n_pairs, uint32_t rows_capacity, uint32_t pair_capacity,
+                                uint32_t* out_pairs, apbf_stream stream);
+/* The reverse: CSR rows over `rows` particles from *d_n_pairs uvec2 pairs in
+ * any order (an atomic-order list of neighborhood_green.comp); the pairs of
+ * one particle keep their relative order. First ids must be < rows.
+ * row_offset holds rows + 1 entries. */
+size_t apbf_neighbor_rows_from_pairs_workspace_size(uint32_t pair_capacity, uint32_t rows);
+apbf_status apbf_neighbor_rows_from_pairs(const uint32_t* pairs, const uint32_t* d_n_pairs, uint32_t pair_capacity,
+                                          uint32_t rows, uint32_t* row_offset, uint32_t* neighbors, void* workspace,
+                                          size_t workspace_bytes, apbf_stream stream);
+
 /* ------------------------------------------ graphs, batches, launch shapes */
 /* No counterpart in the reference, which records one command buffer with a
  * barrier after every dispatch (shader_provider.cpp:20-47). */
@@ -335,7 +492,10 @@ apbf_status apbf_batch_synchronize(apbf_batch* batch);
  * row in the solver (8, 16, 32); every shape computes identical results.
  * Until a shape is set or tuned, the walk takes 4 lanes when a recent
  * substep's mean neighbour row is below 62 entries and 8 otherwise, and the
- * solver 16.
+ * solver 16; walk_lanes 0 returns the walk to that choice. The recent
+ * substep's counts reach the host by a copy that is never waited for; a
+ * graph captured by apbf_scene_capture keeps the choices of the state it
+ * was captured from.
  * apbf_scene_autotune times `substeps` substeps under each shape, every
  * shape from the same saved state (twice, keeping the faster time), and keeps
  * the fastest; the scene and its statistics are put back afterwards, except

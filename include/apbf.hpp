@@ -1,10 +1,13 @@
 /* C++ host interface with the shape of the reference's subsystem classes
- * (namespace pbd in source/*.h): each class binds to a scene with set_data
- * and runs its stage with apply(), so a caller written against
- * velocity_handling / particle_transfer / neighborhood_green /
- * spread_kernel_width / box_collision / incompressibility / update_transfers
- * keeps its call sequence (pool.cpp:67-106). Header-only over include/apbf.h;
- * errors become apbf_pbd::error exceptions (the reference asserts). */
+ * (namespace pbd in the source/ headers). Each class binds with set_data either to
+ * the lists it works on (apbf_lists.hpp: fluid, transfers, neighbors - the
+ * reference's set_data(fluid*, neighbors*, ...) signatures) and then runs its
+ * stage operator on those lists, or to a whole scene and then runs the
+ * scene's stage. A caller written against velocity_handling /
+ * particle_transfer / neighborhood_green / spread_kernel_width /
+ * box_collision / incompressibility / update_transfers keeps its call
+ * sequence (pool.cpp:67-106). Header-only over include/apbf.h; errors become
+ * apbf_pbd::error exceptions (the reference asserts). */
 #ifndef APBF_HPP
 #define APBF_HPP
 
@@ -15,33 +18,12 @@
 #include <vector>
 
 #include "apbf.h"
+#include "apbf_lists.hpp"
 
 namespace apbf_pbd {
 
-struct error : std::runtime_error {
-    apbf_status status;
-    error(const char* what, apbf_status s) : std::runtime_error(std::string(what) + ": status " + std::to_string(s)), status(s) {}
-};
-
-inline void check(apbf_status s, const char* what) {
-    if (s != APBF_OK) throw error(what, s);
-}
-
 struct vec3 {
     float x, y, z;
-};
-
-/* Host copy of one scene state (the lists of list_definitions.h:9-20 with the
- * fluid stored in hidden-slot order). */
-struct state {
-    std::vector<int32_t> position, pos_backup;  // 4 per particle
-    std::vector<float> velocity;                // 4 per particle
-    std::vector<float> inverse_mass, radius, target_radius, kernel_width, boundariness;
-    std::vector<uint32_t> transferring, boundary_distance;
-    std::vector<uint32_t> t_source, t_target;
-    std::vector<float> t_time_left;
-    float last_delta_time = 1.0f;
-    uint32_t size() const { return (uint32_t)radius.size(); }
 };
 
 /* The device-side lists of one simulation (the pool / waterdrop members). */
@@ -99,57 +81,189 @@ private:
     void* h_ = nullptr;
 };
 
+/* The temporaries of the stage operators (apbf_context) for the capacities,
+ * settings and hash grid of a scene's parameters. The list-bound subsystems
+ * below run on the context made current last (the reference's subsystems
+ * share one global shader_provider and settings block in the same way,
+ * shader_provider.cpp:5-8, settings.cpp:70-91). */
+class context {
+public:
+    explicit context(const apbf_scene_params& p) : params_(p) {
+        check(apbf_context_create(&p, &h_), "apbf_context_create");
+        current() = this;
+    }
+    ~context() {
+        if (current() == this) current() = nullptr;
+        if (h_) apbf_context_destroy(h_);
+    }
+    context(const context&) = delete;
+    context& operator=(const context&) = delete;
+    apbf_context* handle() const { return h_; }
+    const apbf_scene_params& params() const { return params_; }
+    void make_current() { current() = this; }
+    static context*& current() {
+        static context* c = nullptr;
+        return c;
+    }
+    static context& get() {
+        if (!current()) throw error("no current apbf_pbd::context", APBF_ERR_INVALID_ARGUMENT);
+        return *current();
+    }
+
+private:
+    apbf_scene_params params_;
+    apbf_context* h_ = nullptr;
+};
+
+/* The owner link of pool.cpp:18-19 (the transfer lists share the particles'
+ * hidden list): reorderings and deletions of the particles are applied to the
+ * transfer endpoints too. */
+inline std::vector<std::pair<fluid*, transfers*>>& transfer_owners() {
+    static std::vector<std::pair<fluid*, transfers*>> v;
+    return v;
+}
+inline void share_hidden_data(transfers& t, fluid& f) {
+    for (auto& e : transfer_owners())
+        if (e.second == &t) {
+            e.first = &f;
+            return;
+        }
+    transfer_owners().push_back({&f, &t});
+}
+inline transfers* transfers_of(fluid* f) {
+    for (auto& e : transfer_owners())
+        if (e.first == f) return e.second;
+    return nullptr;
+}
+
 /* velocity_handling.h:7-18 */
 class velocity_handling {
 public:
     velocity_handling& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
+        return *this;
+    }
+    velocity_handling& set_data(fluid* particles) {
+        f_ = particles;
+        s_ = nullptr;
         return *this;
     }
     velocity_handling& set_acceleration(const vec3& a = vec3{0, 0, 0}) {
-        const float g[3] = {a.x, a.y, a.z};
-        check(apbf_scene_set_gravity(s_->handle(), g), "set_acceleration");
+        acc_ = a;
+        if (s_) {
+            const float g[3] = {a.x, a.y, a.z};
+            check(apbf_scene_set_gravity(s_->handle(), g), "set_acceleration");
+        }
         return *this;
     }
-    void apply(float delta_time) { s_->stage(APBF_STAGE_VELOCITY, delta_time); }
+    void apply(float delta_time) {
+        if (s_) return s_->stage(APBF_STAGE_VELOCITY, delta_time);
+        const float g[3] = {acc_.x, acc_.y, acc_.z};
+        apbf_particle_lists p = f_->lists();
+        check(apbf_op_velocity_handling(context::get().handle(), &p, &last_delta_time_, delta_time, g, list_stream()),
+              "velocity_handling");
+    }
+    float last_delta_time() const { return last_delta_time_; }
+    velocity_handling& set_last_delta_time(float t) {
+        last_delta_time_ = t;
+        return *this;
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    vec3 acc_{0, 0, 0};
+    float last_delta_time_ = 1.0f;  // velocity_handling.h:17
 };
 
-/* particle_transfer.h: set_data, apply(dt) */
+/* particle_transfer.h:7-16 */
 class particle_transfer {
 public:
     particle_transfer& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
         return *this;
     }
-    void apply(float delta_time) { s_->stage(APBF_STAGE_PARTICLE_TRANSFER, delta_time); }
+    particle_transfer& set_data(fluid* f, transfers* t) {
+        f_ = f;
+        t_ = t;
+        s_ = nullptr;
+        return *this;
+    }
+    void apply(float delta_time) {
+        if (s_) return s_->stage(APBF_STAGE_PARTICLE_TRANSFER, delta_time);
+        apbf_particle_lists p = f_->lists();
+        apbf_transfer_lists t = t_->lists();
+        check(apbf_op_particle_transfer(context::get().handle(), &p, &t, delta_time, list_stream()), "particle_transfer");
+        f_->sync_lengths();
+        t_->sync_lengths();
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    transfers* t_ = nullptr;
 };
 
-/* neighborhood_green.h:9-24 */
+/* neighborhood_green.h:8-24. Bound to lists, the range of each particle is
+ * its kernel width (the reference passes the fluid's kernel_width list,
+ * pool.cpp:46) and the particles' transfer owner follows the reordering. */
 class neighborhood_green {
 public:
     neighborhood_green& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
+        return *this;
+    }
+    neighborhood_green& set_data(fluid* particles, neighbors* nb) {
+        f_ = particles;
+        nb_ = nb;
+        s_ = nullptr;
         return *this;
     }
     neighborhood_green& set_range_scale(float scale) {
-        check(apbf_scene_set_range_scale(s_->handle(), scale), "set_range_scale");
+        scale_ = scale;
+        if (s_) check(apbf_scene_set_range_scale(s_->handle(), scale), "set_range_scale");
         return *this;
     }
     neighborhood_green& set_position_range(const vec3& mn, const vec3& mx, uint32_t resolution_log2) {
-        const float a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
-        check(apbf_scene_set_position_range(s_->handle(), a, b, resolution_log2), "set_position_range");
+        mn_ = mn;
+        mx_ = mx;
+        res_ = resolution_log2;
+        ranged_ = true;
+        if (s_) {
+            const float a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
+            check(apbf_scene_set_position_range(s_->handle(), a, b, resolution_log2), "set_position_range");
+        }
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_NEIGHBORHOOD, 0.0f); }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_NEIGHBORHOOD, 0.0f);
+        const apbf_scene_params& prm = context::get().params();
+        const float a[3] = {ranged_ ? mn_.x : prm.hash_min[0], ranged_ ? mn_.y : prm.hash_min[1],
+                            ranged_ ? mn_.z : prm.hash_min[2]};
+        const float b[3] = {ranged_ ? mx_.x : prm.hash_max[0], ranged_ ? mx_.y : prm.hash_max[1],
+                            ranged_ ? mx_.z : prm.hash_max[2]};
+        apbf_particle_lists p = f_->lists();
+        apbf_neighbor_lists n = nb_->lists();
+        transfers* owner = transfers_of(f_);
+        apbf_transfer_lists t{};
+        if (owner) t = owner->lists();
+        check(apbf_op_neighborhood(context::get().handle(), &p, owner ? &t : nullptr, &n, method_, scale_, a, b,
+                                   ranged_ ? res_ : prm.hash_res_log2, list_stream()),
+              "neighborhood");
+    }
 
-private:
+protected:
+    int32_t method_ = 0;
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    neighbors* nb_ = nullptr;
+    float scale_ = 1.0f;
+    vec3 mn_{0, 0, 0}, mx_{1, 1, 1};
+    uint32_t res_ = 4;
+    bool ranged_ = false;
 };
 
 /* neighborhood_binary_search.h / neighborhood_brute_force.h: the scene's
@@ -162,6 +276,11 @@ public:
         check(apbf_scene_set_neighborhood_method(s->handle(), 1), "set_neighborhood_method");
         return *this;
     }
+    neighborhood_binary_search& set_data(fluid* particles, neighbors* nb) {
+        neighborhood_green::set_data(particles, nb);
+        method_ = 1;
+        return *this;
+    }
 };
 class neighborhood_brute_force : public neighborhood_green {
 public:
@@ -170,32 +289,68 @@ public:
         check(apbf_scene_set_neighborhood_method(s->handle(), 2), "set_neighborhood_method");
         return *this;
     }
+    neighborhood_brute_force& set_data(fluid* particles, neighbors* nb) {
+        neighborhood_green::set_data(particles, nb);
+        method_ = 2;
+        return *this;
+    }
 };
 
-/* spread_kernel_width.h:10-11 */
+/* spread_kernel_width.h:7-15 */
 class spread_kernel_width {
 public:
     spread_kernel_width& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_SPREAD, 0.0f); }
+    spread_kernel_width& set_data(fluid* f, neighbors* nb) {
+        f_ = f;
+        nb_ = nb;
+        s_ = nullptr;
+        return *this;
+    }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_SPREAD, 0.0f);
+        apbf_particle_lists p = f_->lists();
+        apbf_neighbor_lists n = nb_->lists();
+        check(apbf_op_spread_kernel_width(context::get().handle(), &p, &n, list_stream()), "spread_kernel_width");
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    neighbors* nb_ = nullptr;
 };
 
-/* box_collision.h */
+/* box_collision.h:7-16: boxes as [min.xyz, pad] / [max.xyz, pad] per box
+ * (the reference's gpu_list<16> box lists, host side here). */
 class box_collision {
 public:
     box_collision& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_BOX, 0.0f); }
+    box_collision& set_data(fluid* particles, const std::vector<float>* box_min4, const std::vector<float>* box_max4) {
+        f_ = particles;
+        mn_ = box_min4;
+        mx_ = box_max4;
+        s_ = nullptr;
+        return *this;
+    }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_BOX, 0.0f);
+        apbf_particle_lists p = f_->lists();
+        const uint32_t nb = (uint32_t)(mn_->size() / 4);
+        check(apbf_op_box_collision(context::get().handle(), &p, nb, mn_->data(), mx_->data(), list_stream()),
+              "box_collision");
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    const std::vector<float>*mn_ = nullptr, *mx_ = nullptr;
 };
 
 /* sphere_collision.h: set_sphere(center, radius, hollow), apply() */
@@ -203,43 +358,97 @@ class sphere_collision {
 public:
     sphere_collision& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
+        return *this;
+    }
+    sphere_collision& set_data(fluid* particles) {
+        f_ = particles;
+        s_ = nullptr;
         return *this;
     }
     sphere_collision& set_sphere(const vec3& center, float radius, bool hollow) {
-        const float c[3] = {center.x, center.y, center.z};
-        check(apbf_scene_set_sphere(s_->handle(), c, radius, hollow ? 1 : 0, 1), "set_sphere");
+        c_ = center;
+        r_ = radius;
+        hollow_ = hollow;
+        if (s_) {
+            const float c[3] = {center.x, center.y, center.z};
+            check(apbf_scene_set_sphere(s_->handle(), c, radius, hollow ? 1 : 0, 1), "set_sphere");
+        }
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_SPHERE, 0.0f); }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_SPHERE, 0.0f);
+        apbf_particle_lists p = f_->lists();
+        const float c[3] = {c_.x, c_.y, c_.z};
+        check(apbf_op_sphere_collision(context::get().handle(), &p, c, r_, hollow_ ? 1 : 0, list_stream()),
+              "sphere_collision");
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    vec3 c_{0, 0, 0};
+    float r_ = 0.0f;
+    bool hollow_ = true;
 };
 
-/* incompressibility.h:9-18: one solver iteration per apply() */
+/* incompressibility.h:7-18: one solver iteration per apply() */
 class incompressibility {
 public:
     incompressibility& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_INCOMPRESSIBILITY, 0.0f); }
+    incompressibility& set_data(fluid* f, neighbors* nb) {
+        f_ = f;
+        nb_ = nb;
+        s_ = nullptr;
+        return *this;
+    }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_INCOMPRESSIBILITY, 0.0f);
+        apbf_particle_lists p = f_->lists();
+        apbf_neighbor_lists n = nb_->lists();
+        check(apbf_op_incompressibility(context::get().handle(), &p, &n, list_stream()), "incompressibility");
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    neighbors* nb_ = nullptr;
 };
 
-/* update_transfers.h:11-12 */
+/* update_transfers.h:7-16 */
 class update_transfers {
 public:
     update_transfers& set_data(scene* s) {
         s_ = s;
+        f_ = nullptr;
         return *this;
     }
-    void apply() { s_->stage(APBF_STAGE_UPDATE_TRANSFERS, 0.0f); }
+    update_transfers& set_data(fluid* f, neighbors* nb, transfers* t) {
+        f_ = f;
+        nb_ = nb;
+        t_ = t;
+        s_ = nullptr;
+        return *this;
+    }
+    void apply() {
+        if (s_) return s_->stage(APBF_STAGE_UPDATE_TRANSFERS, 0.0f);
+        apbf_particle_lists p = f_->lists();
+        apbf_transfer_lists t = t_->lists();
+        apbf_neighbor_lists n = nb_->lists();
+        check(apbf_op_update_transfers(context::get().handle(), &p, &t, &n, list_stream()), "update_transfers");
+        f_->sync_lengths();
+        t_->sync_lengths();
+    }
 
 private:
     scene* s_ = nullptr;
+    fluid* f_ = nullptr;
+    neighbors* nb_ = nullptr;
+    transfers* t_ = nullptr;
 };
 
 /* time_machine.h:9-143 for one scene: keyframes every keyframe_interval
@@ -292,6 +501,9 @@ public:
     }
     uint32_t history_index() const { return std::min(current_, current_ - oldest_history_keyframe_step()); }
     uint32_t sparse_history_index() const { return history_index() / interval_; }
+    /* the time step recorded for the current step (the per-substep history
+     * value that stepping through the past restores) */
+    float current_dt() const { return dts_[history_index()]; }
 
     bool step_forward(float& dt) {
         if (!enabled_) return false;
@@ -373,6 +585,11 @@ private:
 /* uint_to_float_with_indexed_lower_bound (pool.cpp:77-80): kernel width from
  * the boundary distance. */
 inline void kernel_width_from_boundary_distance(scene& s) { s.stage(APBF_STAGE_KERNEL_WIDTH_BD, 0.0f); }
+inline void kernel_width_from_boundary_distance(fluid& f) {
+    apbf_particle_lists p = f.lists();
+    check(apbf_op_kernel_width_from_boundary_distance(context::get().handle(), &p, list_stream()),
+          "kernel_width_from_boundary_distance");
+}
 
 }  // namespace apbf_pbd
 

@@ -156,6 +156,117 @@ static int run_case(const char* name, bool basic) {
     return 0;
 }
 
+// pool::update (pool.cpp:67-106) through subsystems bound to lists
+// (pool.cpp:27-46: set_data(&mParticles), set_data(&mFluid, &mNeighborsFluid),
+// set_data(&mFluid, &mNeighborsFluid, &mTransfers), ...), each apply() running
+// its stage operator on the lists; the lists must equal scene::update's state
+// after every substep, bit for bit.
+static int run_list_case(const char* name, bool basic) {
+    const state st = make_state(10, 0.5f, !basic);
+    const apbf_scene_params p = make_params(st, basic, 16.0f);
+    const float dt = 1.0f / 120.0f;
+    scene whole(p);
+    whole.set_state(st);
+    context ctx(p);
+    fluid particles(p.cap_particles);
+    transfers trans(p.cap_transfers);
+    neighbors nbs(p.cap_particles, p.cap_pairs);
+    share_hidden_data(trans, particles);  // pool.cpp:18-19
+    particles.upload(st);
+    trans.upload(st);
+    std::vector<float> bmin(4 * p.n_boxes), bmax(4 * p.n_boxes);
+    for (uint32_t b = 0; b < p.n_boxes; ++b)
+        for (int d = 0; d < 4; ++d) {
+            bmin[4 * b + d] = p.box_min[b][d];
+            bmax[4 * b + d] = p.box_max[b][d];
+        }
+    velocity_handling velocity;
+    particle_transfer transfer;
+    neighborhood_green neighborhood;
+    spread_kernel_width spread;
+    box_collision box;
+    incompressibility solve;
+    update_transfers transfers_update;
+    velocity.set_data(&particles).set_acceleration(vec3{0.0f, -9.81f, 0.0f}).set_last_delta_time(st.last_delta_time);
+    transfer.set_data(&particles, &trans);
+    neighborhood.set_data(&particles, &nbs).set_position_range(vec3{0, 0, 0}, vec3{16, 16, 16}, 4);
+    spread.set_data(&particles, &nbs);
+    box.set_data(&particles, &bmin, &bmax);
+    solve.set_data(&particles, &nbs);
+    transfers_update.set_data(&particles, &nbs, &trans);
+    for (int step = 0; step < 6; ++step) {
+        whole.update(dt);
+        velocity.apply(dt);
+        if (!basic) transfer.apply(dt);
+        neighborhood.set_range_scale(basic ? 1.0f : 1.5f).apply();
+        if (!basic) spread.apply();
+        for (int it = 0; it < p.solver_iterations; ++it) {
+            box.apply();
+            solve.apply();
+        }
+        if (!basic) transfers_update.apply();
+        state got;
+        particles.download(got);
+        trans.download(got);
+        const state want = whole.get_state();
+        if (!same_state(want, got) || !same(want.t_source, got.t_source) || !same(want.t_target, got.t_target) ||
+            !same(want.transferring, got.transferring) || !same(want.pos_backup, got.pos_backup)) {
+            std::printf("%s: substep %d differs\n", name, step);
+            return 1;
+        }
+    }
+    // the neighbour list in the reference's uvec2 layout
+    gpu_list<8> pairs = nbs.pairs();
+    const std::vector<uint32_t> pr = pairs.read<uint32_t>();
+    bool ordered = !pr.empty();
+    for (size_t k = 2; k < pr.size(); k += 2) ordered = ordered && pr[k - 2] <= pr[k];
+    if (!ordered) {
+        std::printf("%s: pair list not in row order\n", name);
+        return 1;
+    }
+    std::printf("%s: ok, %u particles, %zu pairs\n", name, whole.get_state().size(), pr.size() / 2);
+    return 0;
+}
+
+// gpu_list (gpu_list.h) on the reference's list vectors: concatenation
+// (test.cpp:47-57), append of an empty list (test.cpp:75-89), apply_edit
+// (test.cpp:91-105), sort.
+static int run_gpu_list_case() {
+    bool pass = true;
+    {
+        std::vector<uint32_t> a = {3u, 64u, 12683u, 4294967295u}, b = {432587u, 0u, 5436u};
+        auto la = gpu_list<4>::from_host(a, a.size() + b.size());
+        auto lb = gpu_list<4>::from_host(b);
+        la += lb;
+        std::vector<uint32_t> want = a;
+        want.insert(want.end(), b.begin(), b.end());
+        pass = la.read<uint32_t>() == want && pass;
+    }
+    {
+        std::vector<uint32_t> a = {3u, 64u, 12683u, 4294967295u};
+        auto la = gpu_list<4>::from_host(a, a.size() + 5);
+        auto lb = gpu_list<4>::from_host(std::vector<uint32_t>{}, a.size() + 5);
+        la += lb;
+        pass = la.read<uint32_t>() == a && pass;
+    }
+    {
+        std::vector<uint32_t> v = {77u, 3u, 9999u, 4294967295u, 0u}, e = {1u, 3u, 1u, 4u};
+        auto l = gpu_list<4>::from_host(v);
+        l.apply_edit(gpu_list<4>::from_host(e));
+        pass = l.read<uint32_t>() == std::vector<uint32_t>({3u, 4294967295u, 3u, 0u}) && pass;
+    }
+    {
+        std::vector<uint32_t> v = {5u, 1u, 4u, 1u, 3u};
+        auto l = gpu_list<4>::from_host(v);
+        auto shared = l;  // lazy copy: sorting l leaves the copy as it was
+        l.sort();
+        pass = l.read<uint32_t>() == std::vector<uint32_t>({1u, 1u, 3u, 4u, 5u}) && pass;
+        pass = shared.read<uint32_t>() == v && pass;
+    }
+    std::printf("gpu_list: %s\n", pass ? "ok" : "FAILED");
+    return pass ? 0 : 1;
+}
+
 // spherical_pool::update (spherical_pool.cpp:67-106): the sphere collider in
 // place of the boxes, here with the brute-force neighbour search.
 static int run_sphere_case() {
@@ -245,6 +356,48 @@ static int run_time_machine_case() {
     return 0;
 }
 
+// The reference's own time-machine test (source/test.cpp:585-619): at most
+// two keyframes, one every 3 steps; three steps in the present, jump_back,
+// then step_forward four times with the expected returns. The reference's
+// sparse list is the scene, its per-step value the recorded time step.
+static int run_time_machine_reference_sequence() {
+    const state st = make_state(6, 0.5f, false);
+    const apbf_scene_params p = make_params(st, true, 16.0f);
+    scene s(p);
+    s.set_state(st);
+    time_machine tm(s);
+    tm.set_max_keyframes(2).set_keyframe_interval(3).enable(0.0f);
+    std::vector<state> rec(5);
+    rec[0] = s.get_state();
+    auto dt_of = [](int k) { return (1.0f + 0.1f * (float)k) / 120.0f; };
+    bool pass = true;
+    for (int k = 1; k <= 3; ++k) {
+        float dt = dt_of(k);
+        pass = !tm.step_forward(dt) && pass;
+        s.update(dt);
+        tm.save_state(dt);
+        rec[k] = s.get_state();
+    }
+    pass = tm.jump_back() && pass;
+    pass = tm.current_dt() == 0.0f && same_state(s.get_state(), rec[0]) && pass;
+    for (int k = 1; k <= 2; ++k) {
+        float dt = 99.0f;
+        pass = !tm.step_forward(dt) && pass;
+        pass = dt == dt_of(k) && tm.current_dt() == dt_of(k) && pass;
+        s.update(dt);
+        tm.save_state(dt);
+        pass = same_state(s.get_state(), rec[k]) && pass;
+    }
+    float dt = 99.0f;
+    pass = tm.step_forward(dt) && pass;  // keyframe 3 loaded
+    tm.save_state(dt);
+    pass = tm.current_dt() == dt_of(3) && same_state(s.get_state(), rec[3]) && pass;
+    dt = dt_of(4);
+    pass = !tm.step_forward(dt) && !tm.in_past() && dt == dt_of(4) && pass;  // the present again
+    std::printf("time machine reference sequence: %s\n", pass ? "ok" : "FAILED");
+    return pass ? 0 : 1;
+}
+
 int main() {
     if (apbf_device_count() <= 0) {
         std::printf("no device\n");
@@ -252,7 +405,8 @@ int main() {
     }
     try {
         return run_case("basic pbf", true) | run_case("adaptive", false) | run_sphere_case() |
-               run_time_machine_case();
+               run_time_machine_case() | run_time_machine_reference_sequence() | run_list_case("lists basic pbf", true) |
+               run_list_case("lists adaptive", false) | run_gpu_list_case();
     } catch (const error& e) {
         std::printf("error: %s\n", e.what());
         return 1;

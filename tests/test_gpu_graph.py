@@ -46,8 +46,9 @@ def test_batch_graph_matches_individual_scenes(device):
     singles = [_scene(p, st) for p, st in cases]
     batched = [_scene(p, st) for p, st in cases]
     dt = cases[0][0]["dt"]
-    for s in singles + batched:
-        s.update(dt)
+    for s in singles + batched:  # queued on the null stream, not waited for:
+        for _ in range(3):       # the batch's streams must start after this work
+            s.update(dt)
     batch = Batch(batched)
     batch.update(dt)  # eager batch substep
     for s in singles:

@@ -90,3 +90,40 @@ def test_snapshot_restore_is_exact(device):
         _same(_state(s), a[k], f"substep {k} after the restore")
     tm.close()
     s.close()
+
+
+def test_time_machine_reference_sequence(device):
+    """The reference's own time-machine test (source/test.cpp:585-619): two
+    keyframes at most, one every 3 steps; three steps in the present, then
+    jump_back and four step_forward calls with the expected returns. The
+    reference's sparse list is the scene here, and its per-step value `i` is
+    the recorded time step."""
+    from apbf_amd.scene import Scene
+    from apbf_amd.time_machine import TimeMachine
+    p, st = cfg.dam_break(n_side=6)
+    s = Scene(p)
+    s.set_state(st)
+    tm = TimeMachine(s, max_keyframes=2, keyframe_interval=3).enable(0.0)
+    rec = {0: _state(s)}
+    dts = {k: p["dt"] * (1.0 + 0.1 * k) for k in range(1, 5)}
+    for k in (1, 2, 3):
+        assert tm.step_forward(dts[k]) is not None  # no keyframe load in the present
+        s.update(dts[k])
+        tm.save_state(dts[k])
+        rec[k] = _state(s)
+    assert tm.jump_back()
+    assert tm.current_step == 0 and tm.current_dt() == 0.0
+    _same(_state(s), rec[0], "jump_back to keyframe 0")
+    for k in (1, 2):
+        dt = tm.step_forward(99.0)
+        assert dt == dts[k] and tm.current_dt() == dts[k]
+        s.update(dt)
+        tm.save_state(dt)
+        _same(_state(s), rec[k], f"past step {k}")
+    assert tm.step_forward(99.0) is None  # lands on keyframe 3: loaded
+    assert tm.current_dt() == dts[3]
+    tm.save_state(dts[3])
+    _same(_state(s), rec[3], "keyframe 3 loaded")
+    assert tm.step_forward(dts[4]) == dts[4] and not tm.in_past()  # the present again
+    tm.save_state(dts[4])
+    tm.close()

@@ -29,3 +29,6 @@ def test_host_api_subsystems_match_update(device):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "basic pbf: ok" in r.stdout and "adaptive: ok" in r.stdout and "spherical pool: ok" in r.stdout
     assert "time machine: ok" in r.stdout
+    assert "time machine reference sequence: ok" in r.stdout  # source/test.cpp:585-619
+    assert "lists basic pbf: ok" in r.stdout and "lists adaptive: ok" in r.stdout  # list-bound subsystems
+    assert "gpu_list: ok" in r.stdout
