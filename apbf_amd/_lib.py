@@ -115,6 +115,7 @@ PROTOTYPES = {
     "apbf_neighborhood_green_workspace_size": (_sz, [_u32]),
     "apbf_prefix_sum": (_i32, [_vp, _vp, _u32, _vp, _sz, _vp]),
     "apbf_radix_sort": (_i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "apbf_radix_sort_noalloc": (_i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "apbf_calculate_position_hash": (_i32, [_vp, _vp, _u32, _pf, _pf, _u32, _vp, _vp]),
     "apbf_calculate_position_code": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
     "apbf_find_value_ranges": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _vp]),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libapbf_hip.so")
 SOURCES = ["primitives.hip", "plain_kernels.hip", "rows.hip", "walk.hip", "adaptive.hip", "scene.hip",
            "capi_lists.hip", "mlp.hip", "graph.hip", "bsearch.hip", "particle_info.hip",
-           "snapshot.hip", "lists.hip"]
+           "snapshot.hip", "noalloc.hip", "lists.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -204,6 +204,21 @@ apbf_status apbf_radix_sort(const uint32_t* keys, const uint32_t* values, const 
                             uint32_t* out_values, void* workspace, size_t workspace_bytes,
                             apbf_stream stream);
 
+/* The sort of apbf_radix_sort without a workspace argument and without any
+ * allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its workspace is a fixed device array
+ * inside the library, one per GPU, held until the process ends
+ * (apbf_radix_sort_workspace_size(APBF_RADIX_SORT_NOALLOC_MAX) bytes, about
+ * 8.5 MiB). capacity above APBF_RADIX_SORT_NOALLOC_MAX returns
+ * APBF_ERR_CAPACITY; *d_len must not exceed capacity. Calls that overlap in
+ * time on one GPU (other streams or threads) share that array and race;
+ * calls ordered on one stream are safe. Results equal apbf_radix_sort's. */
+#define APBF_RADIX_SORT_NOALLOC_MAX (1u << 20)
+apbf_status apbf_radix_sort_noalloc(const uint32_t* keys, const uint32_t* values, const uint32_t* d_len,
+                                    uint32_t capacity, uint32_t upper_bound, uint32_t* out_keys,
+                                    uint32_t* out_values, apbf_stream stream);
+
 /* z-order hash of each position in the grid [min, max] with 2^res cells per
  * axis (shader_provider::calculate_position_hash, shader_provider.cpp:805;
  * calculate_position_hash.comp:23-46). */
