@@ -107,7 +107,14 @@ PROTOTYPES = {
     "apbf_scene_error_stats": (_i32, [_vp, _pd, _pd, _pd, _pd, _pu, _i32]),
     "apbf_scene_iteration_errors": (_i32, [_vp, _vp, _u32, _i32]),
     "apbf_scene_particle_substeps": (_i32, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _i32]),
+    "apbf_scene_set_solver_tolerance": (_i32, [_vp, _f32, _i32]),
+    "apbf_scene_get_warm_lambdas": (_i32, [_vp, _vp]),
+    "apbf_scene_solver_stats": (_i32, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       _i32]),
     "apbf_scene_time_kernel": (_i32, [_vp, _i32, _u32]),
+    "apbf_scene_time_kernels": (_i32, [_vp, _u32, _u32]),
+    "apbf_scene_kernel_time_of": (_i32, [_vp, _i32, _pd, _pu]),
+    "apbf_scene_substep_totals": (_i32, [_vp, _vp, _u32, _i32]),
     "apbf_scene_diagnostics": (_i32, [_vp, _vp, _u32]),
     "apbf_scene_kernel_time": (_i32, [_vp, _pd, _pu]),
     "apbf_prefix_sum_workspace_size": (_sz, [_u32]),

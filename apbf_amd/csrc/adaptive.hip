@@ -139,8 +139,9 @@ __global__ __launch_bounds__(BLOCK) void k_compact_transfers(const uint32_t* __r
     }
 }
 
-__global__ void k_commit_compaction(uint32_t* __restrict__ counts) {
+__global__ void k_commit_compaction(uint32_t* __restrict__ counts, unsigned long long* __restrict__ totals) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    totals[4] += (unsigned long long)(counts[C_N] - counts[C_KEEP]);  // particles deleted
     counts[C_N] = counts[C_KEEP];
     counts[C_TRANSFERS] = counts[C_TKEEP];
 }
@@ -487,10 +488,12 @@ __global__ __launch_bounds__(BLOCK) void k_split_finish(const uint32_t* __restri
         tttl[tlen + k] = split_ttl;
     }
 }
-__global__ void k_split_counts(uint32_t* __restrict__ counts, uint32_t cap, uint32_t tcap) {
+__global__ void k_split_counts(uint32_t* __restrict__ counts, uint32_t cap, uint32_t tcap,
+                               unsigned long long* __restrict__ totals) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t ns = counts[C_SPLITS], n = counts[C_N], tlen = counts[C_TRANSFERS];
     uint32_t keep = min(min(ns, tcap - tlen), cap - n);
+    totals[3] += keep;  // particles created
     counts[C_SPLITS] = keep;
     counts[C_N] = n + keep;
     counts[C_TRANSFERS] = tlen + keep;
@@ -544,7 +547,7 @@ void Scene::particle_transfer(float dt) {
     tcur = 1 - tcur;
     LAUNCH(k_compact_particles, stride_grid(cap), del_p, u3, counts, arrays(cur), arrays(1 - cur));
     cur = 1 - cur;
-    LAUNCH1(k_commit_compaction, counts);
+    LAUNCH1(k_commit_compaction, counts, particle_substeps);
     pairs_valid = false;
 }
 
@@ -649,7 +652,7 @@ void Scene::update_transfers(bool with_density_error) {
     if (!P.s.mSplit) return;  // update_transfers.cpp:59
     LAUNCH(k_split_finish, stride_grid(cap), u4, counts, cap, tcap, arrays(cur), t_source[tcur],
            t_target[tcur], t_time_left[tcur], -P.split_duration);
-    LAUNCH1(k_split_counts, counts, cap, tcap);
+    LAUNCH1(k_split_counts, counts, cap, tcap, particle_substeps);
     pairs_valid = false;
 }
 

@@ -83,7 +83,7 @@ extern "C" {
 apbf_status apbf_scene_capture(void* scene, float dt) {
     if (!scene) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
-    if (s->timed_kernel >= 0) return APBF_ERR_INVALID_ARGUMENT;  // event timing is not captured
+    if (s->timed_mask) return APBF_ERR_INVALID_ARGUMENT;  // event timing is not captured
     apbf_status st = ensure_stream(s);
     if (st != APBF_OK) return st;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
@@ -216,7 +216,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     apbf_status st = apbf_snapshot_create(scene, &snap);
     if (st != APBF_OK) return st;
     const size_t err_bytes = 6 * sizeof(double), iter_bytes = MAX_ITER_RECORD * sizeof(double);
-    const size_t ps_bytes = 2 * sizeof(unsigned long long);
+    const size_t ps_bytes = 5 * sizeof(unsigned long long);
     char* stats = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipMalloc(&stats, err_bytes + iter_bytes + ps_bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
@@ -232,10 +232,11 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
         cp(s->iter_err, stats + err_bytes, iter_bytes);
         cp(s->particle_substeps, stats + err_bytes + iter_bytes, ps_bytes);
     };
-    const int timed_kernel = s->timed_kernel, walk0 = s->walk_lanes, solve0 = s->solve_lanes;
+    const uint32_t timed_mask = s->timed_mask;
+    const int walk0 = s->walk_lanes, solve0 = s->solve_lanes;
     const bool auto0 = s->walk_auto;
     s->walk_auto = false;
-    s->timed_kernel = -1;
+    s->timed_mask = 0;
     if (st == APBF_OK) copy_stats(true);
     const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
     float trial_ms[3][3];
@@ -276,7 +277,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)apbf_snapshot_destroy(snap);
-    s->timed_kernel = timed_kernel;
+    s->timed_mask = timed_mask;
     s->walk_lanes = st == APBF_OK ? bw : walk0;
     s->walk_auto = st == APBF_OK ? false : auto0;
     s->solve_lanes = st == APBF_OK ? bs : solve0;

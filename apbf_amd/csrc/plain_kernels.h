@@ -72,7 +72,11 @@ void launch_features_rows(const int4* pos, const float4* vel, const float* inv_m
                           const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
                           uint32_t pcap, const SolverArgs& a, float dt, float* feat, hipStream_t s);
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
-                        const BoxSet& boxes, hipStream_t s);
+                        const BoxSet& boxes, hipStream_t s, const uint32_t* action = nullptr,
+                        const int4* saved = nullptr);
+void launch_solve_begin(uint32_t* counts, double* tol_err, uint32_t slots, hipStream_t s);
+void launch_solve_decide(uint32_t* counts, const double* tol_err, uint32_t k, float tol, bool warm, hipStream_t s);
+void launch_solve_stats(const uint32_t* counts, unsigned long long* stats, uint32_t fixed_iterations, hipStream_t s);
 uint32_t density_rows_blocks(uint32_t cap);
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                          const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,

@@ -237,6 +237,7 @@ __global__ void k_count_particle_substeps(const uint32_t* __restrict__ counts,
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         acc[0] += (unsigned long long)counts[C_ROWS];
         acc[1] += (unsigned long long)counts[C_PAIRS];
+        acc[2] += (unsigned long long)counts[C_PAIRS_RAW];
     }
 }
 

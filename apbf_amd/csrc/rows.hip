@@ -178,6 +178,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const float* __restrict__ kw, const float4* __restrict__ gc, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, uint32_t pcap, SolverArgs a,
     float* __restrict__ boundariness, float* __restrict__ lambda, int* __restrict__ acc) {
+    if (a.skip && *a.skip) return;  // the same word for every block
     __shared__ double s_err;
     __shared__ uint32_t s_key[AGG_SLOTS];
     __shared__ int32_t s_sum[3 * AGG_SLOTS];
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             lam /= a.lambda_scale_base / ird * im;
             if (a.lam_override) lam = active ? a.lam_override[i] : 0.0f;
             if (active && gl == 0) {
-                if (a.it_err) atomicAdd(&s_err, (double)fmax_g(0.0f, densf * ird - 1.0f));
+                if (a.it_err || a.tol_sum) atomicAdd(&s_err, (double)fmax_g(0.0f, densf * ird - 1.0f));
                 if (a.lam_acc) a.lam_acc[i] += fminf(lam, 0.0f);
                 lambda[i] = lam;
                 if (!(lam >= 0.0f)) {  // own shift; a NaN lambda shifts by ivec3(NaN) = 0
@@ -398,9 +399,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
         tab.flush();
         __syncthreads();
     }  // base
-    if (a.it_err) {
+    if (a.it_err || a.tol_sum) {
         __syncthreads();
-        if (threadIdx.x == 0 && s_err != 0.0) atomicAdd(a.it_err, s_err / (double)counts[C_N]);
+        if (threadIdx.x == 0 && s_err != 0.0) {
+            if (a.it_err) atomicAdd(a.it_err, s_err / (double)counts[C_N]);
+            if (a.tol_sum) atomicAdd(a.tol_sum, s_err);
+        }
     }
 }
 
@@ -539,16 +543,28 @@ __global__ __launch_bounds__(BLOCK) void k_features_rows(const int4* __restrict_
 // Applies the accumulated shifts, clears the accumulator, and (when `boxes`
 // holds colliders) runs the box and sphere collision of the next solver
 // iteration.
+// With `action` set, the iteration's decision (SolveAction) selects between
+// adding the shifts, leaving everything as it is, and putting back the
+// positions `saved` before the warm-start shift (then the accumulator is
+// cleared and the next iteration's colliders run as after a shift).
 __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, int4* __restrict__ acc,
                                                        const float* __restrict__ radius,
-                                                       const uint32_t* __restrict__ counts, BoxSet boxes) {
+                                                       const uint32_t* __restrict__ counts, BoxSet boxes,
+                                                       const uint32_t* __restrict__ action,
+                                                       const int4* __restrict__ saved) {
+    const uint32_t act = action ? *action : (uint32_t)SOLVE_APPLY;
+    if (act == SOLVE_SKIP) return;
     const uint32_t n = counts[C_N];
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         int4 p = pos[j];
         const int4 d = acc[j];
-        p.x = iadd(p.x, d.x);
-        p.y = iadd(p.y, d.y);
-        p.z = iadd(p.z, d.z);
+        if (act == SOLVE_RESTORE) {
+            p = saved[j];
+        } else {
+            p.x = iadd(p.x, d.x);
+            p.y = iadd(p.y, d.y);
+            p.z = iadd(p.z, d.z);
+        }
         acc[j] = make_int4(0, 0, 0, 0);
         if (boxes.n) p = box_collide(j, p, radius[j], boxes);
         if (boxes.sphere) p = sphere_collide(j, p, radius[j], boxes);
@@ -961,8 +977,70 @@ void launch_features_rows(const int4* pos, const float4* vel, const float* inv_m
 }
 
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
-                        const BoxSet& boxes, hipStream_t s) {
-    hipLaunchKernelGGL(k_solve_apply, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, pos, acc, radius, counts, boxes);
+                        const BoxSet& boxes, hipStream_t s, const uint32_t* action, const int4* saved) {
+    hipLaunchKernelGGL(k_solve_apply, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, pos, acc, radius, counts, boxes,
+                       action, saved);
+}
+
+// Start of a solve with a tolerance or a warm start: nothing decided yet, the
+// error sums of this substep's passes cleared (slot 0: the warm-start pass,
+// slot 1 + k: plain iteration k).
+__global__ void k_solve_begin(uint32_t* __restrict__ counts, double* __restrict__ tol_err, uint32_t slots) {
+    for (uint32_t k = threadIdx.x; k < slots; k += blockDim.x) tol_err[k] = 0.0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counts[C_SOLVE_DONE] = 0u;
+        counts[C_SOLVE_ACTION] = SOLVE_APPLY;
+        counts[C_SOLVE_ITERS] = 0u;
+        counts[C_WARM_FALLBACK] = 0u;
+    }
+}
+// After the row pass of plain iteration k: its mean density error e_k decides
+// the apply pass. With a warm start, iteration 0 first judges the warm shift:
+// when the error after it exceeds the error before it (e_0 of the warm pass),
+// the positions from before the warm shift are put back. Then, with a
+// tolerance, e_k <= tol ends the solve (this and later iterations apply
+// nothing).
+__global__ void k_solve_decide(uint32_t* __restrict__ counts, const double* __restrict__ tol_err, uint32_t k,
+                               float tol, int warm) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (counts[C_SOLVE_DONE]) {
+        counts[C_SOLVE_ACTION] = SOLVE_SKIP;
+        return;
+    }
+    const uint32_t n = counts[C_N];
+    const double e = n ? tol_err[1 + k] / (double)n : 0.0;
+    if (warm && k == 0) {
+        const double e0 = n ? tol_err[0] / (double)n : 0.0;
+        if (e > e0) {
+            counts[C_SOLVE_ACTION] = SOLVE_RESTORE;
+            counts[C_WARM_FALLBACK] = 1u;
+            return;
+        }
+    }
+    if (tol > 0.0f && e <= (double)tol) {
+        counts[C_SOLVE_DONE] = 1u;
+        counts[C_SOLVE_ACTION] = SOLVE_SKIP;
+        return;
+    }
+    counts[C_SOLVE_ACTION] = SOLVE_APPLY;
+    counts[C_SOLVE_ITERS] += 1u;
+}
+// Per substep: stats[0] += iterations applied, stats[1] += 1, stats[2] += fallbacks.
+__global__ void k_solve_stats(const uint32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
+                              uint32_t fixed_iterations) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    stats[0] += fixed_iterations ? fixed_iterations : counts[C_SOLVE_ITERS];
+    stats[1] += 1ull;
+    stats[2] += counts[C_WARM_FALLBACK];
+}
+void launch_solve_begin(uint32_t* counts, double* tol_err, uint32_t slots, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, s, counts, tol_err, slots);
+}
+void launch_solve_decide(uint32_t* counts, const double* tol_err, uint32_t k, float tol, bool warm, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_decide, dim3(1), dim3(64), 0, s, counts, tol_err, k, tol, warm ? 1 : 0);
+}
+void launch_solve_stats(const uint32_t* counts, unsigned long long* stats, uint32_t fixed_iterations, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_stats, dim3(1), dim3(64), 0, s, counts, stats, fixed_iterations);
 }
 uint32_t density_rows_blocks(uint32_t cap) { return row_grid(cap) < 2048u ? row_grid(cap) : 2048u; }
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,

@@ -35,7 +35,10 @@ struct Scene {
     bool pairs_valid = false;    // CSR rows refer to the current particle slots
     bool measure_error = false;  // accumulate the density error after every solver loop
     double* err_stats = nullptr; // [sum of means, max, last mean, last max, sum of abs means, last abs mean]
-    unsigned long long* particle_substeps = nullptr;  // sum over substeps of the solver's particle count
+    // sums over substeps: [0] the solver's particle count, [1] its neighbour
+    // pairs, [2] the neighbour walk's pairs before the spreading filter,
+    // [3] particles created by splits, [4] particles deleted by finished merges
+    unsigned long long* particle_substeps = nullptr;
     double* iter_err = nullptr;  // per solver iteration: sum over substeps of the mean error it started from
     // learned lambda warm start (mode 0 off, 1 harvest, 2 warm start)
     int ws_mode = 0;
@@ -43,13 +46,22 @@ struct Scene {
     float* feat = nullptr;     // cap x 16 features of the current substep
     float* lam_acc = nullptr;  // per particle: sum of min(lambda, 0) over this substep's iterations
     float* lam_hat = nullptr;  // predicted lambdas
+    // tolerance mode and the warm-start fallback (solver_loop)
+    float tol = 0.0f;          // > 0: iterate until the mean density error is <= tol
+    int tol_max_iters = 0;     // iteration limit in tolerance mode
+    double* tol_err = nullptr; // per substep: error sums of the warm pass (slot 0) and iteration k (slot 1 + k)
+    int4* pos_save = nullptr;  // positions before the warm-start shift
+    unsigned long long* solve_stats = nullptr;  // [iterations applied, substeps, warm-start fallbacks] summed
 
-    // Optional HIP-event timing of one kernel (KernelId) on the scene stream.
-    int timed_kernel = -1;
+    // Optional HIP-event timing of kernels (bit KernelId of timed_mask) on the
+    // scene stream; event pair k belongs to kernel event_ids[k].
+    uint32_t timed_mask = 0;
     std::vector<hipEvent_t> events;
+    std::vector<int> event_ids;
     size_t events_used = 0;
     bool tbegin(int id) {
-        if (id != timed_kernel || events_used + 2 > events.size()) return false;
+        if (!(timed_mask & (1u << id)) || events_used + 2 > events.size()) return false;
+        event_ids[events_used / 2] = id;
         (void)hipEventRecord(events[events_used++], stream);
         return true;
     }
@@ -159,7 +171,8 @@ struct Scene {
     void spread_end();
     void prepare_solver();
     SolverArgs solver_args() const;
-    void incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override = nullptr);
+    void incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override = nullptr,
+                                     bool decided = false);
     void sphere_collision();
     void colliders();
     void solver_loop();

@@ -136,7 +136,10 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(feat, (size_t)cap * MLP_IN);
     A(lam_acc, cap);
     A(lam_hat, cap);
-    A(particle_substeps, 2);
+    A(particle_substeps, 5);
+    A(tol_err, MAX_ITER_RECORD + 1);
+    A(pos_save, cap);
+    A(solve_stats, 4);
 #undef A
     size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
     ws = std::max(ws, scan_workspace_bytes(pcap));
@@ -146,7 +149,8 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     HIP_OK(hipMemset(counts, 0, C_COUNT * 4));
     HIP_OK(hipMemset(err_stats, 0, 6 * sizeof(double)));
     HIP_OK(hipMemset(iter_err, 0, MAX_ITER_RECORD * sizeof(double)));
-    HIP_OK(hipMemset(particle_substeps, 0, 2 * sizeof(unsigned long long)));
+    HIP_OK(hipMemset(particle_substeps, 0, 5 * sizeof(unsigned long long)));
+    HIP_OK(hipMemset(solve_stats, 0, 4 * sizeof(unsigned long long)));
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_counts), C_COUNT * 4, hipHostMallocDefault));
     std::memset(h_counts, 0, C_COUNT * 4);
     HIP_OK(hipEventCreateWithFlags(&h_counts_ready, hipEventDisableTiming));
@@ -165,7 +169,8 @@ Scene::~Scene() {
     void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift,
                   ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_owner2, a_flag, a_scan,
-                  a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps};
+                  a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps,
+                  tol_err, pos_save, solve_stats};
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* q : ps) hipFree(q);
     if (h_counts) (void)hipHostFree(h_counts);
@@ -305,6 +310,8 @@ SolverArgs Scene::solver_args() const {
     a.it_err = nullptr;
     a.lam_override = nullptr;
     a.lam_acc = nullptr;
+    a.tol_sum = nullptr;
+    a.skip = nullptr;
     return a;
 }
 
@@ -332,19 +339,29 @@ void Scene::prepare_solver() {
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
 }
 
-void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override) {
+// `decided`: the iteration's row pass accumulates its density error into
+// tol_err, a decision kernel picks the apply pass's action (tolerance mode and
+// the warm-start fallback), and a finished solve skips the pass.
+void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const float* lam_override, bool decided) {
     hipStream_t s = stream;
     SolverArgs a = solver_args();
     if (measure_error && it >= 0 && it < MAX_ITER_RECORD) a.it_err = iter_err + it;
     if (ws_mode != 0 && it >= 0) a.lam_acc = lam_acc;
     a.lam_override = lam_override;
     if (lam_override) a.lam_acc = lam_acc;
+    if (decided) {
+        a.tol_sum = tol_err + (lam_override ? 0 : 1 + it);
+        if (!lam_override) a.skip = counts + C_SOLVE_DONE;
+    }
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
                       boundariness[cur], lambda, shift, s, solve_lanes, long_rows);
     tend(tk);
+    const bool judge = decided && !lam_override;
+    if (judge) launch_solve_decide(counts, tol_err, (uint32_t)it, tol, ws_mode == 2 && ws_model && it == 0, s);
     tk = tbegin(K_SOLVE_APPLY);
-    launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s);
+    launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s,
+                       judge ? counts + C_SOLVE_ACTION : nullptr, judge ? pos_save : nullptr);
     tend(tk);
 }
 
@@ -353,9 +370,27 @@ void Scene::incompressibility() {
     incompressibility_iteration(no_boxes, -1);
 }
 
+// The solver loop of pool.cpp:92-95 (box collision + incompressibility per
+// iteration). Extensions without a counterpart in the reference:
+//  - tolerance mode (tol > 0): up to tol_max_iters iterations; iteration k's
+//    row pass measures the mean density error e_k of the positions it starts
+//    from, and when e_k <= tol the iteration applies nothing and the solve ends;
+//  - learned warm start (mode 2): the features of the starting state go
+//    through the network and one shift pass with the predicted lambdas runs
+//    before the iterations; the plain solver judges it: when the error after
+//    the warm shift (iteration 0's e_0) exceeds the error before it, the
+//    positions from before the warm shift are put back and the plain
+//    iterations continue from there (cold start).
+// Every launch is fixed, so a substep stays graph-capturable; decisions are
+// device words.
 void Scene::solver_loop() {
-    if (P.solver_iterations <= 0) return;
+    const bool tolerance = tol > 0.0f;
+    const int iters = tolerance ? tol_max_iters : P.solver_iterations;
+    if (iters <= 0) return;
+    const bool warm = ws_mode == 2 && ws_model;
+    const bool decided = tolerance || warm;
     prepare_solver();
+    if (decided) launch_solve_begin(counts, tol_err, MAX_ITER_RECORD + 1, stream);
     colliders();
     if (ws_mode != 0) {
         // learned warm start: features of the state the solver starts from, then
@@ -364,18 +399,21 @@ void Scene::solver_loop() {
         launch_features_rows(pos[cur], vel[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc,
                              boundariness[cur], lam_prev[cur], counts, cap, row, col, pcap, solver_args(), last_dt,
                              feat, stream);
-        if (ws_mode == 2 && ws_model) {
+        if (warm) {
             launch_mlp_infer(ws_model->params, ws_model->norm, feat, counts + C_N, cap, lam_hat, stream);
-            incompressibility_iteration(boxes, -1, lam_hat);
+            copy_u32_dev(reinterpret_cast<uint32_t*>(pos_save), reinterpret_cast<const uint32_t*>(pos[cur]),
+                         counts + C_N4, 4 * cap, stream);
+            incompressibility_iteration(boxes, -1, lam_hat, true);
         }
     }
-    for (int it = 0; it < P.solver_iterations; ++it) {
-        bool last = it == P.solver_iterations - 1;
-        incompressibility_iteration(last ? no_boxes : boxes, it);
+    for (int it = 0; it < iters; ++it) {
+        bool last = it == iters - 1;
+        incompressibility_iteration(last ? no_boxes : boxes, it, nullptr, decided);
     }
     if (ws_mode != 0)
         copy_u32_dev(reinterpret_cast<uint32_t*>(lam_prev[cur]), reinterpret_cast<const uint32_t*>(lam_acc),
                      counts + C_N, cap, stream);
+    launch_solve_stats(counts, solve_stats, decided ? 0u : (uint32_t)iters, stream);
 }
 
 // ----------------------------------------------------------------- update
@@ -723,6 +761,45 @@ apbf_status apbf_scene_iteration_errors(void* scene, double* out, uint32_t max_i
     return APBF_OK;
 }
 
+apbf_status apbf_scene_get_warm_lambdas(void* scene, float* host_lambda) {
+    if (!scene || !host_lambda) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    uint32_t n = 0;
+    if (hipMemcpyAsync(&n, s->counts + apbf::C_ROWS, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (n && hipMemcpy(host_lambda, s->lam_hat, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_solver_tolerance(void* scene, float tolerance, int32_t max_iterations) {
+    if (!scene || (tolerance > 0.0f && (max_iterations <= 0 || max_iterations > apbf::MAX_ITER_RECORD)))
+        return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->tol = tolerance > 0.0f ? tolerance : 0.0f;
+    s->tol_max_iters = tolerance > 0.0f ? max_iterations : 0;
+    apbf::scene_drop_graph(s);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_solver_stats(void* scene, uint64_t* iterations, uint64_t* substeps, uint64_t* fallbacks,
+                                    int32_t reset) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    unsigned long long v[4] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(v, s->solve_stats, sizeof(v), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (iterations) *iterations = v[0];
+    if (substeps) *substeps = v[1];
+    if (fallbacks) *fallbacks = v[2];
+    if (reset && (hipMemsetAsync(s->solve_stats, 0, sizeof(v), s->stream) != hipSuccess ||
+                  hipStreamSynchronize(s->stream) != hipSuccess))
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
 apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t* pairs_total, int32_t reset) {
     if (!scene || !total) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
@@ -732,40 +809,69 @@ apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t*
         return APBF_ERR_HIP;
     *total = v[0];
     if (pairs_total) *pairs_total = v[1];
-    if (reset && (hipMemsetAsync(s->particle_substeps, 0, 16, s->stream) != hipSuccess ||
+    if (reset && (hipMemsetAsync(s->particle_substeps, 0, 5 * sizeof(unsigned long long), s->stream) != hipSuccess ||
                   hipStreamSynchronize(s->stream) != hipSuccess))
         return APBF_ERR_HIP;
     return APBF_OK;
 }
 
-apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches) {
-    if (!scene || kernel_id >= apbf::K_COUNT_IDS) return APBF_ERR_INVALID_ARGUMENT;
+apbf_status apbf_scene_time_kernels(void* scene, uint32_t kernel_mask, uint32_t max_launches) {
+    if (!scene || kernel_mask >= (1u << apbf::K_COUNT_IDS)) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     s->events.clear();
+    s->event_ids.clear();
     s->events_used = 0;
-    s->timed_kernel = kernel_id;
-    if (kernel_id < 0) return APBF_OK;
+    s->timed_mask = kernel_mask;
+    if (!kernel_mask) return APBF_OK;
     s->events.resize((size_t)max_launches * 2);
+    s->event_ids.resize(max_launches);
     for (auto& e : s->events)
         if (hipEventCreate(&e) != hipSuccess) return APBF_ERR_HIP;
     return APBF_OK;
 }
 
-apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches) {
+apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches) {
+    if (!scene || kernel_id >= apbf::K_COUNT_IDS) return APBF_ERR_INVALID_ARGUMENT;
+    return apbf_scene_time_kernels(scene, kernel_id < 0 ? 0u : (1u << kernel_id), max_launches);
+}
+
+apbf_status apbf_scene_kernel_time_of(void* scene, int32_t kernel_id, double* total_ms, uint32_t* launches) {
     if (!scene || !total_ms || !launches) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
     double t = 0.0;
+    uint32_t n = 0;
     for (size_t i = 0; i + 1 < s->events_used; i += 2) {
+        if (kernel_id >= 0 && s->event_ids[i / 2] != kernel_id) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, s->events[i], s->events[i + 1]) != hipSuccess) return APBF_ERR_HIP;
         t += ms;
+        ++n;
     }
     *total_ms = t;
-    *launches = (uint32_t)(s->events_used / 2);
-    s->events_used = 0;
+    *launches = n;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches) {
+    apbf_status st = apbf_scene_kernel_time_of(scene, -1, total_ms, launches);
+    if (st == APBF_OK) static_cast<Scene*>(scene)->events_used = 0;
+    return st;
+}
+
+apbf_status apbf_scene_substep_totals(void* scene, uint64_t* out, uint32_t n_out, int32_t reset) {
+    if (!scene || (!out && n_out)) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    if (hipMemcpyAsync(v, s->particle_substeps, sizeof(v), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    for (uint32_t k = 0; k < n_out; ++k) out[k] = k < 5 ? v[k] : 0;
+    if (reset && (hipMemsetAsync(s->particle_substeps, 0, sizeof(v), s->stream) != hipSuccess ||
+                  hipStreamSynchronize(s->stream) != hipSuccess))
+        return APBF_ERR_HIP;
     return APBF_OK;
 }
 

@@ -31,7 +31,19 @@ enum CountSlot : uint32_t {
     C_MAXROW = 20,     // single-pass walk: longest row that outgrew its slot (0: none)
     C_FALLBACK = 21,   // single-pass walk: 1 when the exact fill must rewrite every row (clamped list)
     C_ROWFIX = 22,     // single-pass walk: 1 when some rows outgrew their slot (exact fill of those rows)
+    C_SOLVE_DONE = 23,     // tolerance mode: the density error reached the tolerance; later iterations do nothing
+    C_SOLVE_ACTION = 24,   // what the current iteration's apply pass does (SolveAction)
+    C_SOLVE_ITERS = 25,    // solver iterations whose shifts were applied in this substep
+    C_WARM_FALLBACK = 26,  // 1 when this substep's warm start was undone (the cold start judged better)
     C_COUNT = 32
+};
+
+// The apply pass of one solver iteration, as decided from that iteration's
+// density error (tolerance mode, warm-start fallback).
+enum SolveAction : uint32_t {
+    SOLVE_APPLY = 0,    // add the shifts
+    SOLVE_SKIP = 1,     // leave the positions (the error is within the tolerance, or the solve is done)
+    SOLVE_RESTORE = 2,  // put back the positions saved before the warm-start shift
 };
 
 struct GridDesc {
@@ -58,6 +70,8 @@ struct SolverArgs {
     double* it_err;  // when set: += mean density error of the positions this iteration starts from
     const float* lam_override;  // when set: shift with these lambdas, no boundariness update or filter
     float* lam_acc;             // when set: += min(lambda, 0) per particle
+    double* tol_sum;            // when set: += sum of max(0, rho/rho0 - 1) over the positions the pass starts from
+    const uint32_t* skip;       // when set and *skip != 0: the pass does nothing (the solve is done)
 };
 
 // -------------------------------------------------------- shared device code

@@ -186,6 +186,48 @@ class Scene:
         check(self._L.apbf_scene_iteration_errors(self._h, _ptr(out), max_iters, int(reset)), "iteration_errors")
         return out
 
+    def set_solver_tolerance(self, tolerance: float, max_iterations: int = 16) -> None:
+        """Iterate until the mean density error is <= tolerance (at most
+        max_iterations per substep); tolerance <= 0: the fixed count."""
+        check(self._L.apbf_scene_set_solver_tolerance(self._h, float(tolerance), int(max_iterations)),
+              "apbf_scene_set_solver_tolerance")
+
+    def warm_lambdas(self) -> np.ndarray:
+        """The network's lambdas of the last warm-started substep (per particle
+        of that substep's solver)."""
+        rows = self.diagnostics()["rows"]
+        out = np.zeros(max(rows, 1), np.float32)
+        check(self._L.apbf_scene_get_warm_lambdas(self._h, _ptr(out)), "apbf_scene_get_warm_lambdas")
+        return out[:rows]
+
+    def solver_stats(self, reset: bool = False):
+        """(iterations applied, substeps, warm starts undone) summed over substeps."""
+        v = [C.c_uint64() for _ in range(3)]
+        check(self._L.apbf_scene_solver_stats(self._h, *[C.byref(x) for x in v], int(reset)), "apbf_scene_solver_stats")
+        return tuple(x.value for x in v)
+
+    def substep_totals(self, reset: bool = False) -> dict:
+        """Sums over substeps since the last reset: particles the solver ran on,
+        its pairs, the walk's pairs before the spreading filter, particles
+        created by splits and deleted by finished merges."""
+        out = np.zeros(5, np.uint64)
+        check(self._L.apbf_scene_substep_totals(self._h, _ptr(out), 5, int(reset)), "apbf_scene_substep_totals")
+        return dict(zip(("particles", "pairs", "raw_pairs", "created", "deleted"), (int(x) for x in out)))
+
+    def time_kernels(self, names, max_launches: int = 4096) -> None:
+        mask = 0
+        for n in names:
+            mask |= 1 << self.KERNELS[n]
+        check(self._L.apbf_scene_time_kernels(self._h, mask, max_launches), "time_kernels")
+
+    def kernel_time_of(self, name: str):
+        """(total milliseconds, launches) of one timed kernel, without restarting."""
+        t = C.c_double()
+        n = C.c_uint32()
+        check(self._L.apbf_scene_kernel_time_of(self._h, self.KERNELS[name], C.byref(t), C.byref(n)),
+              "kernel_time_of")
+        return t.value, n.value
+
     def particle_substeps(self, reset: bool = False):
         """(sum of particle counts, sum of pair counts) over substeps since the last reset."""
         v, p = C.c_uint64(), C.c_uint64()

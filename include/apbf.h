@@ -163,6 +163,23 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
  * after the last iteration is apbf_scene_error_stats. Synchronises; `reset`
  * clears the record (the substep count is cleared by apbf_scene_error_stats). */
 apbf_status apbf_scene_iteration_errors(void* scene, double* out, uint32_t max_iters, int32_t reset);
+/* Tolerance mode of the solver loop (pool.cpp:92-95 runs a fixed
+ * solverIterations; no counterpart in the reference): with tolerance > 0 each
+ * substep runs up to max_iterations iterations (box collision +
+ * incompressibility), and an iteration whose starting positions have a mean
+ * density error max(0, rho/rho0 - 1) <= tolerance applies no shifts and ends
+ * the solve (its boundariness update still happens, as in incompressibility_2);
+ * tolerance <= 0 returns to the fixed count. The decisions are device words,
+ * so substeps stay graph-capturable. With the learned warm start (mode 2) the
+ * plain solver also judges the warm shift: when the error after it exceeds
+ * the error before it, the positions from before it are put back and the
+ * plain iterations continue from there. Drops a captured graph. */
+apbf_status apbf_scene_set_solver_tolerance(void* scene, float tolerance, int32_t max_iterations);
+/* Sums over substeps of the solver iterations whose shifts were applied, the
+ * substeps, and the warm starts that were undone; cleared when `reset` is
+ * non-zero. Synchronises. */
+apbf_status apbf_scene_solver_stats(void* scene, uint64_t* iterations, uint64_t* substeps, uint64_t* fallbacks,
+                                    int32_t reset);
 /* Sums over substeps of the number of particles the solver ran on (the unit of
  * the particle-substeps/s metric) and of the neighbour pairs, kept on the
  * device; cleared when `reset` is non-zero. Synchronises. */
@@ -176,6 +193,18 @@ apbf_status apbf_scene_particle_substeps(void* scene, uint64_t* total, uint64_t*
  * the recorded launches and restarts the recording. Both synchronise. */
 apbf_status apbf_scene_time_kernel(void* scene, int32_t kernel_id, uint32_t max_launches);
 apbf_status apbf_scene_kernel_time(void* scene, double* total_ms, uint32_t* launches);
+/* The same for several kernels at once (bit k of kernel_mask: kernel id k);
+ * apbf_scene_kernel_time_of returns one kernel's share of the recording
+ * (kernel_id -1: all) without restarting it. Both synchronise. */
+apbf_status apbf_scene_time_kernels(void* scene, uint32_t kernel_mask, uint32_t max_launches);
+apbf_status apbf_scene_kernel_time_of(void* scene, int32_t kernel_id, double* total_ms, uint32_t* launches);
+/* Sums over substeps, kept on the device: out[0] particles the solver ran on,
+ * out[1] its neighbour pairs, out[2] the neighbour walk's pairs before the
+ * kernel-width spreading filtered them (equal to out[1] without spreading),
+ * out[3] particles created by splits, out[4] particles deleted when merges
+ * finished. Cleared when `reset` is non-zero (with
+ * apbf_scene_particle_substeps' reset too). Synchronises. */
+apbf_status apbf_scene_substep_totals(void* scene, uint64_t* out, uint32_t n_out, int32_t reset);
 /* Copies the first n_words words of the scene's device-resident count block:
  * 0 particles, 2 neighbour pairs, 3 transfers, 4 splits, 6 overflow flags,
  * 7 neighbour-walk pairs before the capacity clamp, 14 particles covered by
@@ -562,6 +591,11 @@ apbf_status apbf_mlp_gradient(apbf_mlp* model, const float* features, const floa
  * `model`, and one shift pass with the predicted lambdas runs before the
  * solver's iterations. The model must outlive its use by the scene. */
 apbf_status apbf_scene_set_warm_start(void* scene, apbf_mlp* model, int32_t mode);
+/* The lambdas the network predicted in the last warm-started substep, one
+ * per particle of that substep's solver (the particles its neighbour rows
+ * cover, apbf_scene_get_counts' count before split particles were added).
+ * Synchronises. */
+apbf_status apbf_scene_get_warm_lambdas(void* scene, float* host_lambda);
 /* Copies the features (n x 16) and targets (n) of the last harvested substep
  * to device buffers of `capacity` rows; *count receives n. Synchronises. */
 apbf_status apbf_scene_harvest(void* scene, float* features, float* targets, uint32_t capacity, uint32_t* count);

@@ -832,11 +832,39 @@ struct Scene {
         return oldv + gmin(maxStep, gmax(-maxStep, step));
     }
 
-    void incompressibility() {
+    // What an iteration's shifts do (tolerance mode and the warm-start
+    // fallback of the library's solver loop; no counterpart in the reference).
+    enum Action { APPLY = 0, SKIP = 1, RESTORE = 2 };
+
+    // Mean over the fluid of max(0, rho / rho0 - 1) from the sums of
+    // incompressibility_0/1 (the density-error definition of the library's
+    // statistics), summed in id order in double precision.
+    double mean_error_of(const std::vector<Incomp>& inc) const {
+        double sum = 0.0;
+        for (uint32_t id = 0; id < F(); ++id) {
+            uint32_t idx = fidx[id];
+            float ird = pow_f(2.0f * radius[idx], DIM) * inverse_mass[idx];
+            float density = (float)inc[id].density / INCOMP_RES;
+            sum += (double)gmax(0.0f, density * ird - 1.0f);
+        }
+        return F() ? sum / (double)F() : 0.0;
+    }
+
+    // One incompressibility::apply. Library extensions: `lam_override` replaces
+    // the lambdas of incompressibility_2 (no boundariness update or filter:
+    // the warm-start shift pass); `decide`, given the mean density error of the
+    // positions the pass starts from, chooses what the shifts do (the
+    // boundariness update and filter run either way). Returns that error.
+    template <class Decide>
+    double incompressibility_ext(const float* lam_override, Decide decide) {
         std::vector<Incomp> inc;
         std::vector<Com> com;
         std::vector<V3> grads;
         density_pass(inc, com, &grads);
+        const double err = mean_error_of(inc);
+        const int action = decide(err);
+        const bool shifts = action == APPLY;
+        if (action == RESTORE) position = saved_position;
         uint32_t nf = F();
         lambda.assign(nf, 0.0f);
         float lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, DIM);
@@ -854,7 +882,7 @@ struct Scene {
             float selfGradLength = sqrtf(wgss) * ird;
             sgs += wgss * im;
             float under = 1.0f - density * ird;
-            if (P.s.mUpdateBoundariness) {  // compute_boundariness, :43-69
+            if (P.s.mUpdateBoundariness && !lam_override) {  // compute_boundariness, :43-69
                 float sgl = selfGradLength * kw;
                 float b;
                 if (P.s.mBoundarinessCalculationMethod == 1) {
@@ -872,8 +900,9 @@ struct Scene {
             }
             float lam = under / (ird * ird * (sgs + INCOMP_EPSILON));
             lam /= lambda_scale_base / ird * im;
+            if (lam_override) lam = lam_override[id];
             lambda[id] = lam;
-            if (lam >= 0.0f) continue;
+            if (!shifts || lam >= 0.0f) continue;
             float s = lam * im * POS_RES;
             V3 d = muls(selfGrad, s);
             I4& p = position[idx];
@@ -886,18 +915,29 @@ struct Scene {
             uint32_t idxN = fidx[n.b];
             float lam = lambda[n.a];
             V3 sg = grads[id];
-            if (P.s.mBoundarinessCalculationMethod == 2) {  // filter_boundariness, :41-46
+            if (P.s.mBoundarinessCalculationMethod == 2 && !lam_override) {  // filter_boundariness, :41-46
                 V3 empty = neg(normalize(vec_of(inc[n.a].g[0], inc[n.a].g[1], inc[n.a].g[2])));
                 V3 nd = normalize(sg);
                 if (dot(empty, nd) > 0.6f) boundariness[n.a] = 0.0f;
             }
-            if (lam >= 0.0f) continue;
+            if (!shifts || lam >= 0.0f) continue;
             V3 d = muls(sg, lam * POS_RES);
             I4& p = position[idxN];
             p.x = iadd(p.x, to_i32(d.x));
             p.y = iadd(p.y, to_i32(d.y));
             p.z = iadd(p.z, to_i32(d.z));
         }
+        return err;
+    }
+
+    std::vector<I4> saved_position;  // positions before the warm-start shift
+    float tol = 0.0f;                // tolerance mode: iterate until the mean density error is <= tol
+    int tol_max_iters = 0;
+    std::vector<float> warm_lambda;  // predicted lambdas for the next update's warm-start shift
+    uint64_t stat_iters = 0, stat_substeps = 0, stat_fallbacks = 0;
+
+    void incompressibility() {
+        incompressibility_ext(nullptr, [](double) { return (int)APPLY; });
     }
 
     void density_error(double* mean_err, double* max_err) const {
@@ -1057,12 +1097,58 @@ struct Scene {
         else
             neighborhood(scale);
         if (!P.basic_pbf && !s.mBaseKernelWidthOnBoundaryDistance) spread_kernel_width();
-        for (int i = 0; i < P.solver_iterations; ++i) {
-            if (P.n_boxes) box_collision();  // scenes without boxes have no box stage
-            sphere_collision();
-            incompressibility();
-        }
+        solver_loop();
         if (!P.basic_pbf) update_transfers();
+    }
+
+    void colliders() {
+        if (P.n_boxes) box_collision();  // scenes without boxes have no box stage
+        sphere_collision();
+    }
+
+    // The solver loop of pool.cpp:92-95 (box collision + incompressibility,
+    // solverIterations times), with the library's extensions: a tolerance mode
+    // (up to tol_max_iters iterations; an iteration whose starting error is
+    // <= tol applies no shifts and ends the loop) and the warm start (one shift
+    // pass with given lambdas before the iterations; when the first plain
+    // iteration starts from a larger error than the warm pass did, the
+    // positions from before the warm shift are put back).
+    void solver_loop() {
+        const bool tolerance = tol > 0.0f;
+        const int iters = tolerance ? tol_max_iters : P.solver_iterations;
+        const bool warm = !warm_lambda.empty();
+        if (iters <= 0) {
+            warm_lambda.clear();
+            return;
+        }
+        int applied = 0;
+        bool done = false, fell_back = false;
+        double e0 = 0.0;
+        colliders();
+        if (warm) {
+            saved_position = position;
+            e0 = incompressibility_ext(warm_lambda.data(), [](double) { return (int)APPLY; });
+            if (!MUT(12)) colliders();
+        }
+        for (int k = 0; k < iters && !done; ++k) {
+            incompressibility_ext(nullptr, [&](double e) {
+                if (warm && k == 0 && (MUT(11) ? e < e0 : e > e0)) {
+                    fell_back = true;
+                    return (int)RESTORE;
+                }
+                if (tolerance && e <= (double)tol) {
+                    done = true;
+                    return MUT(10) ? (int)APPLY : (int)SKIP;
+                }
+                ++applied;
+                return (int)APPLY;
+            });
+            if (k + 1 < iters && !done) colliders();
+        }
+        stat_iters += (tolerance || warm) ? (uint64_t)applied : (uint64_t)iters;
+        stat_substeps += 1;
+        stat_fallbacks += fell_back ? 1 : 0;
+        warm_lambda.clear();
     }
 
     // sphere_collision.comp:42-60 (the collider of spherical_pool.cpp:29)
@@ -1227,6 +1313,21 @@ void orc_stage(void* h, int stage, float dt) {
 }
 
 void orc_update(void* h, float dt) { static_cast<Scene*>(h)->update(dt); }
+void orc_set_solver_tolerance(void* h, float tol, int max_iters) {
+    Scene* s = static_cast<Scene*>(h);
+    s->tol = tol > 0.0f ? tol : 0.0f;
+    s->tol_max_iters = tol > 0.0f ? max_iters : 0;
+}
+void orc_set_warm_lambdas(void* h, const float* lam, uint32_t n) {
+    Scene* s = static_cast<Scene*>(h);
+    s->warm_lambda.assign(lam, lam + n);
+}
+void orc_solver_stats(void* h, uint64_t* iters, uint64_t* substeps, uint64_t* fallbacks) {
+    Scene* s = static_cast<Scene*>(h);
+    *iters = s->stat_iters;
+    *substeps = s->stat_substeps;
+    *fallbacks = s->stat_fallbacks;
+}
 
 void orc_density_error(void* h, double* mean_err, double* max_err) {
     static_cast<Scene*>(h)->density_error(mean_err, max_err);

@@ -123,6 +123,15 @@ void orc_stage(void* h, int stage, float dt);
 uint32_t orc_get_fsm(void* h, uint32_t* bd_fill, uint32_t* min_dist, uint32_t* nearest);
 /* One full substep (pool::update / waterdrop::update with the time machine off). */
 void orc_update(void* h, float dt);
+/* Library extensions of the solver loop (no counterpart in the reference):
+ * tolerance mode (tol > 0: up to max_iters iterations, stopping at the first
+ * whose starting mean density error is <= tol; tol <= 0: solverIterations),
+ * the warm-start shift with the given lambdas (per fluid id at solver time,
+ * used by the next update only) judged by the first plain iteration, and the
+ * sums of applied iterations, substeps and undone warm starts. */
+void orc_set_solver_tolerance(void* h, float tol, int max_iters);
+void orc_set_warm_lambdas(void* h, const float* lambda, uint32_t n);
+void orc_solver_stats(void* h, uint64_t* iters, uint64_t* substeps, uint64_t* fallbacks);
 /* Mean and maximum per-particle compression error max(0, rho/rho0 - 1) of the
  * current positions, from a density pass that leaves the state untouched. */
 void orc_density_error(void* h, double* mean_err, double* max_err);

@@ -102,6 +102,9 @@ def _declare(L):
     L.orc_set_neighborhood_method.argtypes = [vp, C.c_int]
     L.orc_set_sphere.argtypes = [vp, pf, f32, C.c_int, C.c_int]
     L.orc_update.argtypes = [vp, f32]
+    L.orc_set_solver_tolerance.argtypes = [vp, f32, C.c_int]
+    L.orc_set_warm_lambdas.argtypes = [vp, pf, u32]
+    L.orc_solver_stats.argtypes = [vp, _p(C.c_uint64), _p(C.c_uint64), _p(C.c_uint64)]
     L.orc_density_error.argtypes = [vp, _p(f64), _p(f64)]
     L.orc_stable_radix_sort.argtypes = [pu, pu, u32, u32, pu, pu]
     L.orc_prefix_sum.argtypes = [pu, u32]
@@ -287,6 +290,19 @@ class OracleScene:
 
     def update(self, dt: float) -> None:
         self._L.orc_update(self._h, float(dt))
+
+    def set_solver_tolerance(self, tol: float, max_iterations: int = 16) -> None:
+        self._L.orc_set_solver_tolerance(self._h, float(tol), int(max_iterations))
+
+    def set_warm_lambdas(self, lam) -> None:
+        a = _f32(lam)
+        self._warm = a
+        self._L.orc_set_warm_lambdas(self._h, _ptr(a, C.c_float), len(a))
+
+    def solver_stats(self):
+        v = [C.c_uint64() for _ in range(3)]
+        self._L.orc_solver_stats(self._h, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     def density_error(self):
         m, x = C.c_double(), C.c_double()

@@ -300,7 +300,123 @@ def pin_mlp_gradient(mutation: int = 0):
         assert abs(fd - g[k]) <= 1e-6 + 1e-5 * abs(fd), (k, fd, g[k])
 
 
+# --------------------------------------------- solver loop: tolerance and warm start
+def _solver_case(library=None, iterations=4):
+    """A compressed dam break (lattice at 0.48 h: the density starts above rest)
+    after one substep, ready for the next one's stages."""
+    p, st = cfg.dam_break(n_side=9, spacing=0.48, iterations=iterations)
+    o = OracleScene(p, library=library)
+    o.set_state(st)
+    o.update(p["dt"])
+    return p, o.get_state()
+
+
+def _stages_until_solver(o, dt):
+    o.stage("velocity", dt)
+    o.stage("neighborhood", dt)
+
+
+def pin_tolerance_loop(library=None):
+    """Tolerance mode against the same loop driven from outside through the
+    oracle's stages and its density-error pass: before each iteration the box
+    collision runs, then the mean error of the current positions decides -
+    at or below the tolerance nothing more moves; else one incompressibility
+    pass. Also: a tolerance below every error gives the fixed count."""
+    p, st = _solver_case(library)
+    dt = p["dt"]
+    ref = OracleScene(p, library=library)
+    ref.set_state(st)
+    _stages_until_solver(ref, dt)
+    errs = []
+    for _ in range(8):
+        ref.stage("box", dt)
+        errs.append(ref.density_error()[0])
+        ref.stage("incompressibility", dt)
+    tol = 0.5 * (errs[2] + errs[3])  # three iterations are above it, the fourth starts below
+    assert errs[2] > tol >= errs[3]
+    manual = OracleScene(p, library=library)
+    manual.set_state(st)
+    _stages_until_solver(manual, dt)
+    applied = 0
+    for k in range(8):
+        manual.stage("box", dt)
+        if manual.density_error()[0] <= tol:
+            break
+        manual.stage("incompressibility", dt)
+        applied += 1
+    auto = OracleScene(p, library=library)
+    auto.set_state(st)
+    auto.set_solver_tolerance(tol, 8)
+    auto.update(dt)
+    a, m = auto.get_state(), manual.get_state()
+    assert np.array_equal(a["position"], m["position"])
+    assert auto.solver_stats()[0] == applied == 3
+    # a tolerance no error reaches: the fixed count, bit for bit
+    fixed = OracleScene(dict(p, solver_iterations=5), library=library)
+    fixed.set_state(st)
+    fixed.update(dt)
+    tiny = OracleScene(p, library=library)
+    tiny.set_state(st)
+    tiny.set_solver_tolerance(1e-30, 5)
+    tiny.update(dt)
+    assert np.array_equal(fixed.get_state()["position"], tiny.get_state()["position"])
+    assert np.array_equal(fixed.get_state()["boundariness"].view(np.uint32),
+                          tiny.get_state()["boundariness"].view(np.uint32))
+
+
+def pin_warm_start(library=None):
+    """The warm-start shift pass and its judgement, against the stage sequence:
+    zero predicted lambdas move nothing, so the substep equals box, box, then
+    the plain iterations with a box before each (or, when the second box
+    collision alone raised the error, one iteration fewer); lambdas that raise
+    the compression are undone, and the substep equals box, box, then one
+    iteration fewer."""
+    p, st = _solver_case(library)
+    dt = p["dt"]
+    iters = p["solver_iterations"]
+
+    def staged(n_solves):
+        o = OracleScene(p, library=library)
+        o.set_state(st)
+        _stages_until_solver(o, dt)
+        o.stage("box", dt)
+        for k in range(n_solves):
+            o.stage("box", dt)
+            o.stage("incompressibility", dt)
+        return o.get_state()["position"]
+
+    n = len(st["radius"])
+    # zero lambdas: the judgement compares the error after the second box
+    # collision with the error after the first one (the warm pass moves nothing)
+    probe = OracleScene(p, library=library)
+    probe.set_state(st)
+    _stages_until_solver(probe, dt)
+    probe.stage("box", dt)
+    e0 = probe.density_error()[0]
+    probe.stage("box", dt)
+    undone = probe.density_error()[0] > e0
+    zero = OracleScene(p, library=library)
+    zero.set_state(st)
+    zero.set_warm_lambdas(np.zeros(n, np.float32))
+    zero.update(dt)
+    want = staged(iters - 1) if undone else staged(iters)
+    assert np.array_equal(zero.get_state()["position"], want)
+    assert zero.solver_stats() == ((iters - 1, 1, 1) if undone else (iters, 1, 0))
+    # a few particles with a far too large lambda push their neighbours into
+    # the others: the compression after the warm shift exceeds the one before
+    rng = np.random.default_rng(0)
+    bad_lam = np.where(rng.random(n) < 0.02, -20.0, 0.0).astype(np.float32)
+    bad = OracleScene(p, library=library)
+    bad.set_state(st)
+    bad.set_warm_lambdas(bad_lam)
+    bad.update(dt)
+    assert bad.solver_stats() == (iters - 1, 1, 1)
+    assert np.array_equal(bad.get_state()["position"], staged(iters - 1))
+
+
 PINS = {
+    "tolerance_loop": pin_tolerance_loop,
+    "warm_start": pin_warm_start,
     "flood_fill": pin_flood_fill,
     "nearest": pin_nearest,
     "find_closed_forms": pin_find_closed_forms,
@@ -329,6 +445,9 @@ MUTATIONS = {
     7: "kernel_width_from_bd",  # no snap where the step crosses the target
     8: "kernel_width_from_bd",  # lower bound r instead of 4 r
     9: "claims",              # a merge does not check that its target is free
+    10: "tolerance_loop",     # the iteration that reaches the tolerance still shifts
+    11: "warm_start",         # the warm shift is undone when it helped, kept when it hurt
+    12: "warm_start",         # no box collision between the warm shift and the first iteration
 }
 
 
