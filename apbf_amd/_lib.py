@@ -60,6 +60,35 @@ class SceneParams(C.Structure):
     ]
 
 
+class SegmentDesc(C.Structure):
+    _fields_ = [
+        ("hash_min", C.c_float * 3),
+        ("hash_max", C.c_float * 3),
+        ("hash_res_log2", C.c_uint32),
+        ("n_boxes", C.c_uint32),
+        ("box_min", (C.c_float * 4) * MAX_BOXES),
+        ("box_max", (C.c_float * 4) * MAX_BOXES),
+    ]
+
+
+def make_segment(p: dict) -> SegmentDesc:
+    """The hash grid and boxes of one scene of a packed batch."""
+    out = SegmentDesc()
+    for i in range(3):
+        out.hash_min[i] = p["hash_min"][i]
+        out.hash_max[i] = p["hash_max"][i]
+    out.hash_res_log2 = int(p["hash_res_log2"])
+    boxes = p["boxes"]
+    if len(boxes) > MAX_BOXES:
+        raise ValueError("too many boxes")
+    out.n_boxes = len(boxes)
+    for i, (bmin, bmax) in enumerate(boxes):
+        for d in range(3):
+            out.box_min[i][d] = bmin[d]
+            out.box_max[i][d] = bmax[d]
+    return out
+
+
 def make_params(p: dict) -> SceneParams:
     out = SceneParams()
     for k, v in p["settings"].items():
@@ -92,6 +121,9 @@ _pu, _pi, _pf, _pd = C.POINTER(_u32), C.POINTER(_i32), C.POINTER(_f32), C.POINTE
 PROTOTYPES = {
     "apbf_scene_create": (_i32, [C.POINTER(SceneParams), C.POINTER(_vp)]),
     "apbf_scene_destroy": (_i32, [_vp]),
+    "apbf_scene_create_packed": (_i32, [C.POINTER(SceneParams), _u32, C.POINTER(SegmentDesc), C.POINTER(_vp)]),
+    "apbf_scene_set_segment_ids": (_i32, [_vp, _vp]),
+    "apbf_scene_get_segment_ids": (_i32, [_vp, _vp]),
     "apbf_scene_set_stream": (_i32, [_vp, _vp]),
     "apbf_scene_set_state": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _u32, _vp, _vp, _vp, _f32]),

@@ -151,7 +151,7 @@ def _sphere_points(center, radius_world, spacing, jitter, rng):
 
 def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coarse: float = 1.0,
                r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6, cap_factor: float = 1.5,
-               pairs_per_particle: int = 800, target_scale: float | None = None):
+               pairs_per_particle: int = 800, target_scale: float | None = None, merge_duration: float = 2.0):
     """Adaptive water drop: a sphere of `drop_particles` fine particles (radius
     r_fine = 0.5h) released at t = 0 above a pool of coarse particles (radius
     r_coarse = 1h, a pool_counts lattice at spacing 2h: 150 x 40 x 150 = 900 000
@@ -186,7 +186,7 @@ def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coar
         target_scale = r_coarse / (0.5 * min(pool_max))
     s = dict(REFERENCE_SETTINGS)
     s.update(mBaseKernelWidthOnBoundaryDistance=0, mSmallestTargetRadius=r_coarse,
-             mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=target_scale)
+             mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=target_scale, mMergeDuration=merge_duration)
     tr = (r_coarse + bd * target_scale).astype(np.float32)
     bness = (bd < 2.0 * radius).astype(np.float32)
     st = make_state(pos, radius, bd=bd, boundariness=bness, target_radius=tr)
@@ -201,6 +201,86 @@ def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coar
         workload=f"water_drop_{n}",
     )
     return p, st
+
+
+def _subset(st: dict, keep: np.ndarray) -> dict:
+    """The particles `keep` of a state without transfers, fluid ids renumbered."""
+    out = {k: (st[k][:0] if k.startswith("t_") else st[k][keep]) for k in STATE_KEYS if k != "fluid_index"}
+    out["fluid_index"] = np.arange(int(np.count_nonzero(keep)), dtype=np.uint32)
+    out["last_delta_time"] = st["last_delta_time"]
+    return out
+
+
+def _join(a: dict, b: dict) -> dict:
+    """Two states as one: b's particles after a's, b's transfer endpoints moved with them."""
+    na = len(a["radius"])
+    out = {}
+    for k in STATE_KEYS:
+        if k == "fluid_index":
+            out[k] = np.arange(na + len(b["radius"]), dtype=np.uint32)
+        elif k in ("t_source", "t_target"):
+            out[k] = np.concatenate([a[k], b[k] + np.uint32(na)]).astype(np.uint32)
+        else:
+            out[k] = np.concatenate([a[k], b[k]])
+    out["last_delta_time"] = a["last_delta_time"]
+    return out
+
+
+def settled_water_drop(pool_counts=(180, 50, 180), drop_particles: int = 100_000, settle: int = 300,
+                       drop_settle: int = 100, gap: float = 3.0, merge_duration: float = 0.05, seed: int = 0,
+                       iterations: int = 4, res: int = 6, pairs_per_particle: int = 300):
+    """The adaptive water drop from rest (runs the library on the GPU).
+
+    The coarse pool of :func:`water_drop` first runs `settle` substeps on its
+    own with split and merge on: it settles under gravity and coarsens
+    towards its adaptive equilibrium (merged particles of up to 2h in the
+    deep interior, the surface at the base radius), 1.62 M lattice particles
+    becoming about 0.9 M. The fine drop settles separately for `drop_settle`
+    substeps without gravity (the reference holds its drop in place until it
+    is released, waterdrop.cpp:129-131). The drop is then put `gap` units
+    above the pool's highest particle and released. Both parts start this
+    workload at their rest density, so its density error is the solver's
+    steady error, and particles split and merge in every substep. Merges take
+    `merge_duration` seconds (6 substeps at dt = 1/120; the reference's
+    default is 2 s). Returns (params, state, report)."""
+    from .scene import Scene
+    p, st = water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, iterations=iterations,
+                       res=res, pairs_per_particle=pairs_per_particle, merge_duration=merge_duration,
+                       cap_factor=1.0)
+    n0 = len(st["radius"])
+    fine = st["radius"] < np.float32(0.75)
+    p["cap_transfers"] = max(1000, n0 // 4)
+    dt = p["dt"]
+    report = {"lattice_pool": int(np.count_nonzero(~fine)), "drop": int(np.count_nonzero(fine)),
+              "settle_substeps": settle, "drop_settle_substeps": drop_settle}
+    parts = []
+    for keep, steps, gravity in ((~fine, settle, p["gravity"]), (fine, drop_settle, (0.0, 0.0, 0.0))):
+        q = dict(p, gravity=gravity)
+        sc = Scene(q)
+        sc.set_state(_subset(st, keep))
+        for _ in range(steps):
+            sc.update(dt)
+        parts.append(sc.get_state(pairs=False))
+        sc.close()
+    pool, drop = parts
+    n_pool = len(pool["radius"])
+    pool["fluid_index"] = pool["fluid_index"][:n_pool]
+    drop["fluid_index"] = drop["fluid_index"][: len(drop["radius"])]
+    top = int(pool["position"][:, 1].max())
+    shift = np.int32(top + int(gap * POS_RESOLUTION) - int(drop["position"][:, 1].min()))
+    for k in ("position", "pos_backup"):
+        drop[k] = drop[k].copy()
+        drop[k][:, 1] += shift
+    both = _join(pool, drop)
+    both["last_delta_time"] = pool["last_delta_time"]
+    n = len(both["radius"])
+    p["cap_particles"] = int(n * 1.5)
+    p["cap_pairs"] = p["cap_particles"] * pairs_per_particle
+    p["cap_transfers"] = max(1000, n // 4)
+    p["hash_max"] = (p["hash_max"][0], max(p["hash_max"][1], top / POS_RESOLUTION + 40.0), p["hash_max"][2])
+    p["workload"] = f"water_drop_settled_{n}"
+    report.update(settled_pool=n_pool, settled_drop=len(drop["radius"]), particles=n)
+    return p, both, report
 
 
 def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: int = 2000, res: int = 4,
@@ -379,7 +459,9 @@ def batch_scenes(count: int = 64, n_min: int = 10_000, n_max: int = 500_000, see
     """Independent scenes for batched stepping: particle counts drawn
     log-uniformly from [n_min, n_max] (seeded), alternating a plain dam break
     (basic PBF lattice) and an adaptive water drop whose pool and drop are
-    scaled to the drawn count. Returns a list of (params, state)."""
+    scaled to the drawn count (one target-radius scale for all drops, so the
+    scenes of each kind share their settings and can be packed into one
+    list). Returns a list of (params, state)."""
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(math.log(n_min), math.log(n_max), count)).round().astype(int)
     out = []
@@ -392,7 +474,8 @@ def batch_scenes(count: int = 64, n_min: int = 10_000, n_max: int = 500_000, see
             a = max(6, int(round((0.9 * n / 0.27) ** (1.0 / 3.0))))
             b = max(3, int(round(0.27 * a)))
             p, st = water_drop(pool_counts=(a, b, a), drop_particles=max(100, int(n) // 10), seed=seed + k,
-                               res=5 if n < 100_000 else 6, pairs_per_particle=300, cap_factor=1.5)
+                               res=5 if n < 100_000 else 6, pairs_per_particle=300, cap_factor=1.5,
+                               target_scale=0.05)
         p["workload"] = f"batch_{k}_{p.get('workload', '')}"
         out.append((p, st))
     return out

@@ -116,6 +116,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_particles(const uint32_t* __r
         d.boundariness[o] = s.boundariness[i];
         d.boundary_distance[o] = s.boundary_distance[i];
         if (s.lam_prev) d.lam_prev[o] = s.lam_prev[i];
+        if (s.sid) d.sid[o] = s.sid[i];
     }
 }
 
@@ -479,6 +480,7 @@ __global__ __launch_bounds__(BLOCK) void k_split_finish(const uint32_t* __restri
         a.boundariness[d] = a.boundariness[s];
         a.boundary_distance[d] = a.boundary_distance[s];
         if (a.lam_prev) a.lam_prev[d] = a.lam_prev[s];
+        if (a.sid) a.sid[d] = a.sid[s];
         p.x = iadd(p.x, to_i32(r * POS_RES * 0.1f));
         a.pos[d] = p;
         a.radius[d] = 0.0f;

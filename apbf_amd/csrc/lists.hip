@@ -60,6 +60,7 @@ __global__ __launch_bounds__(BLOCK) void k_copy_particles(const uint32_t* __rest
         dst.boundariness[i] = src.boundariness[i];
         dst.boundary_distance[i] = src.boundary_distance[i];
         dst.lam_prev[i] = src.lam_prev[i];
+        if (src.sid) dst.sid[i] = src.sid[i];
     }
 }
 __global__ __launch_bounds__(BLOCK) void k_copy_transfers(const uint32_t* __restrict__ counts,

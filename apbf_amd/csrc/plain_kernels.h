@@ -19,13 +19,17 @@ struct ParticleArrays {
     uint32_t* boundary_distance;
     float* lam_prev;     // total lambda of the previous substep (warm-start feature); may be null
     uint32_t* prev_old;  // optional: receives the source slot of every gathered particle
+    uint32_t* sid;       // scene index of every particle in a packed batch; null for a single scene
 };
 
 void launch_integrate(int4* pos, int4* backup, float4* vel, const uint32_t* counts, uint32_t cap,
                       float last_dt, float dt, const float acc[3], hipStream_t s);
 void launch_copy_int4(int4* dst, const int4* src, const uint32_t* counts, uint32_t cap, hipStream_t s);
 void launch_hash_iota(const int4* pos, const uint32_t* d_len, uint32_t cap, const GridDesc& g,
-                      uint32_t* keys, uint32_t* vals, hipStream_t s);
+                      uint32_t* keys, uint32_t* vals, hipStream_t s, const SegmentTable* seg = nullptr);
+// seg_start[sid[i]] = i where the (cell-sorted) scene index changes.
+void launch_segment_starts(const uint32_t* sid, const uint32_t* counts, uint32_t cap, uint32_t* seg_start,
+                           hipStream_t s);
 void launch_position_code(const int4* pos, const uint32_t* d_len, uint32_t cap, uint32_t section,
                           uint32_t* out, hipStream_t s);
 void launch_permute(const uint32_t* perm, const uint32_t* counts, uint32_t cap,
@@ -41,7 +45,8 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
-                 int lanes = 8, uint32_t stride = 0, const uint32_t* gate = nullptr);
+                 int lanes = 8, uint32_t stride = 0, const uint32_t* gate = nullptr,
+                 const SegmentTable* seg = nullptr);
 void launch_walk_decide(uint32_t* counts, uint32_t pcap, hipStream_t s);
 void launch_brute(bool fill, const int4* pos, const float* kw, const uint32_t* counts, uint32_t cap,
                   float range_scale, uint32_t pcap, uint32_t* cnt, const uint32_t* row, uint32_t* col,
@@ -57,7 +62,7 @@ void launch_kernel_consts(const float* kw, const float* radius, const float* inv
 void launch_sphere(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap, const BoxSet& c,
                    hipStream_t s);
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
-                const BoxSet& boxes, hipStream_t s);
+                const BoxSet& boxes, hipStream_t s, const SegmentTable* seg = nullptr);
 void launch_count_particle_substeps(const uint32_t* counts, unsigned long long* acc, hipStream_t s);
 void launch_density_error_finish(const double* part_sum, const float* part_max, uint32_t blocks,
                                  uint32_t* counts, double* stats, hipStream_t s);
@@ -73,7 +78,7 @@ void launch_features_rows(const int4* pos, const float4* vel, const float* inv_m
                           uint32_t pcap, const SolverArgs& a, float dt, float* feat, hipStream_t s);
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
                         const BoxSet& boxes, hipStream_t s, const uint32_t* action = nullptr,
-                        const int4* saved = nullptr);
+                        const int4* saved = nullptr, const SegmentTable* seg = nullptr);
 void launch_solve_begin(uint32_t* counts, double* tol_err, uint32_t slots, hipStream_t s);
 void launch_solve_decide(uint32_t* counts, const double* tol_err, uint32_t k, float tol, bool warm, hipStream_t s);
 void launch_solve_stats(const uint32_t* counts, unsigned long long* stats, uint32_t fixed_iterations, hipStream_t s);

@@ -65,6 +65,32 @@ __global__ __launch_bounds__(BLOCK) void k_hash_iota(const int4* __restrict__ po
         if (vals) vals[i] = i;
     }
 }
+// A packed batch: the hash in the particle's own scene grid, above it the
+// scene index, so one stable sort orders by scene, then by cell.
+__global__ __launch_bounds__(BLOCK) void k_hash_iota_seg(const int4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ d_len, SegmentTable seg,
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t n = *d_len;
+    GRID_STRIDE(i, n) {
+        const uint32_t sc = seg.sid[i];
+        const GridDesc& g = seg.grids[sc];
+        v3f p = pos_world(pos[i]);
+        uint32_t gx = grid_cell(p.x, g.mn[0], g.mx[0], g.res);
+        uint32_t gy = grid_cell(p.y, g.mn[1], g.mx[1], g.res);
+        uint32_t gz = grid_cell(p.z, g.mn[2], g.mx[2], g.res);
+        keys[i] = (sc << seg.shift) | zhash(gx, gy, gz, g.res);
+        if (vals) vals[i] = i;
+    }
+}
+__global__ __launch_bounds__(BLOCK) void k_segment_starts(const uint32_t* __restrict__ sid,
+                                                          const uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ seg_start) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        const uint32_t s = sid[i];
+        if (i == 0 || sid[i - 1] != s) seg_start[s] = i;
+    }
+}
 
 // calculate_position_code.comp:23-62: bit b of the 96-bit code is bit b/3 of
 // coordinate b%3 (positions reinterpreted as uint).
@@ -106,6 +132,7 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const uint32_t* __restrict__ 
         dst.boundariness[i] = src.boundariness[o];
         dst.boundary_distance[i] = src.boundary_distance[o];
         if (src.lam_prev) dst.lam_prev[i] = src.lam_prev[o];
+        if (src.sid) dst.sid[i] = src.sid[o];
         if (src.prev_old) dst.prev_old[i] = o;
     }
 }
@@ -182,6 +209,17 @@ __global__ __launch_bounds__(BLOCK) void k_box(int4* __restrict__ pos, const flo
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) pos[i] = box_collide(i, pos[i], radius[i], boxes);
 }
+// A packed batch: each particle against its own scene's boxes, with its list
+// position inside that scene.
+__global__ __launch_bounds__(BLOCK) void k_box_seg(int4* __restrict__ pos, const float* __restrict__ radius,
+                                                   const uint32_t* __restrict__ counts, SegmentTable seg) {
+    const uint32_t n = counts[C_N];
+    GRID_STRIDE(i, n) {
+        const uint32_t sc = seg.sid[i];
+        const BoxSet& b = seg.boxes[sc];
+        if (b.n) pos[i] = box_collide(i - seg.seg_start[sc], pos[i], radius[i], b);
+    }
+}
 // sphere_collision::apply (sphere_collision.cpp): one pass over the particles.
 __global__ __launch_bounds__(BLOCK) void k_sphere(int4* __restrict__ pos, const float* __restrict__ radius,
                                                   const uint32_t* __restrict__ counts, BoxSet c) {
@@ -252,8 +290,15 @@ void launch_copy_int4(int4* dst, const int4* src, const uint32_t* counts, uint32
     LAUNCH(k_copy_int4, stride_grid(cap), dst, src, counts);
 }
 void launch_hash_iota(const int4* pos, const uint32_t* d_len, uint32_t cap, const GridDesc& g,
-                      uint32_t* keys, uint32_t* vals, hipStream_t s) {
-    LAUNCH(k_hash_iota, stride_grid(cap), pos, d_len, g, keys, vals);
+                      uint32_t* keys, uint32_t* vals, hipStream_t s, const SegmentTable* seg) {
+    if (seg && seg->sid)
+        LAUNCH(k_hash_iota_seg, stride_grid(cap), pos, d_len, *seg, keys, vals);
+    else
+        LAUNCH(k_hash_iota, stride_grid(cap), pos, d_len, g, keys, vals);
+}
+void launch_segment_starts(const uint32_t* sid, const uint32_t* counts, uint32_t cap, uint32_t* seg_start,
+                           hipStream_t s) {
+    LAUNCH(k_segment_starts, stride_grid(cap), sid, counts, seg_start);
 }
 void launch_position_code(const int4* pos, const uint32_t* d_len, uint32_t cap, uint32_t section,
                           uint32_t* out, hipStream_t s) {
@@ -279,8 +324,11 @@ void launch_kernel_consts(const float* kw, const float* radius, const float* inv
     LAUNCH(k_kernel_consts, stride_grid(cap), kw, radius, inv_mass, counts, hid, gid, gc);
 }
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
-                const BoxSet& boxes, hipStream_t s) {
-    LAUNCH(k_box, stride_grid(cap), pos, radius, counts, boxes);
+                const BoxSet& boxes, hipStream_t s, const SegmentTable* seg) {
+    if (seg && seg->sid)
+        LAUNCH(k_box_seg, stride_grid(cap), pos, radius, counts, *seg);
+    else
+        LAUNCH(k_box, stride_grid(cap), pos, radius, counts, boxes);
 }
 void launch_sphere(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap, const BoxSet& c,
                    hipStream_t s) {

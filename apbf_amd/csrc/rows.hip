@@ -551,7 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, i
                                                        const float* __restrict__ radius,
                                                        const uint32_t* __restrict__ counts, BoxSet boxes,
                                                        const uint32_t* __restrict__ action,
-                                                       const int4* __restrict__ saved) {
+                                                       const int4* __restrict__ saved, SegmentTable seg) {
     const uint32_t act = action ? *action : (uint32_t)SOLVE_APPLY;
     if (act == SOLVE_SKIP) return;
     const uint32_t n = counts[C_N];
@@ -566,7 +566,15 @@ __global__ __launch_bounds__(BLOCK) void k_solve_apply(int4* __restrict__ pos, i
             p.z = iadd(p.z, d.z);
         }
         acc[j] = make_int4(0, 0, 0, 0);
-        if (boxes.n) p = box_collide(j, p, radius[j], boxes);
+        if (boxes.n) {
+            if (seg.sid) {  // a packed batch: the particle's own scene's boxes and list position
+                const uint32_t sc = seg.sid[j];
+                const BoxSet& b = seg.boxes[sc];
+                if (b.n) p = box_collide(j - seg.seg_start[sc], p, radius[j], b);
+            } else {
+                p = box_collide(j, p, radius[j], boxes);
+            }
+        }
         if (boxes.sphere) p = sphere_collide(j, p, radius[j], boxes);
         pos[j] = p;
     }
@@ -977,9 +985,11 @@ void launch_features_rows(const int4* pos, const float4* vel, const float* inv_m
 }
 
 void launch_solve_apply(int4* pos, int4* acc, const float* radius, const uint32_t* counts, uint32_t cap,
-                        const BoxSet& boxes, hipStream_t s, const uint32_t* action, const int4* saved) {
+                        const BoxSet& boxes, hipStream_t s, const uint32_t* action, const int4* saved,
+                        const SegmentTable* seg) {
+    const SegmentTable none{nullptr, nullptr, nullptr, nullptr, 0u};
     hipLaunchKernelGGL(k_solve_apply, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, pos, acc, radius, counts, boxes,
-                       action, saved);
+                       action, saved, seg ? *seg : none);
 }
 
 // Start of a solve with a tolerance or a warm start: nothing decided yet, the

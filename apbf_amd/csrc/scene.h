@@ -28,6 +28,19 @@ struct Scene {
     uint32_t cap = 0, tcap = 0, pcap = 0, n_cells = 0;
     GridDesc grid{};
     BoxSet boxes{}, no_boxes{};
+    // A packed batch of independent scenes (segments): per-particle scene
+    // index, per-scene grids and boxes, per-scene block of 2^seg_shift cells.
+    uint32_t n_seg = 1, seg_shift = 0;
+    GridDesc* d_grids = nullptr;
+    BoxSet* d_boxes = nullptr;
+    uint32_t* seg_start = nullptr;
+    uint32_t* sid[2] = {};
+    bool packed() const { return n_seg > 1 || sid[0] != nullptr; }
+    SegmentTable seg_table() const {
+        return packed() ? SegmentTable{d_grids, d_boxes, seg_start, sid[cur], seg_shift}
+                        : SegmentTable{nullptr, nullptr, nullptr, nullptr, 0u};
+    }
+    apbf_status init_segments(uint32_t n, const apbf_segment_desc* segs);
     float last_dt = 1.0f;  // velocity_handling.h:17
     float range_scale_override = -1.0f;  // neighborhood_green::set_range_scale; < 0: the mode's default
     int cur = 0;           // current particle buffer set

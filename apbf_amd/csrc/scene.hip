@@ -51,6 +51,7 @@ ParticleArrays Scene::arrays(int set) {
     a.boundary_distance = boundary_distance[set];
     a.lam_prev = lam_prev[set];
     a.prev_old = nullptr;
+    a.sid = sid[set];
     return a;
 }
 
@@ -166,6 +167,8 @@ Scene::~Scene() {
         hipFree(kernel_width[s]); hipFree(boundariness[s]); hipFree(boundary_distance[s]); hipFree(lam_prev[s]);
         hipFree(t_source[s]); hipFree(t_target[s]); hipFree(t_time_left[s]);
     }
+    void* ps_seg[] = {d_grids, d_boxes, seg_start, sid[0], sid[1]};
+    for (void* q : ps_seg) hipFree(q);
     void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift,
                   ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_owner2, a_flag, a_scan,
@@ -189,19 +192,73 @@ void Scene::velocity_handling(float dt) {
     last_dt = dt;
 }
 
+// A packed batch: the scenes' grids (each with its own resolution; the cell
+// tables give every scene a block of 2^(3 max res) cells) and boxes.
+apbf_status Scene::init_segments(uint32_t n, const apbf_segment_desc* segs) {
+    if (n == 0 || !segs) return APBF_ERR_INVALID_ARGUMENT;
+    uint32_t max_res = 0;
+    std::vector<GridDesc> g(n);
+    std::vector<BoxSet> b(n);
+    bool any_box = false;
+    for (uint32_t k = 0; k < n; ++k) {
+        const apbf_segment_desc& d = segs[k];
+        if (d.hash_res_log2 == 0 || d.hash_res_log2 > 10 || d.n_boxes > APBF_MAX_BOXES) return APBF_ERR_INVALID_ARGUMENT;
+        max_res = std::max(max_res, d.hash_res_log2);
+        g[k].res = d.hash_res_log2;
+        std::memset(&b[k], 0, sizeof(BoxSet));
+        b[k].n = d.n_boxes;
+        any_box = any_box || d.n_boxes;
+        for (int a = 0; a < 3; ++a) {
+            g[k].mn[a] = d.hash_min[a];
+            g[k].mx[a] = d.hash_max[a];
+        }
+        for (uint32_t i = 0; i < d.n_boxes; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b[k].mn[i][a] = d.box_min[i][a];
+                b[k].mx[i][a] = d.box_max[i][a];
+            }
+    }
+    uint32_t bits = 0;
+    while ((1u << bits) < n) ++bits;
+    if (3u * max_res + bits > 31u) return APBF_ERR_CAPACITY;  // scene index above the cell hash in a u32 key
+    n_seg = n;
+    seg_shift = 3u * max_res;
+    n_cells = n << seg_shift;
+    boxes.n = any_box ? 1u : 0u;  // the per-scene boxes are in the table; this only says "run the colliders"
+    hipFree(cell_start);
+    hipFree(cell_end);
+    cell_start = cell_end = nullptr;
+    apbf_status st;
+    if ((st = dalloc(&cell_start, n_cells)) != APBF_OK || (st = dalloc(&cell_end, n_cells)) != APBF_OK ||
+        (st = dalloc(&d_grids, n)) != APBF_OK || (st = dalloc(&d_boxes, n)) != APBF_OK ||
+        (st = dalloc(&seg_start, n)) != APBF_OK || (st = dalloc(&sid[0], cap)) != APBF_OK ||
+        (st = dalloc(&sid[1], cap)) != APBF_OK)
+        return st;
+    HIP_OK(hipMemcpy(d_grids, g.data(), n * sizeof(GridDesc), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_boxes, b.data(), n * sizeof(BoxSet), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(seg_start, 0, n * 4));
+    HIP_OK(hipMemset(sid[0], 0, (size_t)cap * 4));
+    HIP_OK(hipMemset(sid[1], 0, (size_t)cap * 4));
+    return APBF_OK;
+}
+
 // ----------------------------------------------------- neighborhood_green
 // neighborhood_green.cpp:27-77: hash every particle, stable sort by hash,
-// permute the particle SoA, cell tables, pair search.
+// permute the particle SoA, cell tables, pair search. In a packed batch the
+// key holds the scene index above the hash, so the scenes stay apart and
+// every scene's particles keep their own sorted order.
 void Scene::neighborhood_green(float range_scale, bool spread) {
     hipStream_t s = stream;
-    uint32_t upper = 1u << (grid.res * 3u);
-    launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s);
+    const SegmentTable seg0 = seg_table();
+    uint32_t upper = packed() ? (n_seg << seg_shift) : (1u << (grid.res * 3u));
+    launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s, &seg0);
     radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s);
     ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
     bool tk = tbegin(K_PERMUTE);
     launch_permute(perm, counts, cap, src, dst, s);
     tend(tk);
     cur = 1 - cur;
+    if (packed()) launch_segment_starts(sid[cur], counts, cap, seg_start, s);
     // The transfer list follows its particles (indexed_list owner rebuild).
     remap_transfers_after_permute();
     fill_u32(cell_start, 0u, n_cells, s);
@@ -215,6 +272,7 @@ void Scene::neighborhood_green(float range_scale, bool spread) {
 // neighborhood_green followed by spread_kernel_width costs two walks.
 void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;
+    const SegmentTable seg = seg_table();
     if (walk_auto) {  // short rows: fewer lanes per particle fill a wave better (large pool: 22.1 vs 22.6 ms)
         refresh_seen_counts();
         walk_lanes = (seen_n != 0u && (uint64_t)seen_pairs < 62ull * (uint64_t)seen_n) ? 4 : 8;
@@ -228,7 +286,7 @@ void Scene::build_pairs(float range_scale, bool spread) {
         bool tk = tbegin(K_WALK_COUNT);
         launch_walk(0, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                     cell_end, counts, cap, range_scale, grid, pcap, cnt, u1, nullptr, nullptr, nullptr, nullptr, s,
-                    walk_lanes);
+                    walk_lanes, 0, nullptr, &seg);
         tend(tk);
         scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
         launch_pairs_total(counts, row_raw, pcap, s);
@@ -236,7 +294,7 @@ void Scene::build_pairs(float range_scale, bool spread) {
         tk = tbegin(K_WALK_FILL);
         launch_walk(1, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                     cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
-                    walk_lanes);
+                    walk_lanes, 0, nullptr, &seg);
         tend(tk);
     } else {
         // one walk into fixed-stride slots (col_tmp), counts of both lists and
@@ -244,7 +302,7 @@ void Scene::build_pairs(float range_scale, bool spread) {
         bool tk = tbegin(K_WALK_FILL);
         launch_walk(2, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                     cell_end, counts, cap, range_scale, grid, pcap, cnt, spread ? u1 : nullptr, nullptr, nullptr,
-                    col_tmp, ukw, s, walk_lanes, stride);
+                    col_tmp, ukw, s, walk_lanes, stride, nullptr, &seg);
         tend(tk);
         scan_u32(cnt, row_raw, counts + C_N_PLUS_1, cap + 1, false, nullptr, ws_mem, s);
         launch_pairs_total(counts, row_raw, pcap, s);
@@ -256,7 +314,7 @@ void Scene::build_pairs(float range_scale, bool spread) {
         if (spread) spread_reset_if_fallback();
         launch_walk(1, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
                     cell_end, counts, cap, range_scale, grid, pcap, nullptr, nullptr, row_raw, row, col, ukw, s,
-                    walk_lanes, stride, counts);
+                    walk_lanes, stride, counts, &seg);
     }
     if (spread) spread_end();
     pairs_valid = true;
@@ -279,7 +337,10 @@ void Scene::neighborhood_bf(float range_scale) {
 }
 
 // --------------------------------------------------------- box_collision
-void Scene::box_collision() { launch_box(pos[cur], radius[cur], counts, cap, boxes, stream); }
+void Scene::box_collision() {
+    const SegmentTable seg = seg_table();
+    launch_box(pos[cur], radius[cur], counts, cap, boxes, stream, &seg);
+}
 // sphere_collision::apply (spherical_pool.cpp:86-89 runs it in place of the
 // box collision)
 void Scene::sphere_collision() {
@@ -360,8 +421,9 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const 
     const bool judge = decided && !lam_override;
     if (judge) launch_solve_decide(counts, tol_err, (uint32_t)it, tol, ws_mode == 2 && ws_model && it == 0, s);
     tk = tbegin(K_SOLVE_APPLY);
+    const SegmentTable seg = seg_table();
     launch_solve_apply(pos[cur], shift, radius[cur], counts, cap, next_boxes, s,
-                       judge ? counts + C_SOLVE_ACTION : nullptr, judge ? pos_save : nullptr);
+                       judge ? counts + C_SOLVE_ACTION : nullptr, judge ? pos_save : nullptr, &seg);
     tend(tk);
 }
 
@@ -537,6 +599,55 @@ apbf_status apbf_scene_create(const apbf_scene_params* params, void** out_scene)
     return APBF_OK;
 }
 
+apbf_status apbf_scene_create_packed(const apbf_scene_params* params, uint32_t n_segments,
+                                     const apbf_segment_desc* segments, void** out_scene) {
+    if (!params || !segments || !out_scene || n_segments == 0) return APBF_ERR_INVALID_ARGUMENT;
+    *out_scene = nullptr;
+    apbf_scene_params p = *params;
+    p.hash_res_log2 = segments[0].hash_res_log2 ? segments[0].hash_res_log2 : 1u;  // replaced by the segments
+    p.n_boxes = 0;
+    void* h = nullptr;
+    apbf_status st = apbf_scene_create(&p, &h);
+    if (st != APBF_OK) return st;
+    Scene* s = static_cast<Scene*>(h);
+    st = s->init_segments(n_segments, segments);
+    if (st == APBF_OK) st = hipDeviceSynchronize() == hipSuccess ? APBF_OK : APBF_ERR_HIP;
+    if (st != APBF_OK) {
+        apbf_scene_destroy(h);
+        return st;
+    }
+    *out_scene = h;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_set_segment_ids(void* scene, const uint32_t* host_ids) {
+    if (!scene || !host_ids) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (!s->packed()) return APBF_ERR_INVALID_ARGUMENT;
+    uint32_t n = 0;
+    if (hipMemcpyAsync(&n, s->counts + apbf::C_N, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    for (uint32_t i = 0; i < n; ++i)
+        if (host_ids[i] >= s->n_seg) return APBF_ERR_INVALID_ARGUMENT;
+    if (n && hipMemcpy(s->sid[s->cur], host_ids, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_get_segment_ids(void* scene, uint32_t* host_ids) {
+    if (!scene || !host_ids) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    if (!s->packed()) return APBF_ERR_INVALID_ARGUMENT;
+    uint32_t n = 0;
+    if (hipMemcpyAsync(&n, s->counts + apbf::C_N, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return APBF_ERR_HIP;
+    if (n && hipMemcpy(host_ids, s->sid[s->cur], (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return APBF_ERR_HIP;
+    return APBF_OK;
+}
+
 apbf_status apbf_scene_destroy(void* scene) {
     if (!scene) return APBF_ERR_INVALID_ARGUMENT;
     hipDeviceSynchronize();
@@ -656,6 +767,7 @@ apbf_status apbf_scene_error_stats(void* scene, double* mean_of_means, double* m
 apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method) {
     if (!scene || method < 0 || method > 2) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
+    if (s->packed() && method != 0) return APBF_ERR_INVALID_ARGUMENT;  // packed batches: the hash grid only
     s->nb_method = method;
     scene_drop_graph(s);
     return APBF_OK;
@@ -664,6 +776,7 @@ apbf_status apbf_scene_set_neighborhood_method(void* scene, int32_t method) {
 apbf_status apbf_scene_set_sphere(void* scene, const float* center3, float radius, int32_t hollow, int32_t enable) {
     if (!scene || (enable && !center3)) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
+    if (s->packed() && enable) return APBF_ERR_INVALID_ARGUMENT;  // packed batches: boxes only
     s->boxes.sphere = enable ? 1u : 0u;
     s->boxes.hollow = hollow ? 1 : 0;
     for (int d = 0; d < 3; ++d) s->boxes.sc[d] = enable ? center3[d] : 0.0f;
@@ -691,6 +804,7 @@ apbf_status apbf_scene_set_range_scale(void* scene, float scale) {
 apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2) {
     if (!scene || !min3 || !max3 || res_log2 == 0 || res_log2 > 10) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
+    if (s->packed()) return APBF_ERR_INVALID_ARGUMENT;  // packed batches: the grids of their segments
     const uint32_t cells = 1u << (3u * res_log2);
     if (cells != s->n_cells) {  // new cell tables (outside any substep)
         if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;

@@ -35,6 +35,7 @@ enum CountSlot : uint32_t {
     C_SOLVE_ACTION = 24,   // what the current iteration's apply pass does (SolveAction)
     C_SOLVE_ITERS = 25,    // solver iterations whose shifts were applied in this substep
     C_WARM_FALLBACK = 26,  // 1 when this substep's warm start was undone (the cold start judged better)
+    C_MAXROW_LAST = 27,    // single-pass walk: longest row that outgrew its slot in the last walk (0: none)
     C_COUNT = 32
 };
 
@@ -62,6 +63,20 @@ struct BoxSet {
     int32_t hollow;
     float sc[3];
     float sr;
+};
+
+// Several independent scenes packed into one particle list (a packed batch):
+// each particle carries its scene's index (sid), the scenes' hash grids and
+// colliders are tables, every scene owns a block of 2^shift cells of the
+// cell tables, and seg_start[s] is the first list position of scene s after
+// the cell sort (list positions inside a scene, the reference's invocation
+// ids, are position - seg_start). All null / 0 for a single scene.
+struct SegmentTable {
+    const GridDesc* grids;
+    const BoxSet* boxes;
+    const uint32_t* seg_start;
+    const uint32_t* sid;
+    uint32_t shift;
 };
 
 struct SolverArgs {

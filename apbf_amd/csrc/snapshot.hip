@@ -42,6 +42,8 @@ struct apbf_snapshot {
     float *inv_mass = nullptr, *radius = nullptr, *target_radius = nullptr, *kernel_width = nullptr;
     float *boundariness = nullptr, *lam_prev = nullptr, *t_time_left = nullptr;
     uint32_t *transferring = nullptr, *boundary_distance = nullptr, *t_source = nullptr, *t_target = nullptr;
+    uint32_t* sid = nullptr;  // scene index per particle (packed batches)
+    bool packed = false;
 };
 
 namespace {
@@ -71,6 +73,7 @@ size_t carve(apbf_snapshot* s) {
     take(s->t_source, t * 4);
     take(s->t_target, t * 4);
     take(s->t_time_left, t * 4);
+    take(s->sid, s->packed ? c * 4 : 0);
     return off + 256;
 }
 
@@ -102,6 +105,7 @@ apbf_status copy_state(apbf_snapshot* snap, Scene* sc, bool to_scene) {
     cp(sc->t_source[t], snap->t_source, nt * 4);
     cp(sc->t_target[t], snap->t_target, nt * 4);
     cp(sc->t_time_left[t], snap->t_time_left, nt * 4);
+    if (snap->packed) cp(sc->sid[c], snap->sid, n * 4);
     if (to_scene) {
         sc->last_dt = snap->last_dt;
         hipLaunchKernelGGL(k_clear_pair_counts, dim3(1), dim3(64), 0, st, sc->counts);
@@ -123,6 +127,7 @@ apbf_status apbf_snapshot_create(void* scene, apbf_snapshot** out) {
     s->cap = sc->cap;
     s->tcap = sc->tcap;
     s->device = sc->device;
+    s->packed = sc->packed();
     const size_t bytes = carve(s);
     if (hipMalloc(&s->mem, bytes) != hipSuccess) {
         delete s;
@@ -142,7 +147,7 @@ apbf_status apbf_snapshot_create(void* scene, apbf_snapshot** out) {
 apbf_status apbf_snapshot_save(apbf_snapshot* snap, void* scene) {
     if (!snap || !scene) return APBF_ERR_INVALID_ARGUMENT;
     Scene* sc = static_cast<Scene*>(scene);
-    if (sc->cap != snap->cap || sc->tcap != snap->tcap) return APBF_ERR_INVALID_ARGUMENT;
+    if (sc->cap != snap->cap || sc->tcap != snap->tcap || sc->packed() != snap->packed) return APBF_ERR_INVALID_ARGUMENT;
     return copy_state(snap, sc, false);
 }
 

@@ -38,6 +38,7 @@ struct WalkArgs {
     uint32_t pcap;
     uint32_t stride;        // PAD, and FILL after PAD: nonzero (the slot stride is slot_stride())
     const uint32_t* gate;   // FILL after PAD: the count block (C_FALLBACK, C_ROWFIX); null otherwise
+    SegmentTable seg;       // a packed batch (SEG walks): each particle's grid and cell block
 };
 
 enum WalkMode { WALK_COUNT = 0, WALK_FILL = 1, WALK_PAD = 2 };
@@ -58,7 +59,7 @@ __device__ __forceinline__ uint64_t walk_group_mask() {
 // Eight waves per SIMD (64 VGPRs, a 28-byte spill) hide the latency of the
 // cell-table and position loads better than seven without the cap
 // (71 VGPRs): 8.04 against 8.24 ms per water-drop substep.
-template <int L, int MODE, bool SPREAD>
+template <int L, int MODE, bool SPREAD, bool SEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_walk_rows(
     const int4* __restrict__ pos, const float* __restrict__ kw, const float* __restrict__ radius,
     const float* __restrict__ trad, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
@@ -84,11 +85,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
     const uint32_t stride = wa.stride ? slot_stride(wa.pcap, n) : 0u;
     const int gl = threadIdx.x & (L - 1);
     const uint32_t groups = BLOCK / L;
-    const GridDesc& g = wa.g;
     for (uint32_t base = blockIdx.x * groups; base < n; base += gridDim.x * groups) {
         const uint32_t i = base + threadIdx.x / L;
         if (i >= n) continue;  // uniform within the group
         if (only_long_rows && row_f[i + 1] - row_f[i] <= stride) continue;
+        // a packed batch: the particle's own scene grid and block of cells
+        const uint32_t sc = SEG ? wa.seg.sid[i] : 0u;
+        const GridDesc& g = SEG ? wa.seg.grids[sc] : wa.g;
+        const uint32_t cbase = SEG ? (sc << wa.seg.shift) : 0u;
         const int4 pi = pos[i];
         const v3f p = pos_world(pi);
         const float kwi = kw[i];
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                     if ((uint32_t)gl < nc) {
                         const uint32_t x = lo[0] + c0 + (uint32_t)gl;
                         if (!(dyz2 + axis_d2(0, x, p.x) > thr)) {
-                            const uint32_t h = spread_bits3(x & hmask) | hyz;
+                            const uint32_t h = cbase + (spread_bits3(x & hmask) | hyz);
                             cstart = cs[h];
                             const uint32_t cend = ce[h];
                             clen = cend > cstart ? cend - cstart : 0u;
@@ -292,6 +296,7 @@ __global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap) {
     const uint32_t stride = slot_stride(pcap, counts[C_N]);
     counts[C_FALLBACK] = counts[C_PAIRS_RAW] > pcap ? 1u : 0u;  // every row exactly (clamped list)
     counts[C_ROWFIX] = counts[C_MAXROW] > stride ? 1u : 0u;     // only rows longer than their slot
+    counts[C_MAXROW_LAST] = counts[C_MAXROW];
     counts[C_MAXROW] = 0u;
 }
 
@@ -406,7 +411,7 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
-                 int lanes, uint32_t stride, const uint32_t* gate) {
+                 int lanes, uint32_t stride, const uint32_t* gate, const SegmentTable* seg) {
     WalkArgs wa;
     wa.range_scale = range_scale;
     wa.g = g;
@@ -414,34 +419,39 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
     wa.pcap = pcap;
     wa.stride = stride;
     wa.gate = gate;
+    wa.seg = seg ? *seg : SegmentTable{nullptr, nullptr, nullptr, nullptr, 0u};
+    const bool packed = seg && seg->sid;
     if (lanes != 4 && lanes != 16) lanes = WALK_LANES;
+    if (packed) lanes = WALK_LANES;  // packed batches walk with the default group width
     const dim3 grid(walk_grid(cap, (uint32_t)lanes)), block(BLOCK);
-#define WALK(L, F, S)                                                                                                \
-    hipLaunchKernelGGL((k_walk_rows<L, F, S>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
+#define WALK(L, F, S, G)                                                                                                 \
+    hipLaunchKernelGGL((k_walk_rows<L, F, S, G>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
                        cnt_f, row_raw, row_f, col, ukw)
-#define WALK_L(L)                       \
-    if (mode == WALK_PAD) {             \
-        if (spread)                     \
-            WALK(L, WALK_PAD, true);    \
-        else                            \
-            WALK(L, WALK_PAD, false);   \
-    } else if (mode == WALK_FILL) {     \
-        if (spread)                     \
-            WALK(L, WALK_FILL, true);   \
-        else                            \
-            WALK(L, WALK_FILL, false);  \
-    } else {                            \
-        if (spread)                     \
-            WALK(L, WALK_COUNT, true);  \
-        else                            \
-            WALK(L, WALK_COUNT, false); \
+#define WALK_L(L, G)                       \
+    if (mode == WALK_PAD) {                \
+        if (spread)                        \
+            WALK(L, WALK_PAD, true, G);    \
+        else                               \
+            WALK(L, WALK_PAD, false, G);   \
+    } else if (mode == WALK_FILL) {        \
+        if (spread)                        \
+            WALK(L, WALK_FILL, true, G);   \
+        else                               \
+            WALK(L, WALK_FILL, false, G);  \
+    } else {                               \
+        if (spread)                        \
+            WALK(L, WALK_COUNT, true, G);  \
+        else                               \
+            WALK(L, WALK_COUNT, false, G); \
     }
-    if (lanes == 4) {
-        WALK_L(4)
+    if (packed) {
+        WALK_L(WALK_LANES, true)
+    } else if (lanes == 4) {
+        WALK_L(4, false)
     } else if (lanes == 16) {
-        WALK_L(16)
+        WALK_L(16, false)
     } else {
-        WALK_L(WALK_LANES)
+        WALK_L(WALK_LANES, false)
     }
 #undef WALK_L
 #undef WALK

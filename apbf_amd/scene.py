@@ -254,12 +254,142 @@ class Scene:
         check(self._L.apbf_scene_diagnostics(self._h, _ptr(c), 32), "diagnostics")
         return {"particles": int(c[0]), "pairs": int(c[2]), "transfers": int(c[3]), "splits": int(c[4]),
                 "overflow_flags": int(c[6]), "pairs_before_clamp": int(c[7]), "rows": int(c[14]),
-                "peak_pairs_before_clamp": int(c[17])}
+                "peak_pairs_before_clamp": int(c[17]), "longest_outgrown_row": int(c[27]),
+                "rows_outgrew_slots": int(c[22])}
 
     def overflow_flags(self) -> int:
         f = C.c_uint32()
         check(self._L.apbf_scene_overflow_flags(self._h, C.byref(f)), "overflow_flags")
         return f.value
+
+
+def _group_key(p: dict):
+    """Scenes that can share a packed list: the same settings, flags, solver,
+    gravity and time step (grids and boxes may differ)."""
+    return (tuple(sorted(p["settings"].items())), int(p["basic_pbf"]), int(p["solver_iterations"]),
+            float(p["split_duration"]), tuple(float(g) for g in p["gravity"]), float(p.get("dt", 0.0)))
+
+
+def _split_states(st: dict, ids: np.ndarray, n_scenes: int) -> list:
+    """The states of the scenes of a packed state: each scene's particles in
+    list order, its transfers and neighbour pairs with endpoints renumbered
+    to the scene's own list."""
+    n = len(ids)
+    order = np.argsort(ids, kind="stable")
+    counts = np.bincount(ids, minlength=n_scenes)
+    start = np.concatenate([[0], np.cumsum(counts)])
+    local = np.empty(n, np.int64)
+    local[order] = np.arange(n) - start[ids[order]]
+    keys = ("position", "velocity", "inverse_mass", "radius", "pos_backup", "transferring", "target_radius",
+            "kernel_width", "boundariness", "boundary_distance")
+    sorted_fields = {k: st[k][order] for k in keys}
+    ts = np.asarray(st["t_source"], np.int64)
+    t_sc = ids[ts] if len(ts) else np.zeros(0, np.int64)
+    t_order = np.argsort(t_sc, kind="stable")
+    t_start = np.concatenate([[0], np.cumsum(np.bincount(t_sc, minlength=n_scenes))])
+    pr = st.get("pairs")
+    if pr is not None and len(pr):
+        p_sc = ids[pr[:, 0]]
+        p_order = np.argsort(p_sc, kind="stable")
+        p_start = np.concatenate([[0], np.cumsum(np.bincount(p_sc, minlength=n_scenes))])
+    out = []
+    for k in range(n_scenes):
+        a, b = start[k], start[k + 1]
+        one = {key: sorted_fields[key][a:b] for key in keys}
+        one["fluid_index"] = np.arange(b - a, dtype=np.uint32)
+        sel = t_order[t_start[k]:t_start[k + 1]]
+        one["t_source"] = local[ts[sel]].astype(np.uint32)
+        one["t_target"] = local[np.asarray(st["t_target"], np.int64)[sel]].astype(np.uint32)
+        one["t_time_left"] = st["t_time_left"][sel]
+        if pr is not None and len(pr):
+            ps = pr[p_order[p_start[k]:p_start[k + 1]]]
+            one["pairs"] = np.stack([local[ps[:, 0]], local[ps[:, 1]]], 1).astype(np.uint32)
+        else:
+            one["pairs"] = np.zeros((0, 2), np.uint32)
+        one["last_delta_time"] = st["last_delta_time"]
+        out.append(one)
+    return out
+
+
+class PackedBatch:
+    """Independent scenes stepped as packed lists (apbf_scene_create_packed):
+    scenes with the same settings share one particle list, so every stage of
+    a substep is one launch over all of them. ``cases`` is a list of
+    (params, state); the groups' capacities are the sums of their scenes'."""
+
+    def __init__(self, cases, stream: int | None = None):
+        self._L = _lib.lib()
+        groups: dict = {}
+        for k, (p, _) in enumerate(cases):
+            groups.setdefault(_group_key(p), []).append(k)
+        self.n_scenes = len(cases)
+        self.groups = []
+        for members in groups.values():
+            ps = [cases[k][0] for k in members]
+            gp = dict(ps[0])
+            for cap in ("cap_particles", "cap_transfers", "cap_pairs"):
+                gp[cap] = int(sum(int(p[cap]) for p in ps))
+            gp["boxes"] = []
+            gp["hash_res_log2"] = max(int(p["hash_res_log2"]) for p in ps)
+            segs = (_lib.SegmentDesc * len(ps))(*[_lib.make_segment(p) for p in ps])
+            cp = _lib.make_params(gp)
+            h = C.c_void_p()
+            check(self._L.apbf_scene_create_packed(C.byref(cp), len(ps), segs, C.byref(h)), "apbf_scene_create_packed")
+            sc = Scene.__new__(Scene)
+            sc._L, sc.params, sc._p, sc._h = self._L, gp, cp, h
+            if stream is not None:
+                sc.set_stream(stream)
+            states = [cases[k][1] for k in members]
+            joined = _join_states(states)
+            sc.set_state(joined)
+            ids = np.concatenate([np.full(len(st["radius"]), j, np.uint32) for j, st in enumerate(states)])
+            check(self._L.apbf_scene_set_segment_ids(sc._h, _ptr(ids)), "apbf_scene_set_segment_ids")
+            self.groups.append({"scene": sc, "members": members})
+
+    def update(self, dt: float) -> None:
+        for g in self.groups:
+            g["scene"].update(dt)
+
+    def synchronize(self) -> None:
+        import torch
+        torch.cuda.synchronize()
+
+    def scene_states(self, pairs: bool = True) -> list:
+        """Per-scene states in the order of ``cases``."""
+        out = [None] * self.n_scenes
+        for g in self.groups:
+            sc = g["scene"]
+            st = sc.get_state(pairs=pairs)
+            n = len(st["radius"])
+            ids = np.zeros(max(n, 1), np.uint32)
+            check(self._L.apbf_scene_get_segment_ids(sc._h, _ptr(ids)), "apbf_scene_get_segment_ids")
+            for k, one in zip(g["members"], _split_states(st, ids[:n].astype(np.int64), len(g["members"]))):
+                out[k] = one
+        return out
+
+    def particle_substeps(self, reset: bool = False) -> int:
+        return sum(g["scene"].particle_substeps(reset)[0] for g in self.groups)
+
+    def close(self):
+        for g in self.groups:
+            g["scene"].close()
+        self.groups = []
+
+
+def _join_states(states: list) -> dict:
+    """Scenes' states one after the other (transfer endpoints moved along)."""
+    keys = ("position", "velocity", "inverse_mass", "radius", "pos_backup", "transferring", "target_radius",
+            "kernel_width", "boundariness", "boundary_distance")
+    out = {k: np.concatenate([np.asarray(st[k]) for st in states]) for k in keys}
+    off = np.cumsum([0] + [len(st["radius"]) for st in states])[:-1]
+    out["t_source"] = np.concatenate([np.asarray(st["t_source"], np.uint32) + np.uint32(o)
+                                      for st, o in zip(states, off)]).astype(np.uint32)
+    out["t_target"] = np.concatenate([np.asarray(st["t_target"], np.uint32) + np.uint32(o)
+                                      for st, o in zip(states, off)]).astype(np.uint32)
+    out["t_time_left"] = np.concatenate([np.asarray(st["t_time_left"], np.float32) for st in states])
+    out["fluid_index"] = np.arange(len(out["radius"]), dtype=np.uint32)
+    out["last_delta_time"] = states[0].get("last_delta_time", 1.0)
+    return out
 
 
 class Batch:

@@ -1,8 +1,11 @@
 #!/usr/bin/env python3
-"""Benchmark: particle-substeps/s of the adaptive water drop (1M particles:
-900k coarse pool particles r = 1h and a 100k fine-particle drop r = 0.5h, split,
-merge and kernel-width spreading on, 4 solver iterations, dt = 1/120) on one
-MI355X, with the mean density error of the timed substeps.
+"""Benchmark: particle-substeps/s of the adaptive water drop (about 1 M
+particles from rest: a coarse pool settled into its adaptive equilibrium and a
+fine drop of 100 k particles released above it; split, merge and kernel-width
+spreading on, 4 solver iterations, dt = 1/120) on one MI355X, with the mean
+density error of the timed substeps, the particles created and deleted in
+them, the rooflines of the two largest kernels, and the learned warm start
+against the plain solver by iterations to the 0.1 % error tolerance.
 
 One step = one substep (the reference's waterdrop::update) over the whole
 scene, inputs resident in HBM. Prints one JSON line.
@@ -22,18 +25,24 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PROFILES = os.path.join(ROOT, "profiles")
 
-# Algorithmic bytes of each timeable kernel, per particle and per neighbour
-# pair of the solver's list, from the fixed byte model of the substep
-# (unpadded elements: xyz 12 B, scalars 4 B, pair entry 8 B; inputs read and
-# outputs written once; neighbour-side gathers assumed cache-served).
+# Algorithmic bytes of each timeable kernel from the fixed byte model of the
+# substep (unpadded elements: xyz 12 B, scalars 4 B, pair entry 8 B; inputs
+# read and outputs written once; neighbour-side gathers assumed cache-served):
+# (bytes per particle, per pair of the solver's list, per pair of the walk's
+# list before the spreading filter).
 BYTE_MODEL = {
     # density/lambda pass (36 B + 8 B/pair) plus the pair reads of the
     # position-delta pass (8 B/pair): the row pass carries both
-    "solve_rows": (36.0, 16.0),
+    "solve_rows": (36.0, 16.0, 0.0),
     # position-delta pass, particle part (28 B) + the next box collision (28 B)
-    "solve_apply": (56.0, 0.0),
-    "permute": (100.0, 0.0),     # permute 48 B of state: index 4 + gather 48 + write 48
-    "fsm12": (38.0, 16.0),       # split/merge find, flood fill and nearest neighbour
+    "solve_apply": (56.0, 0.0, 0.0),
+    "permute": (100.0, 0.0, 0.0),     # permute 48 B of state: index 4 + gather 48 + write 48
+    "fsm12": (38.0, 16.0, 0.0),       # split/merge find, flood fill and nearest neighbour
+    # the neighbour walk with the spreading fused: pair build 16 B/particle
+    # (position, range) + 8 B per found pair (the reference writes the list),
+    # spreading 24 B/particle (radius, target radius, old width, new width ...)
+    # + 8 B per found pair read back + 8 B per kept pair written
+    "walk_fill": (40.0, 8.0, 16.0),
 }
 
 
@@ -73,9 +82,10 @@ def calibrate_copy(n_bytes: int = 1 << 30) -> None:
     print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "launches": 3}), flush=True)
 
 
-def timed_run(params, state, steps: int, mode: str, model=None, iterations=None):
+def timed_run(params, state, steps: int, mode: str, model=None, iterations=None, tolerance=None, max_iterations=8):
     """`steps` substeps from `state` (after 2 untimed ones) with the plain solver
-    or the learned warm start; wall time, error and per-iteration errors."""
+    or the learned warm start, with a fixed iteration count or iterating to a
+    density-error tolerance; wall time, error and iterations applied."""
     import torch
     from apbf_amd import learned as Lm
     from apbf_amd.scene import Scene
@@ -85,6 +95,8 @@ def timed_run(params, state, steps: int, mode: str, model=None, iterations=None)
     s = Scene(p)
     s.set_state(state)
     s.set_error_measurement(True)
+    if tolerance is not None:
+        s.set_solver_tolerance(tolerance, max_iterations)
     if mode == "warm":
         Lm.set_mode(s, model, "warm")
     for _ in range(2):
@@ -92,25 +104,30 @@ def timed_run(params, state, steps: int, mode: str, model=None, iterations=None)
     s.iteration_errors(reset=True)
     s.error_stats(reset=True)
     s.particle_substeps(reset=True)
+    s.solver_stats(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         s.update(p["dt"])
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    before = s.iteration_errors(max_iters=p["solver_iterations"]).tolist()
     stats = s.error_stats()
     ps, _ = s.particle_substeps()
+    iters, subs, fallbacks = s.solver_stats()
     s.close()
-    return {"iterations": p["solver_iterations"], "warm_start": mode == "warm", "ms_per_step": el / steps * 1e3,
-            "particle_substeps_per_s": ps / el, "mean_density_error": stats[0], "mean_abs_density_error": stats[3],
-            "mean_error_before_iteration": before}
+    return {"warm_start": mode == "warm", "tolerance": tolerance,
+            "fixed_iterations": None if tolerance is not None else p["solver_iterations"],
+            "iterations_per_substep": iters / max(subs, 1), "warm_start_undone": fallbacks,
+            "ms_per_step": el / steps * 1e3, "particle_substeps_per_s": ps / el,
+            "mean_density_error": stats[0], "max_density_error": stats[1], "mean_abs_density_error": stats[3]}
 
 
 def learned_comparison(scene, params, args, log):
     """Harvest (feature, total-lambda) pairs from plain substeps, train the
     warm-start network on the GPU, then run the plain solver and the warm
-    start from the same state."""
+    start from the same state, each iterating until the mean density error of
+    the positions an iteration starts from is within the tolerance: the
+    iterations each needs per substep are the comparison."""
     from apbf_amd import learned as Lm
     model, rep = Lm.fit_warm_start(scene, params["dt"], harvest_substeps=args.learned_harvest,
                                    epochs=args.learned_epochs, damping=args.learned_damping)
@@ -119,41 +136,33 @@ def learned_comparison(scene, params, args, log):
     st = scene.get_state(pairs=False)
     st["fluid_index"] = st["fluid_index"][: len(st["radius"])]
     steps = max(10, min(args.steps, 30))
-    iters = params["solver_iterations"]
-    plain = timed_run(params, st, steps, "off")
-    warm = timed_run(params, st, steps, "warm", model, iterations=args.learned_iterations)
-    log(f"plain {plain['ms_per_step']:.2f} ms err {plain['mean_density_error']:.2e}; "
-        f"learned {warm['ms_per_step']:.2f} ms err {warm['mean_density_error']:.2e}")
-    # iterations to the plain solver's error: every plain iteration count below
-    # the workload's and every warm-started one up to it, from the same state
-    table = {"plain": {iters: plain}, "learned": {args.learned_iterations: warm}}
-    for k in range(1, iters + 1):
-        if k not in table["plain"]:
-            table["plain"][k] = timed_run(params, st, steps, "off", iterations=k)
-        if k not in table["learned"]:
-            table["learned"][k] = timed_run(params, st, steps, "warm", model, iterations=k)
-    target = plain["mean_density_error"]
-    plain_k = min(k for k, r in table["plain"].items() if r["mean_density_error"] <= target)
-    warm_ks = [k for k, r in table["learned"].items() if r["mean_density_error"] <= target]
-    warm_k = min(warm_ks) if warm_ks else None
-    log("iterations to the plain error %.2e: plain %d, learned %s" % (target, plain_k, warm_k))
+    out = {"steps": steps, "training": rep, "by_tolerance": {}}
+    for tol in (1e-3, 1e-4):
+        plain = timed_run(params, st, steps, "off", tolerance=tol)
+        warm = timed_run(params, st, steps, "warm", model, tolerance=tol)
+        ratio = plain["iterations_per_substep"] / warm["iterations_per_substep"] \
+            if warm["iterations_per_substep"] > 0 else None
+        log(f"tolerance {tol:g}: plain {plain['iterations_per_substep']:.2f} iterations/substep "
+            f"({plain['ms_per_step']:.2f} ms, err {plain['mean_density_error']:.2e}); learned "
+            f"{warm['iterations_per_substep']:.2f} ({warm['ms_per_step']:.2f} ms, err "
+            f"{warm['mean_density_error']:.2e}, undone {warm['warm_start_undone']})")
+        out["by_tolerance"][f"{tol:g}"] = {"plain": plain, "learned": warm, "iteration_ratio": ratio}
     model.close()
-    brief = {side: {str(k): {"mean_density_error": r["mean_density_error"],
-                             "mean_abs_density_error": r["mean_abs_density_error"], "ms_per_step": r["ms_per_step"]}
-                    for k, r in sorted(rows.items())} for side, rows in table.items()}
-    return {"steps": steps, "training": rep, "plain": plain, "learned": warm, "tolerance": 1e-3,
-            "by_iterations": brief, "plain_error_target": target,
-            "plain_iterations_to_target": plain_k, "learned_iterations_to_target": warm_k,
-            "iteration_ratio": (plain_k / warm_k) if warm_k else None}
+    head = out["by_tolerance"]["0.001"]
+    out.update(tolerance=1e-3, plain_iterations_to_tolerance=head["plain"]["iterations_per_substep"],
+               learned_iterations_to_tolerance=head["learned"]["iterations_per_substep"],
+               iteration_ratio=head["iteration_ratio"])
+    return out
 
 
 def run_batch(args, log):
     """64 independent scenes (10k-500k particles, log-uniform) stepped as one
-    batch: eager (every scene's substep enqueued on its own stream) and as a
-    captured two-substep graph of all scenes."""
+    batch: eager (every scene's substep enqueued on its own stream), as a
+    captured two-substep graph of all scenes, and packed (the scenes of each
+    settings group in one particle list, one launch per stage)."""
     import torch
     from apbf_amd import config as cfg
-    from apbf_amd.scene import Batch, Scene
+    from apbf_amd.scene import Batch, PackedBatch, Scene
     cases = cfg.batch_scenes(64)
     dt = cases[0][0]["dt"]
     scenes = []
@@ -187,10 +196,30 @@ def run_batch(args, log):
     batch.synchronize()
     graph_s = time.perf_counter() - t0
     graph_ps = sum(s.particle_substeps(reset=True)[0] for s in scenes)
-    log(f"eager {eager_s / steps * 1e3:.2f} ms/substep, graph {graph_s / steps * 1e3:.2f} ms/substep")
+    batch.close()
+    for s in scenes:
+        s.close()
+    # packed: the scenes of each settings group in one particle list
+    packed_batch = PackedBatch(cases)
+    for _ in range(args.warmup):
+        packed_batch.update(dt)
+    packed_batch.synchronize()
+    packed_batch.particle_substeps(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        packed_batch.update(dt)
+    packed_batch.synchronize()
+    packed_s = time.perf_counter() - t0
+    packed_ps = packed_batch.particle_substeps(reset=True)
+    n_lists = len(packed_batch.groups)
+    packed_batch.close()
+    log(f"eager {eager_s / steps * 1e3:.2f} ms/substep, graph {graph_s / steps * 1e3:.2f} ms/substep, "
+        f"packed {packed_s / steps * 1e3:.2f} ms/substep ({n_lists} lists)")
     eager = {"value": eager_ps / eager_s, "ms_per_step": eager_s / steps * 1e3}
     graph = {"value": graph_ps / graph_s, "ms_per_step": graph_s / steps * 1e3}
-    best, mode = (eager, "eager streams") if eager["value"] >= graph["value"] else (graph, "graph replay")
+    packed = {"value": packed_ps / packed_s, "ms_per_step": packed_s / steps * 1e3, "lists": n_lists}
+    best, mode = max(((eager, "eager streams"), (graph, "graph replay"), (packed, "packed lists")),
+                     key=lambda m: m[0]["value"])
     out = {
         "metric": "particle-substeps/s, 64 independent scenes of 10k-500k particles stepped as one batch",
         "value": best["value"], "unit": "particle-substeps/s", "n_gpus": 1, "steps": steps,
@@ -198,25 +227,29 @@ def run_batch(args, log):
         "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic",
         "config": {"workload": "batch64", "scenes": len(scenes), "particles": n_total,
                    "kinds": "alternating plain dam break / adaptive water drop", "seed": 0, "mode": mode},
-        "eager": eager, "graph": graph,
+        "eager": eager, "graph": graph, "packed": packed,
     }
-    batch.close()
-    for s in scenes:
-        s.close()
     print(json.dumps(out), flush=True)
 
 
 def workload(name: str):
+    """(params, state, setup report) of a named workload."""
     from apbf_amd import config as cfg
     if name == "water_drop":
-        return cfg.water_drop()
+        return cfg.settled_water_drop()
+    if name == "water_drop_lattice":  # the unsettled lattice start (parity tests use it)
+        p, st = cfg.water_drop()
+        return p, st, {}
+    p, st = None, None
     if name == "dam_break":
-        return cfg.dam_break(n_side=32)
-    if name == "small_water_drop":
-        return cfg.small_water_drop()
-    if name == "large_pool":
-        return cfg.large_pool()
-    raise SystemExit(f"unknown workload {name}")
+        p, st = cfg.dam_break(n_side=32)
+    elif name == "small_water_drop":
+        p, st = cfg.small_water_drop()
+    elif name == "large_pool":
+        p, st = cfg.large_pool()
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return p, st, {}
 
 
 def cpu_baseline(params, state, seconds: float):
@@ -249,14 +282,13 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="water_drop")
-    ap.add_argument("--timed-kernel", default="solve_rows", choices=sorted(BYTE_MODEL))
+    ap.add_argument("--timed-kernel", default=None, choices=sorted(BYTE_MODEL),
+                    help="time only this kernel (default: the neighbour walk and the solver row pass)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's solver iterations (studies of iterations to tolerance)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
-    ap.add_argument("--learned-iterations", type=int, default=1,
-                    help="Jacobi iterations after the learned warm start")
     ap.add_argument("--learned-damping", type=float, default=1.0, help="scale of the predicted lambdas")
     ap.add_argument("--learned-epochs", type=int, default=12)
     ap.add_argument("--learned-harvest", type=int, default=6, help="plain substeps harvested for training")
@@ -286,19 +318,20 @@ def main():
     if args.workload == "batch64":
         run_batch(args, lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
         return
-    params, state = workload(args.workload)
+    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    torch.cuda.init()
+    t_setup = time.perf_counter()
+    params, state, setup = workload(args.workload)
     if args.iterations is not None:
         params["solver_iterations"] = args.iterations
     if args.hash_res is not None:
         params["hash_res_log2"] = args.hash_res
     dt = params["dt"]
-    torch.cuda.init()
     scene = Scene(params)
     scene.set_state(state)
     scene.set_error_measurement(True)
-
-    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
-    log(f"scene {params.get('workload')} with {len(state['radius'])} particles ready")
+    log(f"scene {params.get('workload')} with {len(state['radius'])} particles ready "
+        f"({time.perf_counter() - t_setup:.1f} s of set-up, not timed)")
     shape = "walk 8 lanes (4 when rows average under 62 entries), solver 16 lanes (library default)"
     if args.launch_shape:
         w, v = (int(x) for x in args.launch_shape.split(","))
@@ -312,10 +345,12 @@ def main():
         shape = f"walk {w}, solver {v} lanes (autotuned)"
     torch.cuda.synchronize()
     log("warmup done")
+    n_start = scene.counts()[0]
     scene.iteration_errors(reset=True)
     scene.error_stats(reset=True)
-    scene.particle_substeps(reset=True)
-    scene.time_kernel(args.timed_kernel, max_launches=args.steps * (params["solver_iterations"] + 2))
+    scene.substep_totals(reset=True)
+    timed = (args.timed_kernel,) if args.timed_kernel else ("walk_fill", "solve_rows")
+    scene.time_kernels(timed, max_launches=args.steps * (params["solver_iterations"] + 2))
 
     if args.graph:  # timing through events is not captured
         scene.time_kernel(None)
@@ -333,9 +368,24 @@ def main():
 
     elapsed = t1 - t0
     log(f"timed {args.steps} substeps in {elapsed:.3f} s")
-    ps, pairs = scene.particle_substeps()
-    k_ms, k_launches = scene.kernel_time()
+    tot = scene.substep_totals()
+    rooflines = {}
+    for name in timed:
+        k_ms, k_launches = scene.kernel_time_of(name)
+        per_p, per_pair, per_raw = BYTE_MODEL[name]
+        b = (per_p * tot["particles"] + per_pair * tot["pairs"] + per_raw * tot["raw_pairs"]) / max(args.steps, 1)
+        per_launch = b / (k_launches / args.steps) if k_launches else None
+        avg_s = (k_ms / 1e3) / k_launches if k_launches else None
+        ach = per_launch / avg_s / 1e9 if k_launches else None
+        rooflines[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": (ach / HBM_PEAK_GBS) if ach else None,
+                           "traffic": committed_traffic(name, args.workload),
+                           "launches_per_step": k_launches / args.steps if args.steps else 0,
+                           "avg_launch_ms": avg_s * 1e3 if avg_s else None,
+                           "ms_per_step": k_ms / args.steps if args.steps else None,
+                           "algorithmic_bytes_per_launch": per_launch}
     scene.time_kernel(None)
+    dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"] or 0.0)
     iters = params["solver_iterations"]
     before_it = scene.iteration_errors(max_iters=max(iters, 1)).tolist()[:iters]
     mean_err, max_err, last_err, mean_abs, err_steps = scene.error_stats()
@@ -344,16 +394,8 @@ def main():
     if diag["overflow_flags"]:
         log(f"WARNING: capacity clamp hit (flags {diag['overflow_flags']}, peak pairs "
             f"{diag['peak_pairs_before_clamp']} of {params['cap_pairs']})")
-    value = ps / elapsed
-
-    # roofline of the timed kernel: algorithmic bytes per launch / mean launch time
-    per_p, per_pair = BYTE_MODEL[args.timed_kernel]
-    launches_per_step = k_launches / args.steps if args.steps else 0
-    mean_n = ps / args.steps
-    mean_pairs = pairs / args.steps
-    bytes_per_launch = per_p * mean_n + per_pair * mean_pairs
-    avg_launch_s = (k_ms / 1e3) / k_launches if k_launches else float("nan")
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else None
+    value = tot["particles"] / elapsed
+    mean_n = tot["particles"] / args.steps
 
     learned = None
     if not args.no_learned:
@@ -380,15 +422,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": params.get("workload", args.workload),
-            "particles_start": int(len(state["radius"])),
+            "setup": setup,
+            "particles_start": int(n_start),
             "particles_end": int(n_now),
             "mean_particles": mean_n,
-            "mean_pairs": mean_pairs,
+            "particles_created": tot["created"],
+            "particles_deleted": tot["deleted"],
+            "mean_pairs": tot["pairs"] / args.steps,
+            "mean_walk_pairs_before_spreading": tot["raw_pairs"] / args.steps,
             "transfers_end": int(nt_now),
-            "solver_iterations": params["solver_iterations"],
+            "solver_iterations": iters,
             "dt": dt,
+            "merge_duration_s": params["settings"]["mMergeDuration"],
             "hash_res_log2": params["hash_res_log2"],
-            "solver": "plain (cold-start lambda)",
+            "solver": "plain (cold-start lambda), fixed iteration count",
             "launch_shape": shape,
             "peak_walk_pairs": diag["peak_pairs_before_clamp"],
             "pair_capacity": params["cap_pairs"],
@@ -396,19 +443,10 @@ def main():
         },
         "density_error": {"mean_of_substep_means": mean_err, "max": max_err,
                           "mean_abs_of_substep_means": mean_abs, "substeps": err_steps,
-                          "mean_before_iteration": before_it},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": args.timed_kernel,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": committed_traffic(args.timed_kernel, args.workload),
-            "launches_per_step": launches_per_step,
-            "avg_launch_ms": avg_launch_s * 1e3 if k_launches else None,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-        },
+                          "mean_before_iteration": before_it, "tolerance": 1e-3,
+                          "tolerance_met": bool(mean_err <= 1e-3)},
+        "roofline": rooflines[dominant],
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "plain_vs_learned": learned,
     }

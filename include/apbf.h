@@ -208,7 +208,9 @@ apbf_status apbf_scene_substep_totals(void* scene, uint64_t* out, uint32_t n_out
 /* Copies the first n_words words of the scene's device-resident count block:
  * 0 particles, 2 neighbour pairs, 3 transfers, 4 splits, 6 overflow flags,
  * 7 neighbour-walk pairs before the capacity clamp, 14 particles covered by
- * the neighbour rows, 17 largest pre-clamp pair count seen. Synchronises. */
+ * the neighbour rows, 17 largest pre-clamp pair count seen, 22 whether rows
+ * outgrew their slots in the last single-pass walk, 27 the longest of them.
+ * Synchronises. */
 apbf_status apbf_scene_diagnostics(void* scene, uint32_t* host_counts, uint32_t n_words);
 /* Overflow flags raised by capacity clamps since the last call (bit 0 pairs,
  * bit 1 transfers, bit 2 particles); cleared by the call. Synchronises. */
@@ -510,6 +512,35 @@ size_t apbf_neighbor_rows_from_pairs_workspace_size(uint32_t pair_capacity, uint
 apbf_status apbf_neighbor_rows_from_pairs(const uint32_t* pairs, const uint32_t* d_n_pairs, uint32_t pair_capacity,
                                           uint32_t rows, uint32_t* row_offset, uint32_t* neighbors, void* workspace,
                                           size_t workspace_bytes, apbf_stream stream);
+
+/* ---------------------------------------------------------- packed batches */
+/* Many independent scenes with the same settings (apbf_settings, basicPbf,
+ * solverIterations, splitDuration, gravity) stepped as one particle list:
+ * every stage is one launch over all of them. No counterpart in the
+ * reference, which runs one scene per process (main.cpp:62-68). Each
+ * particle carries its scene's index; the hash key holds that index above
+ * the cell hash of the scene's own grid, so one stable sort orders the list
+ * by scene, then by cell, and the neighbour walk stays inside the scene's
+ * block of cells; the box collision takes the scene's boxes and the
+ * particle's list position inside its scene. Every scene's particles are
+ * ordered, searched, solved, split and merged as if the scene ran alone
+ * (bit for bit as long as no capacity clamps; the capacities of `params` are
+ * shared by all scenes). Neighbour search: the hash grid only; no sphere. */
+typedef struct apbf_segment_desc {
+    float hash_min[3];
+    float hash_max[3];
+    uint32_t hash_res_log2;
+    uint32_t n_boxes;
+    float box_min[APBF_MAX_BOXES][4];
+    float box_max[APBF_MAX_BOXES][4];
+} apbf_segment_desc;
+/* params' hash domain and boxes are ignored (they come from the segments). */
+apbf_status apbf_scene_create_packed(const apbf_scene_params* params, uint32_t n_segments,
+                                     const apbf_segment_desc* segments, void** out_scene);
+/* Scene index of every particle, in the order of the state given to
+ * apbf_scene_set_state (call after it); reading gives the current order. */
+apbf_status apbf_scene_set_segment_ids(void* scene, const uint32_t* host_ids);
+apbf_status apbf_scene_get_segment_ids(void* scene, uint32_t* host_ids);
 
 /* ------------------------------------------ graphs, batches, launch shapes */
 /* No counterpart in the reference, which records one command buffer with a
