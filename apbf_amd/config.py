@@ -228,7 +228,7 @@ def _join(a: dict, b: dict) -> dict:
 
 def settled_water_drop(pool_counts=(180, 50, 180), drop_particles: int = 100_000, settle: int = 300,
                        drop_settle: int = 100, gap: float = 3.0, merge_duration: float = 0.05, seed: int = 0,
-                       iterations: int = 4, res: int = 6, pairs_per_particle: int = 300):
+                       iterations: int = 4, res: int = 6, pairs_per_particle: int = 800):
     """The adaptive water drop from rest (runs the library on the GPU).
 
     The coarse pool of :func:`water_drop` first runs `settle` substeps on its
@@ -242,7 +242,12 @@ def settled_water_drop(pool_counts=(180, 50, 180), drop_particles: int = 100_000
     workload at their rest density, so its density error is the solver's
     steady error, and particles split and merge in every substep. Merges take
     `merge_duration` seconds (6 substeps at dt = 1/120; the reference's
-    default is 2 s). Returns (params, state, report)."""
+    default is 2 s). The pair capacity of 800 per particle makes the
+    neighbour walk's fixed-stride slots (capacity / current particles, about
+    1 200 entries) longer than the longest rows (820 entries), so no row
+    needs the exact refill (7.14 against 8.17 ms per substep at 300, whose
+    slots of 450 entries leave some rows to the refill every substep).
+    Returns (params, state, report)."""
     from .scene import Scene
     p, st = water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, iterations=iterations,
                        res=res, pairs_per_particle=pairs_per_particle, merge_duration=merge_duration,
@@ -414,26 +419,34 @@ def spherical_pool_scene(center=(0.0, 10.0, -60.0), pool_radius: float = 100.0, 
     return p, st
 
 
-def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: float = 0.4675,
+def large_pool(box=(200.0, 100.0, 100.0), n_target: int = 8_000_000, spacing: float | None = None,
                kernel_width: float = 1.0, impulse_width: float = 20.0, impulse_speed: float = 4.0,
-               seed: int = 0, iterations: int = 3, res: int = 7, pairs_per_particle: int = 128):
-    """Large uniform pool, plain PBF: particles of radius kernel_width/4 on a
-    lattice at `spacing` (jitter +-0.05h) filling the box floor to the depth
-    that holds ~n_target particles (427 x 88 x 213 = 8 003 688 at the defaults,
-    depth ~41h), so a kernel-width ball holds (4/3) pi (h/s)^3 ~ 41 lattice
-    sites. A wave impulse: the slab x < impulse_width moves at impulse_speed
-    along +x. The velocity is inferred from the displacement over the last
-    substep (velocity_handling.cpp:26), so it enters as pos_backup = pos - v dt
-    with last_delta_time = dt. The hash domain is the cube [0, max(box)]^3 at
-    2^res cells per axis (cells 1.5625h at the defaults). 128 pairs per particle
-    of capacity keep almost every row inside its slot of the neighbour walk
-    (64: 23.03 against 22.82 ms per substep, same states)."""
+               seed: int = 0, iterations: int = 3, res: int = 8, pairs_per_particle: int = 128,
+               jitter: float = 0.01):
+    """Large uniform pool, plain PBF, at rest density: particles of radius
+    r = kernel_width/4 and inverse mass 1/(2r)^3 (initialize.cpp:25) on a
+    lattice at spacing 2.068 r (jitter +-0.01h), filling the box floor to the
+    depth that holds ~n_target particles (386 x 107 x 193 = 7 971 286 at the
+    defaults, depth 55h). The reference's own box lattice at spacing 2r starts
+    5.4% compressed under the Gaussian kernel (the oracle's density of a
+    perfect lattice); at 2.068 r an interior particle's density is the rest
+    density, and the mean compression error of the jittered lattice is 0.07%.
+    A wave impulse: the slab x < impulse_width moves at impulse_speed along
+    +x. The velocity is inferred from the displacement over the last substep
+    (velocity_handling.cpp:26), so it enters as pos_backup = pos - v dt with
+    last_delta_time = dt. The hash domain is the cube [0, max(box)]^3 at
+    2^res cells per axis (cells 0.78h at the defaults, so a particle's search
+    box spans 3-4 cells per axis instead of 3 cells of 1.56h at 2^7).
+    128 pairs per particle of capacity keep every row inside its slot of the
+    neighbour walk."""
     rng = np.random.default_rng(seed)
     r = kernel_width / KERNEL_SCALE
+    if spacing is None:
+        spacing = 2.068 * r
     nx = int(box[0] / spacing)
     nz = int(box[2] / spacing)
     ny = max(1, int(round(n_target / (nx * nz))))
-    pos = lattice((0.0, 0.0, 0.0), (nx, ny, nz), spacing, 0.05, rng)
+    pos = lattice((0.0, 0.0, 0.0), (nx, ny, nz), spacing, jitter * kernel_width, rng)
     n = len(pos)
     st = make_state(pos, np.full(n, r, np.float32))
     dt = 1.0 / 120.0

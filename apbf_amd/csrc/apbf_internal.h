@@ -52,10 +52,20 @@ void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* d_len, uint32_t
 void scan_u32_n(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint32_t* total,
                 void* ws, hipStream_t s);
 
-size_t sort_workspace_bytes(uint32_t capacity);
+// Digit width (4, 8 or 11 bits) and values per thread of a tile (4, 8 or
+// 16; 256 threads) of the radix sort. Every shape gives the same stable
+// order; they differ in passes and launch sizes.
+struct SortShape {
+    uint32_t bits;
+    uint32_t items;
+};
+constexpr SortShape SORT_DEFAULT{8u, 8u};
+bool sort_shape_valid(SortShape shape);
+size_t sort_workspace_bytes(uint32_t capacity, SortShape shape = SORT_DEFAULT);
+size_t sort_workspace_bytes_any(uint32_t capacity);  // enough for every shape
 void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, const uint32_t* d_len,
                       uint32_t capacity, uint32_t upper_bound, uint32_t* out_keys,
-                      uint32_t* out_vals, void* ws, hipStream_t s);
+                      uint32_t* out_vals, void* ws, hipStream_t s, SortShape shape = SORT_DEFAULT);
 
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void fill_u32_dev(uint32_t* p, uint32_t value, const uint32_t* d_len, uint32_t capacity,

@@ -177,11 +177,11 @@ void Scene::neighborhood_bs(float range_scale) {
     // chained stable sorts by code sections 0, 1, 2 (:44-48)
     iota_u32(vals, counts + C_N, cap, s);
     launch_code_gather(pos[cur], nullptr, counts, cap, 0u, keys, s);
-    radix_sort_pairs(keys, vals, counts + C_N, cap, 0xFFFFFFFFu, skeys, perm, ws_mem, s);
+    radix_sort_pairs(keys, vals, counts + C_N, cap, 0xFFFFFFFFu, skeys, perm, ws_mem, s, sort_shape);
     launch_code_gather(pos[cur], perm, counts, cap, 1u, keys, s);
-    radix_sort_pairs(keys, perm, counts + C_N, cap, 0xFFFFFFFFu, skeys, vals, ws_mem, s);
+    radix_sort_pairs(keys, perm, counts + C_N, cap, 0xFFFFFFFFu, skeys, vals, ws_mem, s, sort_shape);
     launch_code_gather(pos[cur], vals, counts, cap, 2u, keys, s);
-    radix_sort_pairs(keys, vals, counts + C_N, cap, 0xFFFFFFFFu, skeys, perm, ws_mem, s);
+    radix_sort_pairs(keys, vals, counts + C_N, cap, 0xFFFFFFFFu, skeys, perm, ws_mem, s, sort_shape);
     // particles into code order (:50-51)
     ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
     bool tk = tbegin(K_PERMUTE);

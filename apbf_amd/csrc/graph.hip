@@ -190,6 +190,24 @@ apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t
     return APBF_OK;
 }
 
+apbf_status apbf_scene_set_sort_shape(void* scene, int32_t digit_bits, int32_t items_per_thread) {
+    if (!scene || digit_bits <= 0 || items_per_thread <= 0) return APBF_ERR_INVALID_ARGUMENT;
+    const SortShape sh{(uint32_t)digit_bits, (uint32_t)items_per_thread};
+    if (!sort_shape_valid(sh)) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->sort_shape = sh;
+    drop_graph(s);
+    return APBF_OK;
+}
+
+apbf_status apbf_scene_sort_shape(void* scene, int32_t* digit_bits, int32_t* items_per_thread) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    const Scene* s = static_cast<const Scene*>(scene);
+    if (digit_bits) *digit_bits = (int32_t)s->sort_shape.bits;
+    if (items_per_thread) *items_per_thread = (int32_t)s->sort_shape.items;
+    return APBF_OK;
+}
+
 apbf_status apbf_scene_set_long_rows(void* scene, int32_t mode) {
     if (!scene || mode < -1 || mode > 1) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
@@ -266,6 +284,37 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
                     bw = walks[a];
                     bs = solves[b];
                 }
+    // The hash sort's shape, timed alone on the keys of one substep from the
+    // saved state (the sort is a small part of a substep, below the noise of
+    // substep timings). Sorting the hash keys again only rewrites the sorted
+    // copies, which the next neighbourhood build recomputes.
+    SortShape best_sort = s->sort_shape;
+    if (st == APBF_OK) st = apbf_snapshot_restore(snap, scene);
+    if (st == APBF_OK) {
+        s->walk_lanes = bw;
+        s->solve_lanes = bs;
+        s->update(dt);
+        const uint32_t upper = s->packed() ? (s->n_seg << s->seg_shift) : (1u << (s->grid.res * 3u));
+        float best_sort_ms = 1e30f;
+        for (uint32_t bits : {4u, 8u, 11u})
+            for (uint32_t items : {4u, 8u, 16u}) {
+                const SortShape sh{bits, items};
+                float ms = 0.0f;
+                radix_sort_pairs(s->keys, s->vals, s->counts + C_N, s->cap, upper, s->skeys, s->perm, s->ws_mem,
+                                 s->stream, sh);
+                if (hipEventRecord(e0, s->stream) != hipSuccess) st = APBF_ERR_HIP;
+                for (int k = 0; k < 5; ++k)
+                    radix_sort_pairs(s->keys, s->vals, s->counts + C_N, s->cap, upper, s->skeys, s->perm, s->ws_mem,
+                                     s->stream, sh);
+                if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    st = APBF_ERR_HIP;
+                if (st == APBF_OK && ms < best_sort_ms) {
+                    best_sort_ms = ms;
+                    best_sort = sh;
+                }
+            }
+    }
     // put the scene back even after a failed trial
     const apbf_status back = apbf_snapshot_restore(snap, scene);
     if (st == APBF_OK) st = back;
@@ -281,6 +330,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     s->walk_lanes = st == APBF_OK ? bw : walk0;
     s->walk_auto = st == APBF_OK ? false : auto0;
     s->solve_lanes = st == APBF_OK ? bs : solve0;
+    if (st == APBF_OK) s->sort_shape = best_sort;
     drop_graph(s);
     if (st != APBF_OK) return st;
     if (best_walk) *best_walk = bw;

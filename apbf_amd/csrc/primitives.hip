@@ -12,13 +12,16 @@
 // it plus the scan of its own values. Wave-level inclusive scans use
 // __shfl_up over the 64 lanes; the four waves of a block combine through LDS.
 //
-// Radix sort: LSD on 8-bit digits (half the passes of the reference's 4-bit
-// digits). Per pass: (1) per-tile digit histograms written bucket-major
-// [digit][tile], (2) one exclusive scan of that table gives every (digit, tile)
-// its global output offset, (3) a stable scatter: each tile walks its values in
-// eight 256-wide rounds in index order; inside a round a value's rank among
-// equal digits comes from a 64-lane match built from eight __ballot masks, and
+// Radix sort: LSD on 4-, 8- (default: half the passes of the reference's
+// 4-bit digits) or 11-bit digits, tiles of 4, 8 or 16 values per thread.
+// Per pass: (1) per-tile digit histograms written bucket-major [digit][tile],
+// (2) one exclusive scan of that table gives every (digit, tile) its global
+// output offset, (3) a stable scatter: each tile walks its values in 256-wide
+// rounds in index order; inside a round a value's rank among equal digits
+// comes from a 64-lane match built from one __ballot mask per digit bit, and
 // per-wave digit counts in LDS give the cross-wave and cross-round offsets.
+#include <algorithm>
+
 #include "apbf_internal.h"
 
 namespace apbf {
@@ -132,27 +135,37 @@ void scan_u32_n(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, u
 }
 
 // ------------------------------------------------------------------ radix sort
+template <int BITS, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ d_len,
                                                      uint32_t shift, uint32_t tiles_cap,
                                                      uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[RADIX];
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t h[R];
     const uint32_t n = *d_len;
     const uint32_t tile = blockIdx.x;
-    h[threadIdx.x] = 0;
+    for (uint32_t b = threadIdx.x; b < R; b += BLOCK) h[b] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)tile * SORT_TILE;
+    const uint64_t base = (uint64_t)tile * (ITEMS * BLOCK);
     if (base < n) {
 #pragma unroll
-        for (int r = 0; r < SORT_ITEMS; ++r) {
+        for (int r = 0; r < ITEMS; ++r) {
             uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
-            if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+            if (i < n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
         }
     }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * tiles_cap + tile] = h[threadIdx.x];
+    for (uint32_t b = threadIdx.x; b < R; b += BLOCK) hist[(uint64_t)b * tiles_cap + tile] = h[b];
 }
 
+// A tile's values in ITEMS rounds of 256 in index order. In a round, a
+// value's rank among the equal digits of its wave is a 64-lane match built
+// from BITS ballots; the first lane of each digit group publishes the
+// group's size per wave in LDS, every value adds the groups of the waves
+// before it and the running offset of its digit, and the group leaders then
+// advance the running offsets and clear their entries (no pass over all
+// 2^BITS digits per round).
+template <int BITS, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const uint32_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ vals,
                                                         const uint32_t* __restrict__ d_len,
@@ -160,35 +173,36 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const uint32_t* __restri
                                                         const uint32_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ out_keys,
                                                         uint32_t* __restrict__ out_vals) {
-    __shared__ uint32_t running[RADIX];
-    __shared__ uint32_t wcnt[4][RADIX];
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t running[R];
+    __shared__ uint32_t wcnt[4][R];
     const uint32_t n = *d_len;
     const uint32_t tile = blockIdx.x;
-    const uint64_t base = (uint64_t)tile * SORT_TILE;
+    const uint64_t base = (uint64_t)tile * (ITEMS * BLOCK);
     if (base >= n) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offsets[(uint64_t)threadIdx.x * tiles_cap + tile];
+    for (uint32_t b = threadIdx.x; b < R; b += BLOCK) {
+        running[b] = offsets[(uint64_t)b * tiles_cap + tile];
+        wcnt[0][b] = wcnt[1][b] = wcnt[2][b] = wcnt[3][b] = 0;
+    }
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int r = 0; r < SORT_ITEMS; ++r) {
-        wcnt[0][threadIdx.x] = 0;
-        wcnt[1][threadIdx.x] = 0;
-        wcnt[2][threadIdx.x] = 0;
-        wcnt[3][threadIdx.x] = 0;
-        __syncthreads();
+    __syncthreads();
+    for (int r = 0; r < ITEMS; ++r) {
         uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
         bool valid = i < n;
         uint32_t key = valid ? keys[i] : 0u;
         uint32_t val = valid ? vals[i] : 0u;
-        uint32_t d = (key >> shift) & (RADIX - 1);
+        uint32_t d = (key >> shift) & (R - 1);
         uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < RADIX_BITS; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             uint64_t bb = __ballot((d >> b) & 1u);
             match &= ((d >> b) & 1u) ? bb : ~bb;
         }
-        uint32_t rank = (uint32_t)__popcll(match & lt_mask);
-        uint32_t cnt = (uint32_t)__popcll(match);
-        if (valid && rank == 0) wcnt[wave][d] = cnt;
+        const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
+        const uint32_t cnt = (uint32_t)__popcll(match);
+        const bool leader = valid && rank == 0;
+        if (leader) wcnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
             uint32_t off = running[d] + rank;
@@ -197,8 +211,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const uint32_t* __restri
             out_vals[off] = val;
         }
         __syncthreads();
-        running[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
-                                wcnt[3][threadIdx.x];
+        if (leader) {
+            atomicAdd(&running[d], cnt);
+            wcnt[wave][d] = 0;
+        }
         __syncthreads();
     }
 }
@@ -214,27 +230,61 @@ __global__ void k_copy_pairs(const uint32_t* __restrict__ a, const uint32_t* __r
     }
 }
 
-size_t sort_workspace_bytes(uint32_t capacity) {
-    uint32_t tiles = div_up(capacity ? capacity : 1, SORT_TILE);
-    size_t hist = (size_t)tiles * RADIX * 4;
-    size_t tmp = (size_t)(capacity ? capacity : 1) * 8;
-    return hist + scan_workspace_bytes(tiles * RADIX) + tmp + 1024;
+bool sort_shape_valid(SortShape sh) {
+    return (sh.bits == 4u || sh.bits == 8u || sh.bits == 11u) && (sh.items == 4u || sh.items == 8u || sh.items == 16u);
 }
+
+size_t sort_workspace_bytes(uint32_t capacity, SortShape sh) {
+    const uint32_t tiles = div_up(capacity ? capacity : 1, (uint64_t)BLOCK * sh.items);
+    const size_t digits = (size_t)tiles << sh.bits;
+    const size_t tmp = (size_t)(capacity ? capacity : 1) * 8;
+    return digits * 4 + scan_workspace_bytes((uint32_t)digits) + tmp + 1024;
+}
+
+size_t sort_workspace_bytes_any(uint32_t capacity) {
+    size_t m = 0;
+    for (uint32_t bits : {4u, 8u, 11u})
+        for (uint32_t items : {4u, 8u, 16u}) m = std::max(m, sort_workspace_bytes(capacity, SortShape{bits, items}));
+    return m;
+}
+
+namespace {
+template <int BITS, int ITEMS>
+void sort_pass(const uint32_t* src_k, const uint32_t* src_v, const uint32_t* d_len, uint32_t shift, uint32_t tiles,
+               uint32_t* hist, void* scan_ws, uint32_t* dk, uint32_t* dv, hipStream_t s) {
+    hipLaunchKernelGGL((k_sort_hist<BITS, ITEMS>), dim3(tiles), dim3(BLOCK), 0, s, src_k, d_len, shift, tiles, hist);
+    scan_u32_n(hist, hist, tiles << BITS, false, nullptr, scan_ws, s);
+    hipLaunchKernelGGL((k_sort_scatter<BITS, ITEMS>), dim3(tiles), dim3(BLOCK), 0, s, src_k, src_v, d_len, shift,
+                       tiles, hist, dk, dv);
+}
+template <int BITS>
+void sort_pass_items(uint32_t items, const uint32_t* src_k, const uint32_t* src_v, const uint32_t* d_len,
+                     uint32_t shift, uint32_t tiles, uint32_t* hist, void* scan_ws, uint32_t* dk, uint32_t* dv,
+                     hipStream_t s) {
+    if (items == 4u)
+        sort_pass<BITS, 4>(src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
+    else if (items == 16u)
+        sort_pass<BITS, 16>(src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
+    else
+        sort_pass<BITS, 8>(src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
+}
+}  // namespace
 
 void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, const uint32_t* d_len,
                       uint32_t capacity, uint32_t upper_bound, uint32_t* out_keys,
-                      uint32_t* out_vals, void* ws, hipStream_t s) {
-    uint32_t cap = capacity ? capacity : 1;
-    uint32_t tiles = div_up(cap, SORT_TILE);
+                      uint32_t* out_vals, void* ws, hipStream_t s, SortShape sh) {
+    if (!sort_shape_valid(sh)) sh = SORT_DEFAULT;
+    const uint32_t cap = capacity ? capacity : 1;
+    const uint32_t tiles = div_up(cap, (uint64_t)BLOCK * sh.items);
     char* w = static_cast<char*>(ws);
     uint32_t* hist = reinterpret_cast<uint32_t*>(w);
-    w += (size_t)tiles * RADIX * 4;
+    w += ((size_t)tiles << sh.bits) * 4;
     void* scan_ws = w;
-    w += scan_workspace_bytes(tiles * RADIX);
+    w += scan_workspace_bytes(tiles << sh.bits);
     uint32_t* tk = reinterpret_cast<uint32_t*>(w);
     uint32_t* tv = tk + cap;
     int passes = 0;
-    for (uint32_t off = 0; off < 32u && (upper_bound >> off) != 0u; off += RADIX_BITS) ++passes;
+    for (uint32_t off = 0; off < 32u && (upper_bound >> off) != 0u; off += sh.bits) ++passes;
     if (passes == 0) {
         hipLaunchKernelGGL(k_copy_pairs, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, keys, vals,
                            out_keys, out_vals, d_len);
@@ -244,20 +294,19 @@ void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, const uint32_t
     const uint32_t* src_k = keys;
     const uint32_t* src_v = vals;
     for (int p = 0; p < passes; ++p) {
-        bool last = (p == passes - 1);
         // pass p writes to out_* when (passes - 1 - p) is even
-        bool to_out = ((passes - 1 - p) % 2) == 0;
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* dk = to_out ? out_keys : tk;
         uint32_t* dv = to_out ? out_vals : tv;
-        uint32_t shift = (uint32_t)p * RADIX_BITS;
-        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, s, src_k, d_len, shift, tiles,
-                           hist);
-        scan_u32_n(hist, hist, tiles * RADIX, false, nullptr, scan_ws, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(BLOCK), 0, s, src_k, src_v, d_len,
-                           shift, tiles, hist, dk, dv);
+        const uint32_t shift = (uint32_t)p * sh.bits;
+        if (sh.bits == 4u)
+            sort_pass_items<4>(sh.items, src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
+        else if (sh.bits == 11u)
+            sort_pass_items<11>(sh.items, src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
+        else
+            sort_pass_items<8>(sh.items, src_k, src_v, d_len, shift, tiles, hist, scan_ws, dk, dv, s);
         src_k = dk;
         src_v = dv;
-        (void)last;
     }
 }
 

@@ -132,6 +132,7 @@ struct Scene {
     int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8 or 16
     bool walk_auto = true;  // walk lanes from a recent mean row length (4 below 62 entries, else 8)
     int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
+    SortShape sort_shape = SORT_DEFAULT;  // digit width and tile of the hash sorts
     bool exact_walks = false;  // pair build by the two exact walks only (no fixed-stride slots)
     // captured substeps (two per graph: the buffer roles repeat after two)
     hipStream_t own_stream = nullptr;

@@ -142,7 +142,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(pos_save, cap);
     A(solve_stats, 4);
 #undef A
-    size_t ws = std::max(sort_workspace_bytes(cap), scan_workspace_bytes(cap + 1));
+    size_t ws = std::max(sort_workspace_bytes_any(cap), scan_workspace_bytes(cap + 1));
     ws = std::max(ws, scan_workspace_bytes(pcap));
     ws = std::max(ws, scan_workspace_bytes(tcap + 1));
     ws_bytes = ws;
@@ -252,7 +252,7 @@ void Scene::neighborhood_green(float range_scale, bool spread) {
     const SegmentTable seg0 = seg_table();
     uint32_t upper = packed() ? (n_seg << seg_shift) : (1u << (grid.res * 3u));
     launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s, &seg0);
-    radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s);
+    radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s, sort_shape);
     ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
     bool tk = tbegin(K_PERMUTE);
     launch_permute(perm, counts, cap, src, dst, s);

@@ -141,8 +141,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         // so where it already reaches umax (the common case of neighbours no
         // smaller than the particle) the entry changes nothing and the
         // influence is not evaluated.
-        auto spread_to = [&](uint32_t j, float d2r) {  // d2r: squared distance * 2^36
-            const uint32_t cur = ukw[j];
+        // `cur` is the neighbour's stored value, loaded with its position.
+        auto spread_to = [&](uint32_t j, float d2r, uint32_t cur) {  // d2r: squared distance * 2^36
             if (umax <= cur) return;
             uint32_t uval;
             if (d2r <= t_okw) {  // distance <= okw: influence exactly 1
@@ -182,12 +182,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             more = braw < wa.pcap;
         }
         const size_t slot = PAD ? (size_t)i * stride : 0;
-        // The cells of one x-line of the search box are taken L at a time:
-        // lane c of the group looks up cell lo.x + c0 + c, a scan over the
-        // group gives each cell's offset in the line's candidates taken in
-        // cell order, and every step then tests the next L of those
-        // candidates, so steps run full across cell ends. A lane finds the
-        // cell of its candidate by a binary search over the offsets.
+        // The cells of one x-line of the search box are taken CW at a time:
+        // lane c < CW of the group looks up cell lo.x + c0 + c, a scan over
+        // the group gives each cell's offset in the line's candidates taken
+        // in cell order, and the offsets and index deltas of the CW cells are
+        // broadcast to every lane of the group. Every step then tests the
+        // next L of those candidates, so steps run full across cell ends; a
+        // lane picks its candidate's cell by comparing with the offsets held
+        // in its registers. A candidate's position and (spreading) stored
+        // width are loaded together.
+        constexpr int CW = L < 4 ? L : 4;
         const uint32_t hmask = (g.res >= 32u) ? 0xFFFFFFFFu : ((1u << g.res) - 1u);
         const uint32_t nline = hi[0] >= lo[0] ? hi[0] - lo[0] + 1u : 0u;
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
@@ -195,8 +199,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
                 const float dyz2 = dz2 + axis_d2(1, y, p.y);
                 const uint32_t hyz = (spread_bits3(y & hmask) << 1) | (spread_bits3(z & hmask) << 2);
-                for (uint32_t c0 = 0; more && c0 < nline && dyz2 <= thr; c0 += L) {
-                    const uint32_t nc = min(nline - c0, (uint32_t)L);
+                for (uint32_t c0 = 0; more && c0 < nline && dyz2 <= thr; c0 += CW) {
+                    const uint32_t nc = min(nline - c0, (uint32_t)CW);
                     uint32_t cstart = 0, clen = 0;
                     if ((uint32_t)gl < nc) {
                         const uint32_t x = lo[0] + c0 + (uint32_t)gl;
@@ -209,24 +213,33 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                     }
                     uint32_t incl = clen;
 #pragma unroll
-                    for (int m = 1; m < L; m <<= 1) {
+                    for (int m = 1; m < CW; m <<= 1) {
                         const uint32_t v = (uint32_t)__shfl_up((int)incl, m, L);
                         if (gl >= m) incl += v;
                     }
-                    const uint32_t total = (uint32_t)__shfl((int)incl, L - 1, L);
-                    const uint32_t off = incl - clen;  // lanes past the last cell hold total
+                    // lanes c < CW past the last cell hold the total as offset
+                    const uint32_t off = incl - clen;
                     const uint32_t delta = cstart - off;
+                    const uint32_t total = (uint32_t)__shfl((int)incl, CW - 1, L);
+                    uint32_t o[CW], dl[CW];
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        o[c] = (uint32_t)__shfl((int)off, c, L);
+                        dl[c] = (uint32_t)__shfl((int)delta, c, L);
+                    }
                     for (uint32_t t0 = 0; more && t0 < total; t0 += L) {
                         const uint32_t t = t0 + (uint32_t)gl;
-                        uint32_t c = 0;  // last cell whose offset is <= t
+                        uint32_t d = dl[0];
 #pragma unroll
-                        for (int stp = L / 2; stp >= 1; stp >>= 1)
-                            if ((uint32_t)__shfl((int)off, (int)c + stp, L) <= t) c += (uint32_t)stp;
-                        const uint32_t j = t + (uint32_t)__shfl((int)delta, (int)c, L);
+                        for (int c = 1; c < CW; ++c)
+                            if (t >= o[c]) d = dl[c];
+                        const uint32_t j = t + d;
                         bool green = false, keep = false;
                         float d2 = 0.0f;
+                        uint32_t uj = 0u;
                         if (t < total && j != i) {
                             const int4 pj = pos[j];
+                            if (SPREAD && MODE != WALK_COUNT) uj = ukw[j];
                             const v3f u = pr - vec_i(pj.x, pj.y, pj.z);
                             green = !(dot(u, u) > t_range);
                             if (SPREAD && green) {
@@ -237,7 +250,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                         }
                         const uint64_t rbal = __ballot(green) & gm;
                         if (PAD) {
-                            if (SPREAD && green) spread_to(j, d2);
+                            if (SPREAD && green) spread_to(j, d2, uj);
                             const bool kept = SPREAD ? (green && keep) : green;
                             const uint64_t kbal = SPREAD ? (__ballot(kept) & gm) : rbal;
                             const uint32_t k = cf + (uint32_t)__popcll(kbal & below);
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                             const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
                             const bool incl_r = green && ridx < wa.pcap;
                             if (SPREAD) {
-                                if (incl_r) spread_to(j, d2);
+                                if (incl_r) spread_to(j, d2, uj);
                                 const bool kept = incl_r && keep;
                                 const uint64_t fbal = __ballot(kept) & gm;
                                 if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;

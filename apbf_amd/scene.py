@@ -166,6 +166,14 @@ class Scene:
     def set_launch_shape(self, walk_lanes: int, solve_lanes: int) -> None:
         check(self._L.apbf_scene_set_launch_shape(self._h, int(walk_lanes), int(solve_lanes)), "set_launch_shape")
 
+    def set_sort_shape(self, digit_bits: int, items_per_thread: int) -> None:
+        check(self._L.apbf_scene_set_sort_shape(self._h, int(digit_bits), int(items_per_thread)), "set_sort_shape")
+
+    def sort_shape(self):
+        b, i = C.c_int32(), C.c_int32()
+        check(self._L.apbf_scene_sort_shape(self._h, C.byref(b), C.byref(i)), "sort_shape")
+        return b.value, i.value
+
     def set_long_rows(self, mode: int) -> None:
         """The solver's shift-table variant: -1 by row length (default), 0 short rows, 1 long rows."""
         check(self._L.apbf_scene_set_long_rows(self._h, int(mode)), "set_long_rows")

@@ -373,8 +373,8 @@ def main():
     for name in timed:
         k_ms, k_launches = scene.kernel_time_of(name)
         per_p, per_pair, per_raw = BYTE_MODEL[name]
-        b = (per_p * tot["particles"] + per_pair * tot["pairs"] + per_raw * tot["raw_pairs"]) / max(args.steps, 1)
-        per_launch = b / (k_launches / args.steps) if k_launches else None
+        # every launch of these kernels covers the whole substep's particles and pairs
+        per_launch = (per_p * tot["particles"] + per_pair * tot["pairs"] + per_raw * tot["raw_pairs"]) / max(args.steps, 1)
         avg_s = (k_ms / 1e3) / k_launches if k_launches else None
         ach = per_launch / avg_s / 1e9 if k_launches else None
         rooflines[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

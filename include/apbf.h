@@ -573,9 +573,17 @@ apbf_status apbf_batch_synchronize(apbf_batch* batch);
  * was captured from.
  * apbf_scene_autotune times `substeps` substeps under each shape, every
  * shape from the same saved state (twice, keeping the faster time), and keeps
- * the fastest; the scene and its statistics are put back afterwards, except
- * the neighbour list, which is empty until the next substep. */
+ * the fastest; it then times the scene's hash sort alone under every sort
+ * shape (below) on the keys of one substep from the same state and keeps the
+ * fastest. The scene and its statistics are put back afterwards, except the
+ * neighbour list, which is empty until the next substep. */
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes);
+/* The radix sort of the scene's neighbourhood builds: digit width 4 (the
+ * reference's, algorithms.cpp:61-64), 8 (the default) or 11 bits, and 4, 8 or
+ * 16 values per thread of a 256-thread tile. Every shape gives the same
+ * stable order. */
+apbf_status apbf_scene_set_sort_shape(void* scene, int32_t digit_bits, int32_t items_per_thread);
+apbf_status apbf_scene_sort_shape(void* scene, int32_t* digit_bits, int32_t* items_per_thread);
 /* The solver's row groups per shift-table flush in LDS: 2 for long rows, 4
  * for short ones. mode -1 (the default) picks by the mean row length of a
  * recent substep, read without synchronisation; 0 and 1 pin the short- and

@@ -83,6 +83,34 @@ def test_launch_shapes_match_the_oracle(device, walk, solve):
     assert not bad, "; ".join(bad)
 
 
+@pytest.mark.parametrize("kind", ["small_water_drop", "large_keys"])
+def test_sort_shapes_match_the_oracle(device, kind):
+    """Every digit width (4, 8, 11 bits) and tile (4, 8, 16 values per
+    thread) of the hash sort gives the oracle's substeps bit for bit: once on
+    the small water drop (hash keys of 12 bits at 2^4 cells per axis: 3, 2
+    and 2 passes) and once with 2^7 cells per axis (21-bit keys: 6, 3 and 2
+    passes, the last 11-bit pass partial)."""
+    from oracle.oracle import OracleScene
+    p, st = cfg.small_water_drop(seed=6, active=True)
+    if kind == "large_keys":
+        p = dict(p, hash_res_log2=7)
+    r = OracleScene(p)
+    r.set_state(st)
+    for _ in range(3):
+        r.update(p["dt"])
+    want = r.get_state()
+    for bits in (4, 8, 11):
+        for items in (4, 8, 16):
+            g = _scene(p, st)
+            g.set_sort_shape(bits, items)
+            assert g.sort_shape() == (bits, items)
+            for _ in range(3):
+                g.update(p["dt"])
+            bad = compare(g.get_state(), want, check_pairs=True)
+            g.close()
+            assert not bad, f"sort shape {bits} bits x {items}: " + "; ".join(bad)
+
+
 def test_autotune_keeps_results(device):
     """The autotuner times every shape from the same saved state and puts the
     scene and its statistics back: afterwards the scene equals one that was
@@ -96,6 +124,8 @@ def test_autotune_keeps_results(device):
             s.update(dt)
     w, s, ms = tuned.autotune(dt, substeps=2)
     assert w in (4, 8, 16) and s in (8, 16, 32) and ms > 0
+    bits, items = tuned.sort_shape()
+    assert bits in (4, 8, 11) and items in (4, 8, 16)
     bad = compare(tuned.get_state(), plain.get_state(), check_pairs=False)
     assert not bad, "after autotune: " + "; ".join(bad)
     assert tuned.counts()[2] == 0
