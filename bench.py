@@ -56,6 +56,10 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+# name of each workload in the committed profile summaries
+PROFILE_KEY = {"water_drop": "water_drop_settled"}
+
+
 def committed_traffic(kernel: str, workload_name: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel
@@ -155,6 +159,36 @@ def learned_comparison(scene, params, args, log):
     return out
 
 
+def measure_rooflines(scenes, steps: int, names, profile_key):
+    """Roofline entries of the kernels `names`, timed with HIP events on
+    every launch of the timed region in each of `scenes` ((scene, its
+    substep totals) pairs, stepped on one stream). Every launch of these
+    kernels covers its scene's whole substep, so the algorithmic bytes of the
+    region are the byte model over the substep totals times the launches per
+    substep; achieved = those bytes / the launches' summed duration."""
+    out = {}
+    for name in names:
+        per_p, per_pair, per_raw = BYTE_MODEL[name]
+        total_bytes, total_ms, launches = 0.0, 0.0, 0
+        for sc, tot in scenes:
+            k_ms, k_launches = sc.kernel_time_of(name)
+            if not k_launches:
+                continue
+            substep_bytes = per_p * tot["particles"] + per_pair * tot["pairs"] + per_raw * tot["raw_pairs"]
+            total_bytes += substep_bytes * k_launches / steps
+            total_ms += k_ms
+            launches += k_launches
+        if not launches:
+            continue
+        avg_s = total_ms / 1e3 / launches
+        ach = total_bytes / (total_ms / 1e3) / 1e9
+        out[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": committed_traffic(name, profile_key),
+                     "launches_per_step": launches / steps, "avg_launch_ms": avg_s * 1e3,
+                     "ms_per_step": total_ms / steps, "algorithmic_bytes_per_launch": total_bytes / launches}
+    return out
+
+
 def run_batch(args, log):
     """64 independent scenes (10k-500k particles, log-uniform) stepped as one
     batch: eager (every scene's substep enqueued on its own stream), as a
@@ -205,6 +239,11 @@ def run_batch(args, log):
         packed_batch.update(dt)
     packed_batch.synchronize()
     packed_batch.particle_substeps(reset=True)
+    timed = ("walk_fill", "solve_rows")
+    lists = [g["scene"] for g in packed_batch.groups]
+    for sc in lists:
+        sc.substep_totals(reset=True)
+        sc.time_kernels(timed, max_launches=steps * 16)
     t0 = time.perf_counter()
     for _ in range(steps):
         packed_batch.update(dt)
@@ -212,6 +251,7 @@ def run_batch(args, log):
     packed_s = time.perf_counter() - t0
     packed_ps = packed_batch.particle_substeps(reset=True)
     n_lists = len(packed_batch.groups)
+    rooflines = measure_rooflines([(sc, sc.substep_totals()) for sc in lists], steps, timed, "batch64")
     packed_batch.close()
     log(f"eager {eager_s / steps * 1e3:.2f} ms/substep, graph {graph_s / steps * 1e3:.2f} ms/substep, "
         f"packed {packed_s / steps * 1e3:.2f} ms/substep ({n_lists} lists)")
@@ -229,6 +269,10 @@ def run_batch(args, log):
                    "kinds": "alternating plain dam break / adaptive water drop", "seed": 0, "mode": mode},
         "eager": eager, "graph": graph, "packed": packed,
     }
+    if mode == "packed lists" and rooflines:
+        dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"] or 0.0)
+        out["roofline"] = rooflines[dominant]
+        out["rooflines"] = rooflines
     print(json.dumps(out), flush=True)
 
 
@@ -369,21 +413,8 @@ def main():
     elapsed = t1 - t0
     log(f"timed {args.steps} substeps in {elapsed:.3f} s")
     tot = scene.substep_totals()
-    rooflines = {}
-    for name in timed:
-        k_ms, k_launches = scene.kernel_time_of(name)
-        per_p, per_pair, per_raw = BYTE_MODEL[name]
-        # every launch of these kernels covers the whole substep's particles and pairs
-        per_launch = (per_p * tot["particles"] + per_pair * tot["pairs"] + per_raw * tot["raw_pairs"]) / max(args.steps, 1)
-        avg_s = (k_ms / 1e3) / k_launches if k_launches else None
-        ach = per_launch / avg_s / 1e9 if k_launches else None
-        rooflines[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": (ach / HBM_PEAK_GBS) if ach else None,
-                           "traffic": committed_traffic(name, args.workload),
-                           "launches_per_step": k_launches / args.steps if args.steps else 0,
-                           "avg_launch_ms": avg_s * 1e3 if avg_s else None,
-                           "ms_per_step": k_ms / args.steps if args.steps else None,
-                           "algorithmic_bytes_per_launch": per_launch}
+    rooflines = measure_rooflines([(scene, tot)], args.steps, timed,
+                                  PROFILE_KEY.get(args.workload, args.workload))
     scene.time_kernel(None)
     dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"] or 0.0)
     iters = params["solver_iterations"]
