@@ -49,12 +49,6 @@ enum WalkMode { WALK_COUNT = 0, WALK_FILL = 1, WALK_PAD = 2 };
 // count, so they agree on it.
 __device__ __forceinline__ uint32_t slot_stride(uint32_t pcap, uint32_t n) { return pcap / (n ? n : 1u); }
 
-template <int L>
-__device__ __forceinline__ uint64_t walk_group_mask() {
-    const int lane = threadIdx.x & 63;
-    const uint64_t m = (L == 64) ? ~0ull : ((1ull << L) - 1ull);
-    return m << (lane & ~(L - 1));
-}
 
 // Eight waves per SIMD (64 VGPRs, a 28-byte spill) hide the latency of the
 // cell-table and position loads better than seven without the cap
@@ -79,8 +73,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         }
     }
     const int lane = threadIdx.x & 63;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t gm = walk_group_mask<L>();
+    // A ballot's bits of this lane's group, shifted down: their count is the
+    // group's total, and those below the lane's bit its rank.
+    const uint32_t gshift = (uint32_t)lane & ~(uint32_t)(L - 1);
+    constexpr uint32_t GBITS = (L >= 32) ? 0xFFFFFFFFu : ((1u << L) - 1u);
+    const uint32_t gbelow = (1u << (threadIdx.x & (L - 1))) - 1u;
+    auto group_bits = [&](uint64_t bal) { return (uint32_t)(bal >> gshift) & GBITS; };
     const uint32_t n = counts[C_N];
     const uint32_t stride = wa.stride ? slot_stride(wa.pcap, n) : 0u;
     const int gl = threadIdx.x & (L - 1);
@@ -248,31 +246,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                                 keep = d2 <= t_cut;
                             }
                         }
-                        const uint64_t rbal = __ballot(green) & gm;
+                        const uint32_t rbal = group_bits(__ballot(green));
                         if (PAD) {
                             if (SPREAD && green) spread_to(j, d2, uj);
                             const bool kept = SPREAD ? (green && keep) : green;
-                            const uint64_t kbal = SPREAD ? (__ballot(kept) & gm) : rbal;
-                            const uint32_t k = cf + (uint32_t)__popcll(kbal & below);
+                            const uint32_t kbal = SPREAD ? group_bits(__ballot(kept)) : rbal;
+                            const uint32_t k = cf + (uint32_t)__popc(kbal & gbelow);
                             if (kept && k < stride) col[slot + k] = j;
-                            cf += (uint32_t)__popcll(kbal);
-                            craw += (uint32_t)__popcll(rbal);
+                            cf += (uint32_t)__popc(kbal);
+                            craw += (uint32_t)__popc(rbal);
                         } else if (!FILL) {
-                            craw += (uint32_t)__popcll(rbal);
-                            if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep) & gm);
+                            craw += (uint32_t)__popc(rbal);
+                            if (SPREAD) cf += (uint32_t)__popc(group_bits(__ballot(keep)));
                         } else {
-                            const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
+                            const uint32_t ridx = braw + craw + (uint32_t)__popc(rbal & gbelow);
                             const bool incl_r = green && ridx < wa.pcap;
                             if (SPREAD) {
                                 if (incl_r) spread_to(j, d2, uj);
                                 const bool kept = incl_r && keep;
-                                const uint64_t fbal = __ballot(kept) & gm;
-                                if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
-                                cf += (uint32_t)__popcll(fbal);
+                                const uint32_t fbal = group_bits(__ballot(kept));
+                                if (kept) col[bf + cf + (uint32_t)__popc(fbal & gbelow)] = j;
+                                cf += (uint32_t)__popc(fbal);
                             } else if (incl_r) {
                                 col[ridx] = j;
                             }
-                            craw += (uint32_t)__popcll(rbal);
+                            craw += (uint32_t)__popc(rbal);
                             more = braw + craw < wa.pcap;  // later entries fall beyond the capacity
                         }
                     }
