@@ -323,10 +323,14 @@ class PackedBatch:
     """Independent scenes stepped as packed lists (apbf_scene_create_packed):
     scenes with the same settings share one particle list, so every stage of
     a substep is one launch over all of them. ``cases`` is a list of
-    (params, state); the groups' capacities are the sums of their scenes'."""
+    (params, state); the groups' capacities are the sums of their scenes'.
+    Each list is stepped on a stream of its own (default priority), so the
+    lists overlap on the device."""
 
     def __init__(self, cases, stream: int | None = None):
+        import torch
         self._L = _lib.lib()
+        self._streams = []
         groups: dict = {}
         for k, (p, _) in enumerate(cases):
             groups.setdefault(_group_key(p), []).append(k)
@@ -347,6 +351,10 @@ class PackedBatch:
             sc._L, sc.params, sc._p, sc._h = self._L, gp, cp, h
             if stream is not None:
                 sc.set_stream(stream)
+            else:
+                st_obj = torch.cuda.Stream()
+                self._streams.append(st_obj)
+                sc.set_stream(st_obj.cuda_stream)
             states = [cases[k][1] for k in members]
             joined = _join_states(states)
             sc.set_state(joined)

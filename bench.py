@@ -239,11 +239,6 @@ def run_batch(args, log):
         packed_batch.update(dt)
     packed_batch.synchronize()
     packed_batch.particle_substeps(reset=True)
-    timed = ("walk_fill", "solve_rows")
-    lists = [g["scene"] for g in packed_batch.groups]
-    for sc in lists:
-        sc.substep_totals(reset=True)
-        sc.time_kernels(timed, max_launches=steps * 16)
     t0 = time.perf_counter()
     for _ in range(steps):
         packed_batch.update(dt)
@@ -251,7 +246,20 @@ def run_batch(args, log):
     packed_s = time.perf_counter() - t0
     packed_ps = packed_batch.particle_substeps(reset=True)
     n_lists = len(packed_batch.groups)
-    rooflines = measure_rooflines([(sc, sc.substep_totals()) for sc in lists], steps, timed, "batch64")
+    # kernel rooflines from a few more substeps with the lists on one stream
+    # (durations of kernels that overlap on the device are not their own)
+    timed = ("walk_fill", "solve_rows")
+    lists = [g["scene"] for g in packed_batch.groups]
+    one = torch.cuda.current_stream().cuda_stream
+    roof_steps = 4
+    for sc in lists:
+        sc.set_stream(one)
+        sc.substep_totals(reset=True)
+        sc.time_kernels(timed, max_launches=roof_steps * 16)
+    for _ in range(roof_steps):
+        packed_batch.update(dt)
+    packed_batch.synchronize()
+    rooflines = measure_rooflines([(sc, sc.substep_totals()) for sc in lists], roof_steps, timed, "batch64")
     packed_batch.close()
     log(f"eager {eager_s / steps * 1e3:.2f} ms/substep, graph {graph_s / steps * 1e3:.2f} ms/substep, "
         f"packed {packed_s / steps * 1e3:.2f} ms/substep ({n_lists} lists)")
@@ -320,6 +328,36 @@ def cpu_baseline(params, state, seconds: float):
                       f"(nproc {os.cpu_count()}), g++ -O2 -ffp-contract=off"}
 
 
+def cpu_baseline_threads(params, state, threads: int):
+    """The oracle on `threads` host threads at once: each thread steps its own
+    copy of the same scene by one substep (the oracle itself is the sequential
+    schedule, so threads multiply scenes, not split one). The pair capacity
+    is cut to 250 per particle (above the 212 per particle the walk peaks at)
+    so that the copies fit in host memory."""
+    import threading
+    from oracle import oracle as O
+    O.build()
+    p = dict(params, cap_pairs=int(len(state["radius"]) * 250))
+    scenes = []
+    for _ in range(threads):
+        sc = O.OracleScene(p)
+        sc.set_state(state)
+        scenes.append(sc)
+    n = scenes[0].counts()[1]
+    workers = [threading.Thread(target=sc.update, args=(params["dt"],)) for sc in scenes]
+    t0 = time.perf_counter()
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    dt = time.perf_counter() - t0
+    for sc in scenes:
+        sc.close()
+    return {"value": n * threads / dt, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} copies of the post-warmup scene, one substep each on its own thread, "
+                      f"{n * threads} particle-substeps in {dt:.1f} s on '{cpu_model()}' (nproc {os.cpu_count()})"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +368,8 @@ def main():
                     help="time only this kernel (default: the neighbour walk and the solver row pass)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="also time the oracle on this many host threads at once (one scene copy per thread)")
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's solver iterations (studies of iterations to tolerance)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -432,11 +472,14 @@ def main():
     if not args.no_learned:
         learned = learned_comparison(scene, params, args, log)
 
-    cpu = None
-    if not args.no_cpu_baseline:
+    cpu = cpu_threads = None
+    if not args.no_cpu_baseline or args.cpu_threads:
         st = scene.get_state(pairs=False)
         st["fluid_index"] = st["fluid_index"][: len(st["radius"])]
-        cpu = cpu_baseline(params, st, args.cpu_seconds)
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(params, st, args.cpu_seconds)
+        if args.cpu_threads:
+            cpu_threads = cpu_baseline_threads(params, st, args.cpu_threads)
 
     out = {
         "metric": "particle-substeps/s, 1M adaptive particles at <=0.1% density error, plain vs learned",
@@ -481,6 +524,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_vs_learned": learned,
     }
+    if cpu_threads:
+        out["cpu_baseline_threads"] = cpu_threads
     print(json.dumps(out), flush=True)
 
 
