@@ -44,6 +44,15 @@ BYTE_MODEL = {
     # + 8 B per found pair read back + 8 B per kept pair written
     "walk_fill": (40.0, 8.0, 16.0),
 }
+# the neighbour walk of a scene without spreading (basic PBF, or kernel widths
+# from boundary distances): the pair build alone, 16 B/particle + 8 B per pair
+PLAIN_WALK = (16.0, 8.0, 0.0)
+
+
+def byte_model(name: str, params) -> tuple:
+    s = params["settings"]
+    spread = not params["basic_pbf"] and not s["mBaseKernelWidthOnBoundaryDistance"]
+    return PLAIN_WALK if (name == "walk_fill" and not spread) else BYTE_MODEL[name]
 
 
 def cpu_model():
@@ -168,9 +177,9 @@ def measure_rooflines(scenes, steps: int, names, profile_key):
     substep; achieved = those bytes / the launches' summed duration."""
     out = {}
     for name in names:
-        per_p, per_pair, per_raw = BYTE_MODEL[name]
         total_bytes, total_ms, launches = 0.0, 0.0, 0
         for sc, tot in scenes:
+            per_p, per_pair, per_raw = byte_model(name, sc.params)
             k_ms, k_launches = sc.kernel_time_of(name)
             if not k_launches:
                 continue
