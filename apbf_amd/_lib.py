@@ -204,6 +204,7 @@ PROTOTYPES = {
     "apbf_scene_set_launch_shape": (_i32, [_vp, _i32, _i32]),
     "apbf_scene_set_long_rows": (_i32, [_vp, _i32]),
     "apbf_scene_set_sort_shape": (_i32, [_vp, _i32, _i32]),
+    "apbf_scene_count_walk_candidates": (_i32, [_vp, _i32]),
     "apbf_scene_sort_shape": (_i32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "apbf_scene_autotune": (_i32, [_vp, _f32, _u32, _pi, _pi, _pf]),
     "apbf_mlp_create": (_i32, [C.POINTER(_vp)]),

@@ -46,7 +46,7 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
                  int lanes = 8, uint32_t stride = 0, const uint32_t* gate = nullptr,
-                 const SegmentTable* seg = nullptr);
+                 const SegmentTable* seg = nullptr, unsigned long long* cand = nullptr);
 void launch_walk_decide(uint32_t* counts, uint32_t pcap, hipStream_t s);
 void launch_brute(bool fill, const int4* pos, const float* kw, const uint32_t* counts, uint32_t cap,
                   float range_scale, uint32_t pcap, uint32_t* cnt, const uint32_t* row, uint32_t* col,

@@ -134,6 +134,7 @@ struct Scene {
     int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
     SortShape sort_shape = SORT_DEFAULT;  // digit width and tile of the hash sorts
     bool exact_walks = false;  // pair build by the two exact walks only (no fixed-stride slots)
+    bool count_candidates = false;  // diagnostic: a counting walk adds its tested candidates to C_CANDIDATES
     // captured substeps (two per graph: the buffer roles repeat after two)
     hipStream_t own_stream = nullptr;
     hipGraph_t graph = nullptr;

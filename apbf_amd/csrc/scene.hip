@@ -281,6 +281,11 @@ void Scene::build_pairs(float range_scale, bool spread) {
     uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list
     const uint32_t stride = exact_walks ? 0u : pcap / (cap ? cap : 1u);
     launch_set_n_plus_1(counts, s);
+    if (count_candidates)  // diagnostic: the counting walk visits the cells and candidates of every walk
+        launch_walk(0, spread, pos[cur], kernel_width[cur], radius[cur], target_radius[cur], on_t, cell_start,
+                    cell_end, counts, cap, range_scale, grid, pcap, cnt, spread ? u1 : nullptr, nullptr, nullptr,
+                    nullptr, nullptr, s, walk_lanes, 0, nullptr, &seg,
+                    reinterpret_cast<unsigned long long*>(counts + C_CANDIDATES));
     if (spread) spread_begin();
     if (stride == 0) {  // no room for slots: the two exact walks
         bool tk = tbegin(K_WALK_COUNT);
@@ -597,6 +602,14 @@ apbf_status apbf_scene_create(const apbf_scene_params* params, void** out_scene)
     }
     *out_scene = s;
     return APBF_OK;
+}
+
+apbf_status apbf_scene_count_walk_candidates(void* scene, int32_t enable) {
+    if (!scene) return APBF_ERR_INVALID_ARGUMENT;
+    Scene* s = static_cast<Scene*>(scene);
+    s->count_candidates = enable != 0;
+    apbf::fill_u32(s->counts + apbf::C_CANDIDATES, 0u, 2, s->stream);
+    return hipGetLastError() == hipSuccess ? APBF_OK : APBF_ERR_HIP;
 }
 
 apbf_status apbf_scene_create_packed(const apbf_scene_params* params, uint32_t n_segments,

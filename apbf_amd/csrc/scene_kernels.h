@@ -36,6 +36,7 @@ enum CountSlot : uint32_t {
     C_SOLVE_ITERS = 25,    // solver iterations whose shifts were applied in this substep
     C_WARM_FALLBACK = 26,  // 1 when this substep's warm start was undone (the cold start judged better)
     C_MAXROW_LAST = 27,    // single-pass walk: longest row that outgrew its slot in the last walk (0: none)
+    C_CANDIDATES = 28,     // 28-29: candidates tested by the diagnostic counting walks (a 64-bit sum)
     C_COUNT = 32
 };
 

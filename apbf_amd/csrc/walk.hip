@@ -38,6 +38,7 @@ struct WalkArgs {
     uint32_t pcap;
     uint32_t stride;        // PAD, and FILL after PAD: nonzero (the slot stride is slot_stride())
     const uint32_t* gate;   // FILL after PAD: the count block (C_FALLBACK, C_ROWFIX); null otherwise
+    unsigned long long* cand;  // COUNT: when set, += the candidates the walk tests (a diagnostic)
     SegmentTable seg;       // a packed batch (SEG walks): each particle's grid and cell block
 };
 
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             return t * t;
         };
         uint32_t craw = 0, cf = 0, braw = 0, bf = 0;
+        uint64_t tested = 0;  // COUNT: candidates tested for this particle
         bool more = true;
         if (FILL) {
             braw = row_raw[i];
@@ -219,6 +221,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                     const uint32_t off = incl - clen;
                     const uint32_t delta = cstart - off;
                     const uint32_t total = (uint32_t)__shfl((int)incl, CW - 1, L);
+                    if (MODE == WALK_COUNT) tested += total;
                     uint32_t o[CW], dl[CW];
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
@@ -287,6 +290,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             } else if (!FILL) {
                 cnt_raw[i] = craw;
                 if (SPREAD) cnt_f[i] = cf;
+                if (wa.cand) atomicAdd(wa.cand, (unsigned long long)tested);
             } else if (SPREAD && braw <= wa.pcap && row_raw[i + 1] > wa.pcap) {
                 // the row the capacity cuts: its kept short-list length
                 counts[C_CLAMP_ROW] = i;
@@ -422,7 +426,8 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
                  int on_target, const uint32_t* cs, const uint32_t* ce, uint32_t* counts, uint32_t cap,
                  float range_scale, const GridDesc& g, uint32_t pcap, uint32_t* cnt_raw, uint32_t* cnt_f,
                  const uint32_t* row_raw, const uint32_t* row_f, uint32_t* col, uint32_t* ukw, hipStream_t s,
-                 int lanes, uint32_t stride, const uint32_t* gate, const SegmentTable* seg) {
+                 int lanes, uint32_t stride, const uint32_t* gate, const SegmentTable* seg,
+                 unsigned long long* cand) {
     WalkArgs wa;
     wa.range_scale = range_scale;
     wa.g = g;
@@ -430,6 +435,7 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
     wa.pcap = pcap;
     wa.stride = stride;
     wa.gate = gate;
+    wa.cand = cand;
     wa.seg = seg ? *seg : SegmentTable{nullptr, nullptr, nullptr, nullptr, 0u};
     const bool packed = seg && seg->sid;
     if (lanes != 4 && lanes != 16) lanes = WALK_LANES;

@@ -263,7 +263,10 @@ class Scene:
         return {"particles": int(c[0]), "pairs": int(c[2]), "transfers": int(c[3]), "splits": int(c[4]),
                 "overflow_flags": int(c[6]), "pairs_before_clamp": int(c[7]), "rows": int(c[14]),
                 "peak_pairs_before_clamp": int(c[17]), "longest_outgrown_row": int(c[27]),
-                "rows_outgrew_slots": int(c[22])}
+                "rows_outgrew_slots": int(c[22]), "walk_candidates": int(c[28]) | (int(c[29]) << 32)}
+
+    def count_walk_candidates(self, enable: bool) -> None:
+        check(self._L.apbf_scene_count_walk_candidates(self._h, int(bool(enable))), "count_walk_candidates")
 
     def overflow_flags(self) -> int:
         f = C.c_uint32()

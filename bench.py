@@ -490,6 +490,17 @@ def main():
         if args.cpu_threads:
             cpu_threads = cpu_baseline_threads(params, st, args.cpu_threads)
 
+    # one more substep with the candidate counter: cells and candidates the walk tests
+    scene.count_walk_candidates(True)
+    scene.update(dt)
+    cd = scene.diagnostics()
+    scene.count_walk_candidates(False)
+    walk_candidates = {"per_particle": cd["walk_candidates"] / max(cd["rows"], 1),
+                       "per_walk_pair": cd["walk_candidates"] / max(cd["pairs_before_clamp"], 1),
+                       "per_solver_pair": cd["walk_candidates"] / max(cd["pairs"], 1),
+                       "substep": {"candidates": cd["walk_candidates"], "particles": cd["rows"],
+                                   "walk_pairs": cd["pairs_before_clamp"], "solver_pairs": cd["pairs"]}}
+
     out = {
         "metric": "particle-substeps/s, 1M adaptive particles at <=0.1% density error, plain vs learned",
         "value": value,
@@ -532,6 +543,7 @@ def main():
         "rooflines": rooflines,
         "cpu_baseline": cpu,
         "plain_vs_learned": learned,
+        "walk_candidates": walk_candidates,
     }
     if cpu_threads:
         out["cpu_baseline_threads"] = cpu_threads

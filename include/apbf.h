@@ -209,9 +209,17 @@ apbf_status apbf_scene_substep_totals(void* scene, uint64_t* out, uint32_t n_out
  * 0 particles, 2 neighbour pairs, 3 transfers, 4 splits, 6 overflow flags,
  * 7 neighbour-walk pairs before the capacity clamp, 14 particles covered by
  * the neighbour rows, 17 largest pre-clamp pair count seen, 22 whether rows
- * outgrew their slots in the last single-pass walk, 27 the longest of them.
- * Synchronises. */
+ * outgrew their slots in the last single-pass walk, 27 the longest of them,
+ * 28-29 (low, high word) the candidates counted since
+ * apbf_scene_count_walk_candidates enabled counting. Synchronises. */
 apbf_status apbf_scene_diagnostics(void* scene, uint32_t* host_counts, uint32_t n_words);
+/* Diagnostic with no counterpart in the reference: while enabled, every
+ * neighbourhood build also runs the counting walk, which adds the number of
+ * candidates the walk tests (the particles of the visited, unculled cells)
+ * to diagnostics words 28-29. Enabling or disabling clears the sum. Costs
+ * one more walk per substep; eager updates only (a graph captured before
+ * keeps its launches). */
+apbf_status apbf_scene_count_walk_candidates(void* scene, int32_t enable);
 /* Overflow flags raised by capacity clamps since the last call (bit 0 pairs,
  * bit 1 transfers, bit 2 particles); cleared by the call. Synchronises. */
 apbf_status apbf_scene_overflow_flags(void* scene, uint32_t* flags);

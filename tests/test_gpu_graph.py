@@ -137,3 +137,25 @@ def test_autotune_keeps_results(device):
         tuned.update(dt)
         plain.update(dt)
         _same(tuned, plain, f"tuned substep {k}")
+
+
+def test_walk_candidate_counter(device):
+    """The candidate counter (a diagnostic counting walk) changes no result,
+    counts at least every in-range pair of the walk, and never more than the
+    particles of the whole scene per particle."""
+    p, st = cfg.small_water_drop(seed=2, active=True)
+    dt = p["dt"]
+    a, b = _scene(p, st), _scene(p, st)
+    a.count_walk_candidates(True)
+    a.update(dt)
+    b.update(dt)
+    d = a.diagnostics()
+    n = d["rows"]
+    assert d["pairs_before_clamp"] <= d["walk_candidates"] <= n * n
+    a.update(dt)
+    b.update(dt)
+    _same(a, b, "with and without candidate counting")
+    a.count_walk_candidates(False)
+    assert a.diagnostics()["walk_candidates"] == 0
+    a.close()
+    b.close()
