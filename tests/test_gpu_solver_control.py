@@ -93,3 +93,31 @@ def test_warm_start_fallback_is_taken_and_undone(device):
     assert (gi, gs, gf) == r.solver_stats()
     Lm.set_mode(g, None, "off")
     model.close()
+
+
+@pytest.mark.parametrize("tol", [0.5, 3e-2, 1e-3, 1e-9])
+def test_warm_start_with_tolerance_matches_the_oracle(device, tol):
+    """Warm start and tolerance together: iteration 0 judges the warm shift,
+    and when the shift helped and the error after it is within the tolerance
+    no plain iteration applies anything (0.5: every substep); with 1e-9 the
+    tolerance is never reached and every substep runs the limit."""
+    from apbf_amd import learned as Lm
+    p, st = cfg.small_water_drop(seed=9, active=True)
+    g, r = _pair(p, st)
+    model = _warm_model(1, -0.02, 0.05)
+    Lm.set_mode(g, model, "warm")
+    g.set_solver_tolerance(tol, 8)
+    r.set_solver_tolerance(tol, 8)
+    for k in range(4):
+        g.update(p["dt"])
+        r.set_warm_lambdas(g.warm_lambdas())
+        r.update(p["dt"])
+        bad = compare(g.get_state(), r.get_state(), check_pairs=True)
+        assert not bad, f"substep {k}: " + "; ".join(bad)
+    stats = g.solver_stats()
+    print(f"tolerance {tol}: (iterations, substeps, warm starts undone) = {stats}", flush=True)
+    assert stats == r.solver_stats()
+    if tol == 0.5 and stats[2] == 0:  # no shift undone: the warm shift alone met the tolerance
+        assert stats[0] == 0
+    Lm.set_mode(g, None, "off")
+    model.close()
