@@ -1,8 +1,12 @@
 // Learned lambda warm start: a 16-64-64-1 ReLU network on the matrix cores.
 //
-// BF16 weights and activations feed v_mfma_f32_16x16x16_bf16 (FP32
-// accumulation); biases, the last layer and all reductions stay FP32. One
-// 64-lane wavefront handles a tile of 16 samples; with the 16x16x16 operand
+// BF16 weights and activations feed the matrix cores with FP32 accumulation:
+// gfx950's v_mfma_f32_16x16x32_bf16 for the 64-deep products (layer 2 and
+// its input gradient), v_mfma_f32_16x16x16_bf16 for the 16-deep ones (layer
+// 1, whose input has 16 features, and the weight gradients, summed over the
+// 16 samples of a tile). Biases, the last layer and all reductions stay FP32.
+// One 64-lane wavefront handles a tile of 16 samples (the tiles of a block's
+// waves are independent: LDS is synchronised per wavefront); with the 16x16x16 operand
 // layout (A: lane l holds row l%16, k = 4(l/16)..+3; B: lane l holds column
 // l%16, k = 4(l/16)..+3; D: lane l holds column l%16, rows 4(l/16)..+3) the
 // layer-1 output tile of a lane is exactly the transposed operand the weight
@@ -18,6 +22,7 @@
 namespace apbf {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
@@ -39,18 +44,34 @@ __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
 __device__ __forceinline__ f32x4_t mfma(bf16x4_t a, bf16x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// gfx950's 16x16x32 form for the 64-deep products (layer 2 and its input
+// gradient): lane l holds row / column l%16 and k = 8(l/16) .. +7 of A / B;
+// D as in the 16x16x16 form.
+__device__ __forceinline__ f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8_t load8(const uint16_t* p) {  // 16-byte aligned
+    return *reinterpret_cast<const bf16x8_t*>(p);
+}
+// A wavefront's LDS tile written by some lanes and read by others (and the
+// reverse before the next write): a wavefront-level fence, no block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 constexpr int MLP_WAVES = 4;               // waves per block
 constexpr int MLP_THREADS = 64 * MLP_WAVES;
 
 // Model in LDS: bf16 W1 [64][16], W2 [64][64] and W2 transposed, fp32 rest.
 struct MlpShared {
-    uint16_t w1[MLP_H * MLP_IN];
-    uint16_t w2[MLP_H * MLP_H];
-    uint16_t w2t[MLP_H * MLP_H];
+    alignas(16) uint16_t w1[MLP_H * MLP_IN];
+    alignas(16) uint16_t w2[MLP_H * MLP_H];
+    alignas(16) uint16_t w2t[MLP_H * MLP_H];
     float b1[MLP_H], b2[MLP_H], w3[MLP_H], b3;
     float mean[MLP_IN], scale[MLP_IN], out_scale;
-    uint16_t xfer[MLP_WAVES][16 * MLP_H];  // per-wave transposition tile
+    alignas(16) uint16_t xfer[MLP_WAVES][16 * MLP_H];  // per-wave transposition tile
 };
 
 __device__ void load_model(const float* __restrict__ p, const float* __restrict__ norm, MlpShared& sh) {
@@ -107,22 +128,19 @@ __device__ void tile_forward(const float* __restrict__ x, uint32_t n, uint32_t b
 #pragma unroll
         for (int r = 0; r < 4; ++r) xf[(4 * hi + r) * MLP_H + nn] = f2bf(fmaxf(T.z1[t][r], 0.0f));
     }
-    __syncthreads();
-    // layer 2: A = relu(z1) (sample lo, hidden 16kt + 4hi ..)
-    bf16x4_t a2[4];
+    wave_lds_sync();
+    // layer 2: A = relu(z1) (sample lo, hidden 32kt + 8hi ..), 16x16x32 products
+    bf16x8_t a2[2];
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) a2[kt] = *reinterpret_cast<const bf16x4_t*>(&xf[lo * MLP_H + 16 * kt + 4 * hi]);
-    __syncthreads();
+    for (int kt = 0; kt < 2; ++kt) a2[kt] = load8(&xf[lo * MLP_H + 32 * kt + 8 * hi]);
+    wave_lds_sync();
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int nn = 16 * t + lo;
         const float bias = sh.b2[nn];
         f32x4_t c = {bias, bias, bias, bias};
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
-            const bf16x4_t b = *reinterpret_cast<const bf16x4_t*>(&sh.w2[nn * MLP_H + 16 * kt + 4 * hi]);
-            c = mfma(a2[kt], b, c);
-        }
+        for (int kt = 0; kt < 2; ++kt) c = mfma32(a2[kt], load8(&sh.w2[nn * MLP_H + 32 * kt + 8 * hi]), c);
         T.z2[t] = c;
     }
     // layer 3 (FP32): y = relu(z2) . w3 + b3, reduced over the 16 lanes of a row group
@@ -237,21 +255,18 @@ __global__ __launch_bounds__(MLP_THREADS) void k_mlp_grad(const float* __restric
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) xf[(4 * hi + r) * MLP_H + 16 * t + lo] = (uint16_t)dz2b[t][r];
-        __syncthreads();
-        bf16x4_t ad[4];
+        wave_lds_sync();
+        bf16x8_t ad[2];
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) ad[kt] = *reinterpret_cast<const bf16x4_t*>(&xf[lo * MLP_H + 16 * kt + 4 * hi]);
-        __syncthreads();
+        for (int kt = 0; kt < 2; ++kt) ad[kt] = load8(&xf[lo * MLP_H + 32 * kt + 8 * hi]);
+        wave_lds_sync();
         f32x4_t dz1[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             f32x4_t c = {0, 0, 0, 0};
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt) {
-                // B[kk = n][col = k]: lane holds W2[n = 16kt + 4hi + r][k = 16t + lo] = w2t[k][n ..]
-                const bf16x4_t b = *reinterpret_cast<const bf16x4_t*>(&sh.w2t[(16 * t + lo) * MLP_H + 16 * kt + 4 * hi]);
-                c = mfma(ad[kt], b, c);
-            }
+            for (int kt = 0; kt < 2; ++kt)  // B[n][col k]: lane holds W2[n = 32kt + 8hi ..][k = 16t + lo] = w2t[k][n ..]
+                c = mfma32(ad[kt], load8(&sh.w2t[(16 * t + lo) * MLP_H + 32 * kt + 8 * hi]), c);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const float g = T.z1[t][r] > 0.0f ? c[r] : 0.0f;
