@@ -34,6 +34,15 @@ struct apbf_mlp {
     float* partial = nullptr;  // max blocks x (MLP_PARAMS + 1)
     float* xs = nullptr;       // gathered batch rows
     float* loss = nullptr;     // per step
-    uint32_t max_batch = 0, max_blocks = 0, max_steps = 0;
-    uint64_t steps_done = 0;
+    uint32_t* perm = nullptr;       // the call's sample order (a copy the captured steps read)
+    uint32_t* step_base = nullptr;  // Adam steps done before the current call (device word)
+    uint32_t max_batch = 0, max_blocks = 0, max_steps = 0, max_n = 0;
+    // a training call's steps captured as one graph, replayed while the call's
+    // arguments stay the same (one call per epoch)
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    const float* g_features = nullptr;
+    const float* g_targets = nullptr;
+    uint32_t g_n = 0, g_batch = 0, g_steps = 0;
+    float g_lr = 0.0f;
 };

@@ -345,14 +345,20 @@ __global__ __launch_bounds__(BLOCK) void k_mlp_gather(const float* __restrict__ 
 }
 
 // Sums the block partials (in block order) into grad[0 .. MLP_PARAMS] and,
-// when lr > 0, applies one Adam step to the FP32 master parameters.
+// when lr > 0, applies one Adam step to the FP32 master parameters. The step
+// number for the bias corrections is *step_base + k + 1: the device keeps the
+// count, so captured steps replay with the right corrections.
 __global__ __launch_bounds__(BLOCK) void k_mlp_adam(const float* __restrict__ partial, uint32_t blocks,
                                                     float* __restrict__ grad, float* __restrict__ params,
                                                     float* __restrict__ m1, float* __restrict__ m2, float lr,
-                                                    float beta1, float beta2, float eps, float bc1, float bc2,
+                                                    float beta1, float beta2, float eps,
+                                                    const uint32_t* __restrict__ step_base, uint32_t k,
                                                     float* __restrict__ loss_out) {
+    const float t = (float)(*step_base + k + 1u);
+    const float bc1 = 1.0f - powf(beta1, t), bc2 = 1.0f - powf(beta2, t);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= (uint32_t)MLP_PARAMS; p += gridDim.x * blockDim.x) {
         float g = 0.0f;
+#pragma unroll 8
         for (uint32_t b = 0; b < blocks; ++b) g += partial[(size_t)b * (MLP_PARAMS + 1) + p];
         if (p == (uint32_t)MLP_PARAMS) {
             if (loss_out) *loss_out = g;
@@ -367,6 +373,10 @@ __global__ __launch_bounds__(BLOCK) void k_mlp_adam(const float* __restrict__ pa
             params[p] -= lr * (a / bc1) / (sqrtf(v / bc2) + eps);
         }
     }
+}
+
+__global__ void k_mlp_steps_done(uint32_t* __restrict__ step_base, uint32_t steps) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_base += steps;
 }
 
 // ------------------------------------------------------------------- host
@@ -386,8 +396,17 @@ void launch_mlp_infer(const float* params, const float* norm, const float* x, co
 using namespace apbf;
 
 
-static apbf_status mlp_reserve(apbf_mlp* m, uint32_t batch, uint32_t steps) {
+// The captured training launches hold the buffers' addresses: any
+// reallocation drops them.
+static void drop_training_graph(apbf_mlp* m) {
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+}
+
+static apbf_status mlp_reserve(apbf_mlp* m, uint32_t batch, uint32_t steps, uint32_t n) {
     const uint32_t blocks = mlp_grid_tiles(batch);
+    if (batch > m->max_batch || blocks > m->max_blocks || steps > m->max_steps || n > m->max_n)
+        drop_training_graph(m);
     if (batch > m->max_batch) {
         if (m->xs) (void)hipFree(m->xs);
         if (hipMalloc(&m->xs, (size_t)batch * MLP_IN * 4) != hipSuccess) return APBF_ERR_HIP;
@@ -403,7 +422,31 @@ static apbf_status mlp_reserve(apbf_mlp* m, uint32_t batch, uint32_t steps) {
         if (hipMalloc(&m->loss, (size_t)steps * 4) != hipSuccess) return APBF_ERR_HIP;
         m->max_steps = steps;
     }
+    if (n > m->max_n) {
+        if (m->perm) (void)hipFree(m->perm);
+        if (hipMalloc(&m->perm, (size_t)n * 4) != hipSuccess) return APBF_ERR_HIP;
+        m->max_n = n;
+    }
     return APBF_OK;
+}
+
+// The launches of one training call (steps of gather, gradient and Adam,
+// then the step counter), on stream s.
+static void enqueue_training(apbf_mlp* m, const float* features, const float* targets, const uint32_t* perm,
+                             uint32_t n, uint32_t batch, uint32_t steps, float lr, hipStream_t s) {
+    const uint32_t blocks = mlp_grid_tiles(batch);
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    for (uint32_t k = 0; k < (steps ? steps : 1u); ++k) {
+        const uint32_t start = (uint32_t)(((uint64_t)k * batch) % n);
+        hipLaunchKernelGGL(k_mlp_gather, dim3(stride_grid((uint64_t)batch * MLP_IN)), dim3(BLOCK), 0, s, features, perm,
+                           n, start, batch, m->xs);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(MLP_THREADS), 0, s, m->params, m->norm, targets, perm, n,
+                           start, batch, m->xs, m->partial);
+        hipLaunchKernelGGL(k_mlp_adam, dim3(div_up(MLP_PARAMS + 1, BLOCK)), dim3(BLOCK), 0, s, m->partial, blocks,
+                           m->grad, m->params, m->m1, m->m2, steps ? lr : 0.0f, beta1, beta2, eps, m->step_base, k,
+                           m->loss + k);
+    }
+    if (steps) hipLaunchKernelGGL(k_mlp_steps_done, dim3(1), dim3(64), 0, s, m->step_base, steps);
 }
 
 extern "C" {
@@ -415,10 +458,11 @@ apbf_status apbf_mlp_create(apbf_mlp** out) {
     apbf_mlp* m = new apbf_mlp();
     if (hipMalloc(&m->params, MLP_PARAMS * 4) != hipSuccess || hipMalloc(&m->norm, (2 * MLP_IN + 1) * 4) != hipSuccess ||
         hipMalloc(&m->m1, MLP_PARAMS * 4) != hipSuccess || hipMalloc(&m->m2, MLP_PARAMS * 4) != hipSuccess ||
-        hipMalloc(&m->grad, MLP_PARAMS * 4) != hipSuccess) {
+        hipMalloc(&m->grad, MLP_PARAMS * 4) != hipSuccess || hipMalloc(&m->step_base, 4) != hipSuccess) {
         apbf_mlp_destroy(m);
         return APBF_ERR_HIP;
     }
+    (void)hipMemset(m->step_base, 0, 4);
     (void)hipMemset(m->m1, 0, MLP_PARAMS * 4);
     (void)hipMemset(m->m2, 0, MLP_PARAMS * 4);
     (void)hipMemset(m->params, 0, MLP_PARAMS * 4);
@@ -428,9 +472,13 @@ apbf_status apbf_mlp_create(apbf_mlp** out) {
 
 apbf_status apbf_mlp_destroy(apbf_mlp* m) {
     if (!m) return APBF_ERR_INVALID_ARGUMENT;
+    drop_training_graph(m);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     float* ps[] = {m->params, m->norm, m->m1, m->m2, m->grad, m->partial, m->xs, m->loss};
     for (float* p : ps)
         if (p) (void)hipFree(p);
+    if (m->perm) (void)hipFree(m->perm);
+    if (m->step_base) (void)hipFree(m->step_base);
     delete m;
     return APBF_OK;
 }
@@ -439,9 +487,9 @@ apbf_status apbf_mlp_set_params(apbf_mlp* m, const float* host_params, const flo
     if (!m || !host_params || !host_norm) return APBF_ERR_INVALID_ARGUMENT;
     if (hipMemcpy(m->params, host_params, MLP_PARAMS * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(m->norm, host_norm, (2 * MLP_IN + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(m->m1, 0, MLP_PARAMS * 4) != hipSuccess || hipMemset(m->m2, 0, MLP_PARAMS * 4) != hipSuccess)
+        hipMemset(m->m1, 0, MLP_PARAMS * 4) != hipSuccess || hipMemset(m->m2, 0, MLP_PARAMS * 4) != hipSuccess ||
+        hipMemset(m->step_base, 0, 4) != hipSuccess)
         return APBF_ERR_HIP;
-    m->steps_done = 0;
     return APBF_OK;
 }
 
@@ -464,26 +512,49 @@ apbf_status apbf_mlp_train(apbf_mlp* m, const float* features, const float* targ
                            uint32_t n, uint32_t batch, uint32_t steps, float lr, float* host_losses,
                            apbf_stream stream) {
     if (!m || !features || !targets || !perm || n == 0 || batch == 0 || batch > n) return APBF_ERR_INVALID_ARGUMENT;
-    apbf_status st = mlp_reserve(m, batch, steps ? steps : 1);
+    apbf_status st = mlp_reserve(m, batch, steps ? steps : 1, n);
     if (st != APBF_OK) return st;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const uint32_t blocks = mlp_grid_tiles(batch);
-    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    for (uint32_t k = 0; k < (steps ? steps : 1u); ++k) {
-        const uint32_t start = (uint32_t)(((uint64_t)k * batch) % n);
-        hipLaunchKernelGGL(k_mlp_gather, dim3(stride_grid((uint64_t)batch * MLP_IN)), dim3(BLOCK), 0, s, features, perm,
-                           n, start, batch, m->xs);
-        hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(MLP_THREADS), 0, s, m->params, m->norm, targets, perm, n,
-                           start, batch, m->xs, m->partial);
-        float bc1 = 1.0f, bc2 = 1.0f;
-        if (steps) {
-            ++m->steps_done;
-            bc1 = 1.0f - powf(beta1, (float)m->steps_done);
-            bc2 = 1.0f - powf(beta2, (float)m->steps_done);
+    // The steps read the sample order from the network's own copy, so the
+    // same captured launches serve every call with the same arguments.
+    if (hipMemcpyAsync(m->perm, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return APBF_ERR_HIP;
+    if (steps < 16) {
+        enqueue_training(m, features, targets, m->perm, n, batch, steps, lr, s);
+    } else {
+        // many steps: one graph of the call's launches on the network's own
+        // stream (captured at the first call with these arguments), ordered
+        // after the caller's stream and before its later work
+        if (!m->stream && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return APBF_ERR_HIP;
+        if (!(m->graph && m->g_features == features && m->g_targets == targets && m->g_n == n &&
+              m->g_batch == batch && m->g_steps == steps && m->g_lr == lr)) {
+            drop_training_graph(m);
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return APBF_ERR_HIP;
+            enqueue_training(m, features, targets, m->perm, n, batch, steps, lr, m->stream);
+            if (hipStreamEndCapture(m->stream, &g) != hipSuccess || !g) return APBF_ERR_HIP;
+            const bool ok = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0) == hipSuccess;
+            (void)hipGraphDestroy(g);
+            if (!ok) {
+                m->graph = nullptr;
+                return APBF_ERR_HIP;
+            }
+            m->g_features = features;
+            m->g_targets = targets;
+            m->g_n = n;
+            m->g_batch = batch;
+            m->g_steps = steps;
+            m->g_lr = lr;
         }
-        hipLaunchKernelGGL(k_mlp_adam, dim3(div_up(MLP_PARAMS + 1, BLOCK)), dim3(BLOCK), 0, s, m->partial, blocks,
-                           m->grad, m->params, m->m1, m->m2, steps ? lr : 0.0f, beta1, beta2, eps, bc1, bc2,
-                           m->loss + k);
+        hipEvent_t before = nullptr, after = nullptr;
+        if (hipEventCreateWithFlags(&before, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&after, hipEventDisableTiming) != hipSuccess)
+            return APBF_ERR_HIP;
+        const bool ok = hipEventRecord(before, s) == hipSuccess && hipStreamWaitEvent(m->stream, before, 0) == hipSuccess &&
+                        hipGraphLaunch(m->graph, m->stream) == hipSuccess && hipEventRecord(after, m->stream) == hipSuccess &&
+                        hipStreamWaitEvent(s, after, 0) == hipSuccess;
+        (void)hipEventDestroy(before);
+        (void)hipEventDestroy(after);
+        if (!ok) return APBF_ERR_HIP;
     }
     if (hipGetLastError() != hipSuccess) return APBF_ERR_HIP;
     if (host_losses) {

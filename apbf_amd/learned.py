@@ -49,8 +49,18 @@ def init_params(seed: int = 0) -> np.ndarray:
     return p
 
 
-def fit_norm(features: np.ndarray, targets: np.ndarray) -> np.ndarray:
-    """Feature means and inverse standard deviations, and the target scale."""
+def fit_norm(features, targets) -> np.ndarray:
+    """Feature means and inverse standard deviations, and the target scale
+    (NumPy arrays, or torch tensors reduced on their own device in FP64)."""
+    if type(features).__module__.startswith("torch"):
+        import torch
+        f = features.reshape(-1, FEATURES).to(torch.float64)
+        mean = f.mean(0)
+        std = f.std(0, unbiased=False)
+        scale = torch.where(std > 1e-12, 1.0 / std.clamp_min(1e-12), torch.ones_like(std))
+        t = targets.to(torch.float64)
+        out = float(torch.sqrt(torch.mean(t * t))) or 1.0
+        return np.concatenate([mean.cpu().numpy(), scale.cpu().numpy(), [out]]).astype(np.float32)
     f = np.asarray(features, np.float64).reshape(-1, FEATURES)
     mean = f.mean(0)
     std = f.std(0)
@@ -139,12 +149,12 @@ class WarmStartModel:
         n = features.shape[0]
         batch = min(batch, n)
         steps = max(1, n // batch)
-        g = torch.Generator(device="cpu").manual_seed(seed)
+        g = torch.Generator(device=features.device).manual_seed(seed)
         losses = []
         x = features.contiguous()
         t = targets.contiguous()
         for e in range(epochs):
-            perm = torch.randperm(n, generator=g).to(torch.int32).to(features.device)
+            perm = torch.randperm(n, generator=g, device=features.device).to(torch.int32)
             out = np.zeros(steps, np.float32)
             check(self._L.apbf_mlp_train(self._h, _dp(x), _dp(t), _dp(perm), n, batch, steps, C.c_float(lr),
                                          _fp(out), _stream()), "apbf_mlp_train")
@@ -162,7 +172,7 @@ def fit_warm_start(scene, dt: float, harvest_substeps: int = 6, epochs: int = 12
     f, t = harvest(scene, harvest_substeps, dt)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    norm = fit_norm(f.cpu().numpy(), t.cpu().numpy())
+    norm = fit_norm(f, t)
     model = WarmStartModel(None, norm, seed=seed)
     losses = model.train(f, t, epochs=epochs, batch=batch, lr=lr, seed=seed)
     torch.cuda.synchronize()

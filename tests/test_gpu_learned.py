@@ -57,6 +57,41 @@ def test_mlp_training_reduces_loss(device):
     assert l1 < 0.2 * l0, (l0, l1)
 
 
+def _train_call(m, xd, td, perm, batch, steps, lr):
+    import ctypes as C
+    from apbf_amd import learned as Lm
+    from apbf_amd._lib import check
+    out = np.zeros(max(steps, 1), np.float32)
+    check(m._L.apbf_mlp_train(m.handle, Lm._dp(xd), Lm._dp(td), Lm._dp(perm), xd.shape[0], batch, steps,
+                              C.c_float(lr), Lm._fp(out), Lm._stream()), "apbf_mlp_train")
+    return out
+
+
+def test_captured_training_equals_step_by_step(device):
+    """A call of 24 steps runs as one captured graph (replayed on the second
+    epoch); the same steps as calls of 8 run launch by launch. The sample
+    order of each short call is the long call's rotated by its first step,
+    and the device step count carries the Adam corrections across calls, so
+    parameters and losses must agree bit for bit."""
+    import torch
+    n, batch, steps, chunk = 24 * 512 + 100, 512, 24, 8
+    ma, p, norm, x, t, xd, td = _model_and_data(n=n, seed=11)
+    mb = type(ma)(p, norm)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    la, lb = [], []
+    for epoch in range(2):
+        perm = torch.randperm(n, generator=g).to(torch.int32)
+        la.append(_train_call(ma, xd, td, perm.cuda(), batch, steps, 3e-3))
+        for c in range(steps // chunk):
+            rot = torch.roll(perm, -c * chunk * batch).cuda()
+            lb.append(_train_call(mb, xd, td, rot, batch, chunk, 3e-3))
+    pa, _ = ma.get_params()
+    pb, _ = mb.get_params()
+    assert np.array_equal(np.concatenate(la), np.concatenate(lb))
+    assert np.array_equal(pa, pb)
+    assert not np.array_equal(pa, p)
+
+
 def test_warm_start_gain_on_a_scene(device):
     """Trained on pairs from plain substeps of a small water drop, the warm
     start followed by 2 Jacobi iterations ends each substep with a lower mean
