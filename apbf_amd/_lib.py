@@ -57,6 +57,7 @@ class SceneParams(C.Structure):
         ("cap_particles", C.c_uint32),
         ("cap_transfers", C.c_uint32),
         ("cap_pairs", C.c_uint32),
+        ("dimensions", C.c_uint32),
     ]
 
 
@@ -112,6 +113,7 @@ def make_params(p: dict) -> SceneParams:
     out.cap_particles = int(p["cap_particles"])
     out.cap_transfers = int(p["cap_transfers"])
     out.cap_pairs = int(p["cap_pairs"])
+    out.dimensions = int(p.get("dimensions", 3))
     return out
 
 
@@ -155,10 +157,10 @@ PROTOTYPES = {
     "apbf_prefix_sum": (_i32, [_vp, _vp, _u32, _vp, _sz, _vp]),
     "apbf_radix_sort": (_i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
     "apbf_radix_sort_noalloc": (_i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
-    "apbf_calculate_position_hash": (_i32, [_vp, _vp, _u32, _pf, _pf, _u32, _vp, _vp]),
+    "apbf_calculate_position_hash": (_i32, [_vp, _vp, _u32, _pf, _pf, _u32, _u32, _vp, _vp]),
     "apbf_calculate_position_code": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
     "apbf_find_value_ranges": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp, _vp]),
-    "apbf_neighborhood_green": (_i32, [_vp, _vp, _vp, _u32, _f32, _pf, _pf, _u32, _vp, _vp, _vp, _vp,
+    "apbf_neighborhood_green": (_i32, [_vp, _vp, _vp, _u32, _f32, _pf, _pf, _u32, _u32, _vp, _vp, _vp, _vp,
                                        _u32, _vp, _vp, _sz, _vp]),
     "apbf_neighborhood_brute_force": (_i32, [_vp, _vp, _vp, _u32, _f32, _vp, _vp, _u32, _vp, _vp, _sz, _vp]),
     "apbf_copy_scattered_read": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _vp]),

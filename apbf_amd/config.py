@@ -35,8 +35,9 @@ STATE_KEYS = ("position", "velocity", "inverse_mass", "radius", "pos_backup", "t
               "boundary_distance", "t_source", "t_target", "t_time_left")
 
 
-def walls(box_min, box_max, radius, ceiling=True, thickness=4.0):
-    """Wall boxes around [box_min, box_max] (pool.cpp:30-40, waterdrop.cpp:28-38)."""
+def walls(box_min, box_max, radius, ceiling=True, thickness=4.0, dims: int = 3):
+    """Wall boxes around [box_min, box_max] (pool.cpp:30-40, waterdrop.cpp:28-38);
+    in 2D without the two z walls (pool.cpp:37-40)."""
     mn = np.asarray(box_min, np.float32)
     mx = np.asarray(box_max, np.float32)
     t = np.float32(thickness)
@@ -49,8 +50,9 @@ def walls(box_min, box_max, radius, ceiling=True, thickness=4.0):
     ]
     if ceiling:
         b.append((np.array([wmin[0], wmax[1] - t, wmin[2]], np.float32), wmax))
-    b.append((wmin, np.array([wmax[0], wmax[1], wmin[2] + t], np.float32)))
-    b.append((np.array([wmin[0], wmin[1], wmax[2] - t], np.float32), wmax))
+    if dims > 2:
+        b.append((wmin, np.array([wmax[0], wmax[1], wmin[2] + t], np.float32)))
+        b.append((np.array([wmin[0], wmin[1], wmax[2] - t], np.float32), wmax))
     return [(tuple(float(x) for x in lo), tuple(float(x) for x in hi)) for lo, hi in b]
 
 
@@ -74,7 +76,8 @@ def _fixed(xyz: np.ndarray) -> np.ndarray:
 
 def make_state(pos_world: np.ndarray, radius: np.ndarray, bd: np.ndarray | None = None,
                boundariness: np.ndarray | None = None, target_radius: np.ndarray | None = None,
-               velocity: np.ndarray | None = None) -> dict:
+               velocity: np.ndarray | None = None, dims: int = 3) -> dict:
+    """Inverse mass 1 / (2r)^D (initialize.cpp:25,60 at inverse density 1)."""
     n = len(pos_world)
     radius = np.asarray(radius, np.float32)
     pos = np.zeros((n, 4), np.int32)
@@ -82,7 +85,7 @@ def make_state(pos_world: np.ndarray, radius: np.ndarray, bd: np.ndarray | None 
     vel = np.zeros((n, 4), np.float32)
     if velocity is not None:
         vel[:, :3] = velocity
-    inv_mass = (np.float32(1.0) / (np.float32(2.0) * radius) ** 3).astype(np.float32)
+    inv_mass = (np.float32(1.0) / (np.float32(2.0) * radius) ** dims).astype(np.float32)
     st = dict(
         position=pos, velocity=vel, inverse_mass=inv_mass, radius=radius.copy(),
         pos_backup=pos.copy(), transferring=np.zeros(n, np.uint32),
@@ -138,20 +141,24 @@ def dam_break(n_side: int = 32, spacing: float = 0.5, seed: int = 0, box_scale=(
     return p, st
 
 
-def _sphere_points(center, radius_world, spacing, jitter, rng):
+def _sphere_points(center, radius_world, spacing, jitter, rng, dims: int = 3):
+    """Lattice points inside a ball (in 2D: a disc in the x-y plane)."""
     m = int(math.ceil(radius_world / spacing)) + 1
     g = np.arange(-m, m + 1) * spacing
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    X, Y, Z = np.meshgrid(g, g, g if dims == 3 else np.zeros(1), indexing="ij")
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
     pts = pts[np.linalg.norm(pts, axis=1) <= radius_world - spacing * 0.5]
     if jitter > 0:
         pts = pts + rng.uniform(-jitter, jitter, pts.shape)
+        if dims == 2:
+            pts[:, 2] = 0.0
     return pts + np.asarray(center, np.float64)
 
 
 def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coarse: float = 1.0,
                r_fine: float = 0.5, seed: int = 0, iterations: int = 4, res: int = 6, cap_factor: float = 1.5,
-               pairs_per_particle: int = 800, target_scale: float | None = None, merge_duration: float = 2.0):
+               pairs_per_particle: int = 800, target_scale: float | None = None, merge_duration: float = 2.0,
+               dims: int = 3):
     """Adaptive water drop: a sphere of `drop_particles` fine particles (radius
     r_fine = 0.5h) released at t = 0 above a pool of coarse particles (radius
     r_coarse = 1h, a pool_counts lattice at spacing 2h: 150 x 40 x 150 = 900 000
@@ -167,38 +174,52 @@ def water_drop(pool_counts=(150, 40, 150), drop_particles: int = 100_000, r_coar
     about 1 200 entries per particle in the neighbour walk), leaves almost no
     row longer than its slot, so the exact refill has nearly nothing to do;
     the list never reaches the capacity, so the results do not depend on it
-    (400: 7.98 against 7.86 ms per substep, same states)."""
+    (400: 7.98 against 7.86 ms per substep, same states).
+
+    dims=2 builds the planar version of the reference's default build
+    (DIMENSIONS 2): one layer of pool particles at z = r_coarse (pool_counts[2]
+    is ignored), a disc-shaped drop in the same plane, no z walls, inverse
+    masses 1 / (2r)^2."""
     rng = np.random.default_rng(seed)
     sc = 2 * r_coarse
+    if dims == 2:
+        pool_counts = (pool_counts[0], pool_counts[1], 1)
     pool_max = tuple(c * sc for c in pool_counts)
     pool = lattice((0, 0, 0), pool_counts, sc, 0.05, rng)
     sf = 2 * r_fine
-    drop_r = (drop_particles * sf ** 3 * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) + sf * 0.5
-    center = (pool_max[0] * 0.5, pool_max[1] + 8.0 + drop_r, pool_max[2] * 0.5)
-    drop = _sphere_points(center, drop_r, sf, 0.05, rng)
+    if dims == 2:
+        pool[:, 2] = r_coarse
+        drop_r = math.sqrt(drop_particles * sf ** 2 / math.pi) + sf * 0.5
+        center = (pool_max[0] * 0.5, pool_max[1] + 8.0 + drop_r, r_coarse)
+    else:
+        drop_r = (drop_particles * sf ** 3 * 3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) + sf * 0.5
+        center = (pool_max[0] * 0.5, pool_max[1] + 8.0 + drop_r, pool_max[2] * 0.5)
+    drop = _sphere_points(center, drop_r, sf, 0.05, rng, dims)
     pos = np.concatenate([pool, drop], 0)
     radius = np.concatenate([np.full(len(pool), r_coarse), np.full(len(drop), r_fine)]).astype(np.float32)
-    d_pool = np.minimum.reduce([pool[:, 0], pool_max[0] - pool[:, 0], pool[:, 1], pool_max[1] - pool[:, 1],
-                                pool[:, 2], pool_max[2] - pool[:, 2]])
+    faces = [pool[:, 0], pool_max[0] - pool[:, 0], pool[:, 1], pool_max[1] - pool[:, 1]]
+    if dims == 3:
+        faces += [pool[:, 2], pool_max[2] - pool[:, 2]]
+    d_pool = np.minimum.reduce(faces)
     d_drop = drop_r - np.linalg.norm(drop - np.asarray(center), axis=1)
     bd = np.maximum(np.concatenate([d_pool, d_drop]), radius)
     if target_scale is None:
-        target_scale = r_coarse / (0.5 * min(pool_max))
+        target_scale = r_coarse / (0.5 * min(pool_max[:dims]))
     s = dict(REFERENCE_SETTINGS)
     s.update(mBaseKernelWidthOnBoundaryDistance=0, mSmallestTargetRadius=r_coarse,
              mTargetRadiusOffset=0.0, mTargetRadiusScaleFactor=target_scale, mMergeDuration=merge_duration)
     tr = (r_coarse + bd * target_scale).astype(np.float32)
     bness = (bd < 2.0 * radius).astype(np.float32)
-    st = make_state(pos, radius, bd=bd, boundariness=bness, target_radius=tr)
+    st = make_state(pos, radius, bd=bd, boundariness=bness, target_radius=tr, dims=dims)
     n = len(pos)
     top = center[1] + drop_r + 4
     cap = int(n * cap_factor)
     p = base_params(
         settings=s, basic_pbf=0, solver_iterations=iterations,
         hash_min=(-4.0, -4.0, -4.0), hash_max=(pool_max[0] + 4.0, top, pool_max[2] + 4.0), hash_res_log2=res,
-        boxes=walls((0, 0, 0), pool_max, r_coarse, ceiling=False),
+        boxes=walls((0, 0, 0), pool_max, r_coarse, ceiling=False, dims=dims),
         cap_particles=cap, cap_transfers=max(1000, n // 4), cap_pairs=cap * pairs_per_particle,
-        workload=f"water_drop_{n}",
+        workload=f"water_drop{'_2d' if dims == 2 else ''}_{n}", dimensions=dims,
     )
     return p, st
 
@@ -288,24 +309,37 @@ def settled_water_drop(pool_counts=(180, 50, 180), drop_particles: int = 100_000
     return p, both, report
 
 
-def small_water_drop(seed: int = 0, pool_counts=(24, 10, 24), drop_particles: int = 2000, res: int = 4,
-                     active: bool = False):
-    """A ~8k-particle version of :func:`water_drop` for parity tests. With
+def small_water_drop(seed: int = 0, pool_counts=None, drop_particles: int | None = None, res: int | None = None,
+                     active: bool = False, dims: int = 3):
+    """A ~8k-particle version of :func:`water_drop` for parity tests (2D: a
+    80 x 30 pool layer and a 600-particle disc, 3 000 particles). With
     active=True the smallest target radius is the fine radius, so most pool
     particles split (a stress case for split, transfer and claim handling)."""
+    if pool_counts is None:
+        pool_counts = (24, 10, 24) if dims == 3 else (80, 30, 1)
+    if drop_particles is None:
+        drop_particles = 2000 if dims == 3 else 600
+    if res is None:
+        res = 4 if dims == 3 else 6
     p, st = water_drop(pool_counts=pool_counts, drop_particles=drop_particles, seed=seed, res=res,
-                       pairs_per_particle=400 if active else 200, cap_factor=3.0 if active else 2.0)
+                       pairs_per_particle=400 if active else 200, cap_factor=3.0 if active else 2.0, dims=dims)
     if active:
         p["settings"]["mSmallestTargetRadius"] = 0.5
         p["settings"]["mTargetRadiusScaleFactor"] = 0.05
     return p, st
 
 
-def box_shape(mn, mx, radius: float):
+def box_shape(mn, mx, radius: float, dims: int = 3):
     """initialize::add_box_shape (initialize.cpp:5-33, initialize_box.comp):
-    floor((max - min) / 2r) particles per axis, centred at min + r + 2r * index."""
-    mn = np.asarray(mn, np.float64)
-    counts = np.floor((np.asarray(mx, np.float64) - mn) / (2.0 * radius)).astype(int)
+    floor((max - min) / 2r) particles per axis, centred at min + r + 2r * index.
+    In 2D the z range is first narrowed to [mid - r, mid + 1.5 r] around the
+    box's middle, one layer (initialize.cpp:16-19)."""
+    mn = np.asarray(mn, np.float64).copy()
+    mx = np.asarray(mx, np.float64).copy()
+    if dims < 3:
+        mid = (mn[2] + mx[2]) / 2.0
+        mn[2], mx[2] = mid - radius, mid + radius * 1.5
+    counts = np.floor((mx - mn) / (2.0 * radius)).astype(int)
     g = np.stack(np.meshgrid(*[np.arange(c) for c in counts], indexing="ij"), -1).reshape(-1, 3)
     return mn + radius + 2.0 * radius * g
 
@@ -338,51 +372,61 @@ def _sphere_shells(center, radius: float, particle_radius: float, dims: int = 3)
     return pts
 
 
-def sphere_shape(center, outer_radius: float, particle_radius: float):
+def sphere_shape(center, outer_radius: float, particle_radius: float, dims: int = 3):
     """initialize::add_sphere_shape(center, outer, 0, true, r) (initialize.cpp:35-69):
     concentric shells from the outside in, each shell's radius chosen so the
-    particles placed so far fill the volume between it and the outer surface."""
-    a = outer_radius ** 3
-    b = -(particle_radius ** 3) * 2 * 3 / math.pi
+    particles placed so far fill the volume between it and the outer surface
+    (in 2D: rings in the x-y plane filling the disc's area)."""
+    a = outer_radius ** dims
+    b = -(particle_radius ** dims) * 2 * dims / math.pi
     shell = outer_radius - particle_radius
     out = []
     while 0.0 <= shell <= outer_radius - particle_radius:
-        out += _sphere_shells(center, shell, particle_radius)
+        out += _sphere_shells(center, shell, particle_radius, dims)
         if shell < particle_radius:
             break  # the centre particle is placed
         v = a + b * len(out)
-        shell = (v ** (1.0 / 3.0) if v >= 0 else math.nan) - particle_radius  # pow of a negative: NaN, the loop ends
+        shell = (v ** (1.0 / dims) if v >= 0 else math.nan) - particle_radius  # pow of a negative: NaN, the loop ends
     return np.array(out, np.float64)
 
 
-def _reference_scene(pos, r, res, hash_min, hash_max, boxes, name, cap_pairs_per_particle=64):
+def _reference_scene(pos, r, res, hash_min, hash_max, boxes, name, cap_pairs_per_particle=64, dims=3):
     n = len(pos)
-    st = make_state(pos, np.full(n, r, np.float32))
+    st = make_state(pos, np.full(n, r, np.float32), dims=dims)
     s = dict(REFERENCE_SETTINGS)
     s["mSmallestTargetRadius"] = r
+    if dims == 2:
+        s["mTargetRadiusScaleFactor"] = 0.4  # settings.cpp:26
     p = base_params(
         settings=s, basic_pbf=0, solver_iterations=3, gravity=(0.0, -10.0, 0.0),
         hash_min=tuple(float(x) for x in hash_min), hash_max=tuple(float(x) for x in hash_max), hash_res_log2=res,
         boxes=boxes, cap_particles=max(2 * n, 16), cap_transfers=max(n, 16),
-        cap_pairs=max(2 * n, 16) * cap_pairs_per_particle, workload=f"{name}_{n}",
+        cap_pairs=max(2 * n, 16) * cap_pairs_per_particle, workload=f"{name}{'_2d' if dims == 2 else ''}_{n}",
+        dimensions=dims,
     )
     return p, st
 
 
-def pool_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4):
+def pool_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4,
+               dims: int = 3):
     """The reference's pool (pool.cpp:6-56 as main.cpp:62 builds it): a box of
     particles filling [mn, mx], six walls 4 units thick around it, gravity
     (0, -10, 0), the hash domain [mn, mx] at 2^4 cells per axis, the reference
-    settings (adaptive, boundary-distance kernel widths)."""
-    pos = box_shape(mn, mx, radius)
-    return _reference_scene(pos, radius, res, mn, mx, walls(mn, mx, radius, ceiling=True), "pool")
+    settings (adaptive, boundary-distance kernel widths). dims=2 is the
+    reference's default build (DIMENSIONS 2, cpu_gpu_shared_config.h:2): one
+    particle layer, four walls, targetRadiusScaleFactor 0.4."""
+    pos = box_shape(mn, mx, radius, dims)
+    return _reference_scene(pos, radius, res, mn, mx, walls(mn, mx, radius, ceiling=True, dims=dims), "pool",
+                            dims=dims)
 
 
-def waterfall_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4):
+def waterfall_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: float = 1.0, res: int = 4,
+                    dims: int = 3):
     """The reference's waterfall (waterfall.cpp:6-67): the pool's particles and
     an upper basin whose x-max side is open, over a lower basin shifted by
-    (min - max) in x and y; eleven wall boxes in 3D."""
-    pos = box_shape(mn, mx, radius)
+    (min - max) in x and y; eleven wall boxes in 3D, seven in 2D (no z walls,
+    waterfall.cpp:41-47)."""
+    pos = box_shape(mn, mx, radius, dims)
     t = 4.0
     w0 = np.asarray(mn, np.float64) - t
     w1 = np.asarray(mx, np.float64) + t + np.array([0.0, 2 * radius, 0.0])
@@ -394,27 +438,29 @@ def waterfall_scene(mn=(-40.0, -10.0, -80.0), mx=(40.0, 30.0, -40.0), radius: fl
          (np.array([w0[0], w1[1] - t, w0[2]]), w1),
          (shift + w0, np.array([w0[0] + t, w1[1], w1[2]]) + shift),
          (shift + w0, np.array([w1[0], w0[1] + t, w1[2]]) + shift),
-         (shift + np.array([w1[0] - t, w0[1], w0[2]]), w1 + shift),
-         (w0, np.array([w1[0], w1[1], w0[2] + t])),
-         (np.array([w0[0], w0[1], w1[2] - t]), w1),
-         (shift + w0, np.array([w1[0], w1[1], w0[2] + t]) + shift),
-         (shift + np.array([w0[0], w0[1], w1[2] - t]), w1 + shift)]
+         (shift + np.array([w1[0] - t, w0[1], w0[2]]), w1 + shift)]
+    if dims > 2:
+        b += [(w0, np.array([w1[0], w1[1], w0[2] + t])),
+              (np.array([w0[0], w0[1], w1[2] - t]), w1),
+              (shift + w0, np.array([w1[0], w1[1], w0[2] + t]) + shift),
+              (shift + np.array([w0[0], w0[1], w1[2] - t]), w1 + shift)]
     boxes = [(tuple(float(x) for x in lo), tuple(float(x) for x in hi)) for lo, hi in b]
-    return _reference_scene(pos, radius, res, mn, mx, boxes, "waterfall")
+    return _reference_scene(pos, radius, res, mn, mx, boxes, "waterfall", dims=dims)
 
 
 def spherical_pool_scene(center=(0.0, 10.0, -60.0), pool_radius: float = 100.0, particle_radius: float = 1.0,
-                         res: int = 4, max_particles: int | None = None):
+                         res: int = 4, max_particles: int | None = None, dims: int = 3):
     """The reference's spherical pool (spherical_pool.cpp:6-45): a ball of
     particles on concentric shells inside a hollow sphere collider of the pool
     radius, gravity (0, -10, 0), the hash domain center +- pool radius at 2^4
     cells per axis. max_particles keeps the first particles (the reference's
     100 000-particle capacity clamps a 3D ball of radius 100)."""
-    pos = sphere_shape(center, pool_radius, particle_radius)
+    pos = sphere_shape(center, pool_radius, particle_radius, dims)
     if max_particles is not None:
         pos = pos[:max_particles]
     c = np.asarray(center, np.float64)
-    p, st = _reference_scene(pos, particle_radius, res, c - pool_radius, c + pool_radius, [], "spherical_pool")
+    p, st = _reference_scene(pos, particle_radius, res, c - pool_radius, c + pool_radius, [], "spherical_pool",
+                             dims=dims)
     p["sphere"] = (tuple(float(x) for x in center), float(pool_radius), True)
     return p, st
 

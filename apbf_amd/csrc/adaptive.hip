@@ -48,7 +48,7 @@ __global__ __launch_bounds__(BLOCK) void k_particle_transfer(const uint32_t* __r
                                                              float* __restrict__ inv_mass,
                                                              uint32_t* __restrict__ transferring,
                                                              uint32_t* __restrict__ del_p,
-                                                             uint32_t* __restrict__ del_t, float dt) {
+                                                             uint32_t* __restrict__ del_t, float dt, float dim) {
     const uint32_t nt = counts[C_TRANSFERS];
     GRID_STRIDE(t, nt) {
         float tt = ttl_arr[t];
@@ -57,14 +57,14 @@ __global__ __launch_bounds__(BLOCK) void k_particle_transfer(const uint32_t* __r
         uint32_t s = src[t], g = tgt[t];
         float rS = radius[s], rT = radius[g];
         float imS = inv_mass[s], imT = inv_mass[g];
-        float volS = pow_f(rS, DIM), volT = pow_f(rT, DIM);
+        float volS = pow_f(rS, dim), volT = pow_f(rT, dim);
         float f = merge ? fmin_g(1.0f, dt / ttl) : dt * (volS - volT) / (2.0f * ttl * volS);
         float tv = f * volS;
         float nf = 1.0f / (imS + f * imT);
         inv_mass[s] = imS / (1.0f - f);
         inv_mass[g] = imS * imT * nf;
-        radius[s] = pow_f(volS - tv, 1.0f / DIM);
-        radius[g] = pow_f(volT + tv, 1.0f / DIM);
+        radius[s] = pow_f(volS - tv, 1.0f / dim);
+        radius[g] = pow_f(volT + tv, 1.0f / dim);
         ttl_arr[t] = (ttl - dt) * (merge ? 1.0f : -1.0f);
         uint32_t dt_flag = 0;
         if (ttl == dt) {
@@ -196,7 +196,7 @@ __global__ void k_set_pairs_from_row(uint32_t* __restrict__ counts, const uint32
 
 struct Fsm3Args {
     int update_tr, base_bd, split, merge;
-    float smallest, offset, scale, merge_duration, split_factor;
+    float smallest, offset, scale, merge_duration, split_factor, dim;
 };
 
 // find_split_and_merge_3.comp:44-90: target radius, boundary-distance decay,
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(BLOCK) void k_fsm3(const int4* __restrict__ pos,
         bd[i] = to_u32((bdist * (1.0f - b) + r * b) * POS_RES);
         boundariness[i] = b;
         const bool split = a.split && (tr * a.split_factor <= r);
-        const bool merge = a.merge && nnDist < r && pow_f(r, DIM) + pow_f(nnR, DIM) <= pow_f(tr, DIM);
+        const bool merge = a.merge && nnDist < r && pow_f(r, a.dim) + pow_f(nnR, a.dim) <= pow_f(tr, a.dim);
         uint32_t type = merge ? P_MERGE : (split ? P_SPLIT : P_NONE);
         uint32_t s = (merge && nnLarger) ? nn : i;
         uint32_t t = (merge && nnLarger) ? i : nn;
@@ -537,7 +537,7 @@ void Scene::particle_transfer(float dt) {
     uint32_t* del_p = u1;
     fill_u32_dev(del_p, 0u, counts + C_N, cap, s);
     LAUNCH(k_particle_transfer, stride_grid(tcap), t_source[tcur], t_target[tcur], t_time_left[tcur], counts,
-           radius[cur], inv_mass[cur], transferring[cur], del_p, t_flag, dt);
+           radius[cur], inv_mass[cur], transferring[cur], del_p, t_flag, dt, dim());
     // particle compaction map
     LAUNCH(k_invert_flags, stride_grid(cap), del_p, u2, counts);
     scan_u32(u2, u3, counts + C_N, cap, false, counts + C_KEEP, ws_mem, s);
@@ -606,10 +606,10 @@ void Scene::update_transfers(bool with_density_error) {
         // radii and masses (it moves positions only)
         if (P.solver_iterations <= 0)
             launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                                 P.s.mGradientKernelId, gc, s);
+                                 P.s.mGradientKernelId, dim(), gc, s);
         launch_fsm12_density_rows(pos[cur], inv_mass[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
                                   old_bd, boundary_distance[cur], nearest, P.s.mHeightKernelId,
-                                  P.s.mGradientKernelId, part_sum, part_max, s);
+                                  P.s.mGradientKernelId, dim(), part_sum, part_max, s);
         launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
     } else {
         launch_fsm12_rows(pos[cur], counts, cap, row, col, pcap, old_bd, boundary_distance[cur], nearest, s);
@@ -624,7 +624,8 @@ void Scene::update_transfers(bool with_density_error) {
     fa.offset = P.s.mTargetRadiusOffset;
     fa.scale = P.s.mTargetRadiusScaleFactor;
     fa.merge_duration = P.s.mMergeDuration;
-    fa.split_factor = pow_f(2.0f, 1.0f / DIM) * 0.99f;
+    fa.split_factor = pow_f(2.0f, 1.0f / dim()) * 0.99f;
+    fa.dim = dim();
     LAUNCH(k_fsm3, stride_grid(cap), pos[cur], radius[cur], boundariness[cur], boundary_distance[cur],
            target_radius[cur], nearest, transferring[cur], counts, fa, a_type, a_s, a_t, a_state, a_flag);
     // candidate list in id order

@@ -9,13 +9,15 @@
 
 namespace apbf {
 
-// shaders/cpu_gpu_shared_config.h:7-17, DIMENSIONS = 3
+// shaders/cpu_gpu_shared_config.h:7-17 (DIMENSIONS is a scene parameter)
+// A hash grid resolution the cell tables can index: 2^(D res) cells, at
+// most 2^30.
+inline bool hash_res_ok(uint32_t res, uint32_t dims) { return res >= 1u && res * dims <= 30u; }
 constexpr float POS_RES = 262144.0f;
 constexpr float KW_RES = 262144.0f;
 constexpr float INCOMP_RES = 262144.0f;
 constexpr float KERNEL_SCALE = 4.0f;
 constexpr float KW_PROPAGATION = 0.5f;
-constexpr float DIM = 3.0f;
 constexpr float PI_F = 3.14159265f;
 constexpr float INCOMP_EPSILON = 0.01f;
 // initialize_split_particles.comp:26 stores the GLSL integer 1/0, which glslang

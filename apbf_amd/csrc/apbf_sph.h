@@ -1,4 +1,6 @@
-// SPH smoothing kernels of the reference (shaders/kernels.glsl:3-123), 3D.
+// SPH smoothing kernels of the reference (shaders/kernels.glsl:3-123). `dim`
+// is DIMENSIONS (cpu_gpu_shared_config.h:2), 2.0f or 3.0f; the cubic, poly6
+// and spiky kernels carry their 3D constants in either (as the reference).
 //
 // The reference evaluates kernel_height / kernel_gradient per pair and
 // recomputes the Gaussian constants from the kernel width on every call; they
@@ -14,11 +16,11 @@ struct GaussConst {
     float norm;  // pow(pow(height, 2/D), D/2)  (kernels.glsl:88)
 };
 
-APBF_HD GaussConst gauss_const(float kw) {
+APBF_HD GaussConst gauss_const(float kw, float dim) {
     const float pi = 3.14159265f;
-    float height = 0.6f / pow_f(kw / 2.0f, 3.0f);  // kernels.glsl:104,117
-    float ivdwp = pow_f(height, 2.0f / 3.0f);
-    return GaussConst{ivdwp * pi, pow_f(ivdwp, 3.0f / 2.0f)};
+    float height = 0.6f / pow_f(kw / 2.0f, dim);  // kernels.glsl:104,117
+    float ivdwp = pow_f(height, 2.0f / dim);
+    return GaussConst{ivdwp * pi, pow_f(ivdwp, dim / 2.0f)};
 }
 
 APBF_HD float gauss_height(v3f r, GaussConst g) {  // kernels.glsl:84-89
@@ -71,34 +73,37 @@ APBF_HD v3f cubic_gradient(v3f r, float h) {  // kernels.glsl:33-46
     float factor = 1.0f - q;
     return gradq * (l * (-factor * factor));
 }
-APBF_HD float cone_height(v3f r, float h) {  // kernels.glsl:48-52
+APBF_HD float cone_height(v3f r, float h, float dim) {  // kernels.glsl:48-52
     const float pi = 3.14159265f;
-    float height = 3.0f / (pi * pow_f(h, 3.0f));
+    float height = 3.0f / (pi * pow_f(h, dim));
     return fmax_g(0.0f, (1.0f - length(r) / h) * height);
 }
-APBF_HD v3f cone_gradient(v3f r, float h) {  // kernels.glsl:54-60
+APBF_HD v3f cone_gradient(v3f r, float h, float dim) {  // kernels.glsl:54-60
     const float pi = 3.14159265f;
-    float steep = 3.0f / (pi * pow_f(h, 4.0f));
+    float steep = 3.0f / (pi * pow_f(h, dim + 1.0f));
     float dist = length(r);
     if (dist > h || dist < 0.0001f) return mk(0, 0, 0);
     return (-r) * (steep / dist);
 }
-APBF_HD float quadratic_spike_height(v3f r, float h) {  // kernels.glsl:62-70
+// kernels.glsl:64-68,76-80: the 3D and 2D normalisations
+APBF_HD float quadratic_spike_a(float h, float dim) {
     const float pi = 3.14159265f;
-    float a = 15.0f / (2.0f * pi * pow_f(h, 5.0f));
+    return dim == 3.0f ? 15.0f / (2.0f * pi * pow_f(h, 5.0f)) : 6.0f / (pi * pow_f(h, 4.0f));
+}
+APBF_HD float quadratic_spike_height(v3f r, float h, float dim) {  // kernels.glsl:62-70
+    float a = quadratic_spike_a(h, dim);
     return a * pow_f(fmin_g(0.0f, length(r) - h), 2.0f);
 }
-APBF_HD v3f quadratic_spike_gradient(v3f r, float h) {  // kernels.glsl:72-82
-    const float pi = 3.14159265f;
+APBF_HD v3f quadratic_spike_gradient(v3f r, float h, float dim) {  // kernels.glsl:72-82
     float dist = length(r);
     if (dist > h || dist < 0.0001f) return mk(0, 0, 0);
-    float a = 15.0f / (2.0f * pi * pow_f(h, 5.0f));
+    float a = quadratic_spike_a(h, dim);
     return r * (-2.0f * a * fmax_g(0.0f, h - dist) / dist);
 }
 
 // kernel_height / kernel_gradient dispatch (kernels.glsl:99-123). Returns W
 // and/or grad W for the pair offset r and kernel width kw.
-APBF_HD void kernel_eval(int hid, int gid, v3f r, float kw, GaussConst g, bool need_w, bool need_g,
+APBF_HD void kernel_eval(int hid, int gid, float dim, v3f r, float kw, GaussConst g, bool need_w, bool need_g,
                          float* w, v3f* grad) {
     float gh = 0.0f;
     bool have_gh = false;
@@ -112,9 +117,9 @@ APBF_HD void kernel_eval(int hid, int gid, v3f r, float kw, GaussConst g, bool n
         } else if (hid == 2) {
             *w = poly6_height(r, kw);
         } else if (hid == 3) {
-            *w = cone_height(r, kw);
+            *w = cone_height(r, kw, dim);
         } else if (hid == 4) {
-            *w = quadratic_spike_height(r, kw);
+            *w = quadratic_spike_height(r, kw, dim);
         } else {
             *w = 0.0f;
         }
@@ -128,9 +133,9 @@ APBF_HD void kernel_eval(int hid, int gid, v3f r, float kw, GaussConst g, bool n
         } else if (gid == 2) {
             *grad = spiky_gradient(r, kw);
         } else if (gid == 3) {
-            *grad = cone_gradient(r, kw);
+            *grad = cone_gradient(r, kw, dim);
         } else if (gid == 4) {
-            *grad = quadratic_spike_gradient(r, kw);
+            *grad = quadratic_spike_gradient(r, kw, dim);
         } else {
             *grad = mk(0, 0, 0);
         }

@@ -243,10 +243,13 @@ apbf_status apbf_stream_synchronize(apbf_stream stream) {
 }
 
 apbf_status apbf_calculate_position_hash(const int32_t* position, const uint32_t* d_len, uint32_t capacity,
-                                         const float* mn, const float* mx, uint32_t res_log2,
+                                         const float* mn, const float* mx, uint32_t res_log2, uint32_t dimensions,
                                          uint32_t* out_hash, apbf_stream stream) {
-    if (!position || !d_len || !mn || !mx || !out_hash || res_log2 > 10) return APBF_ERR_INVALID_ARGUMENT;
+    if (!position || !d_len || !mn || !mx || !out_hash || (dimensions != 2 && dimensions != 3) ||
+        res_log2 * dimensions > 30)
+        return APBF_ERR_INVALID_ARGUMENT;
     GridDesc g;
+    g.dims = dimensions;
     for (int d = 0; d < 3; ++d) {
         g.mn[d] = mn[d];
         g.mx[d] = mx[d];
@@ -281,11 +284,12 @@ size_t apbf_neighborhood_green_workspace_size(uint32_t capacity) {
 
 apbf_status apbf_neighborhood_green(const int32_t* position, const float* range, const uint32_t* d_len,
                                     uint32_t capacity, float range_scale, const float* mn, const float* mx,
-                                    uint32_t res_log2, const uint32_t* cell_start, const uint32_t* cell_end,
-                                    uint32_t* row_offset, uint32_t* neighbors, uint32_t pair_capacity,
-                                    uint32_t* d_n_pairs, void* ws, size_t ws_bytes, apbf_stream stream) {
+                                    uint32_t res_log2, uint32_t dimensions, const uint32_t* cell_start,
+                                    const uint32_t* cell_end, uint32_t* row_offset, uint32_t* neighbors,
+                                    uint32_t pair_capacity, uint32_t* d_n_pairs, void* ws, size_t ws_bytes,
+                                    apbf_stream stream) {
     if (!position || !range || !d_len || !mn || !mx || !cell_start || !cell_end || !row_offset ||
-        !neighbors || !d_n_pairs || !ws || res_log2 > 10)
+        !neighbors || !d_n_pairs || !ws || (dimensions != 2 && dimensions != 3) || res_log2 * dimensions > 30)
         return APBF_ERR_INVALID_ARGUMENT;
     // workspace: count block + counts per particle (capacity + 1) + scan workspace
     if (ws_bytes < apbf_neighborhood_green_workspace_size(capacity)) return APBF_ERR_WORKSPACE;
@@ -294,6 +298,7 @@ apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
     uint32_t* cnt = counts + C_COUNT;
     void* scan_ws = cnt + capacity + 1;
     GridDesc g;
+    g.dims = dimensions;
     for (int d = 0; d < 3; ++d) {
         g.mn[d] = mn[d];
         g.mx[d] = mx[d];

@@ -294,7 +294,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
         s->walk_lanes = bw;
         s->solve_lanes = bs;
         s->update(dt);
-        const uint32_t upper = s->packed() ? (s->n_seg << s->seg_shift) : (1u << (s->grid.res * 3u));
+        const uint32_t upper = s->packed() ? (s->n_seg << s->seg_shift) : (1u << (s->grid.res * s->dims()));
         float best_sort_ms = 1e30f;
         for (uint32_t bits : {4u, 8u, 11u})
             for (uint32_t items : {4u, 8u, 16u}) {

@@ -295,9 +295,9 @@ apbf_status apbf_op_kernel_width_from_boundary_distance(apbf_context* c, const a
 apbf_status apbf_op_neighborhood(apbf_context* c, const apbf_particle_lists* p, const apbf_transfer_lists* t,
                                  const apbf_neighbor_lists* nb, int32_t method, float range_scale,
                                  const float* min3, const float* max3, uint32_t res_log2, apbf_stream stream) {
-    if (!nb || method < 0 || method > 2 || !min3 || !max3 || res_log2 == 0 || res_log2 > 10)
-        return APBF_ERR_INVALID_ARGUMENT;
-    if (c && c->sc && (1u << (3u * res_log2)) != c->sc->n_cells) return APBF_ERR_CAPACITY;  // cell tables
+    if (!nb || method < 0 || method > 2 || !min3 || !max3 || res_log2 == 0) return APBF_ERR_INVALID_ARGUMENT;
+    if (c && c->sc && !hash_res_ok(res_log2, c->sc->dims())) return APBF_ERR_INVALID_ARGUMENT;
+    if (c && c->sc && (1u << (c->sc->dims() * res_log2)) != c->sc->n_cells) return APBF_ERR_CAPACITY;  // cell tables
     return run(c, stream, p, t, nb, [&](Scene* s) {
         for (int d = 0; d < 3; ++d) {
             s->grid.mn[d] = min3[d];

@@ -57,7 +57,7 @@ void launch_fix_clamped_rows(const uint32_t* counts, uint32_t cap, uint32_t pcap
 void launch_pairs_total(uint32_t* counts, const uint32_t* row, uint32_t pair_cap, hipStream_t s);
 void launch_set_n_plus_1(uint32_t* counts, hipStream_t s);
 void launch_kernel_consts(const float* kw, const float* radius, const float* inv_mass, const uint32_t* counts,
-                          uint32_t cap, int hid, int gid, float4* gc,
+                          uint32_t cap, int hid, int gid, float dim, float4* gc,
                           hipStream_t s);
 void launch_sphere(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap, const BoxSet& c,
                    hipStream_t s);
@@ -85,12 +85,12 @@ void launch_solve_stats(const uint32_t* counts, unsigned long long* stats, uint3
 uint32_t density_rows_blocks(uint32_t cap);
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                          const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                         const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
+                         const uint32_t* col, uint32_t pcap, int hid, int gid, float dim, double* part_sum,
                          float* part_max, hipStream_t s);
 void launch_fsm12_density_rows(const int4* pos, const float* inv_mass, const float* kw, const float4* gc,
                                const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
                                uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest, int hid,
-                               int gid, double* part_sum, float* part_max, hipStream_t s);
+                               int gid, float dim, double* part_sum, float* part_max, hipStream_t s);
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest,
                        hipStream_t s);

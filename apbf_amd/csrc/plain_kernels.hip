@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BLOCK) void k_hash_iota(const int4* __restrict__ po
         uint32_t gx = grid_cell(p.x, g.mn[0], g.mx[0], g.res);
         uint32_t gy = grid_cell(p.y, g.mn[1], g.mx[1], g.res);
         uint32_t gz = grid_cell(p.z, g.mn[2], g.mx[2], g.res);
-        keys[i] = zhash(gx, gy, gz, g.res);
+        keys[i] = zhash(gx, gy, gz, g.res, g.dims);
         if (vals) vals[i] = i;
     }
 }
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(BLOCK) void k_hash_iota_seg(const int4* __restrict_
         uint32_t gx = grid_cell(p.x, g.mn[0], g.mx[0], g.res);
         uint32_t gy = grid_cell(p.y, g.mn[1], g.mx[1], g.res);
         uint32_t gz = grid_cell(p.z, g.mn[2], g.mx[2], g.res);
-        keys[i] = (sc << seg.shift) | zhash(gx, gy, gz, g.res);
+        keys[i] = (sc << seg.shift) | zhash(gx, gy, gz, g.res, g.dims);
         if (vals) vals[i] = i;
     }
 }
@@ -189,17 +189,17 @@ __global__ __launch_bounds__(BLOCK) void k_kernel_consts(const float* __restrict
                                                          const float* __restrict__ radius,
                                                          const float* __restrict__ inv_mass,
                                                          const uint32_t* __restrict__ counts, int hid, int gid,
-                                                         float4* __restrict__ gc) {
+                                                         float dim, float4* __restrict__ gc) {
     const uint32_t n = counts[C_N];
     GRID_STRIDE(i, n) {
         const float k = kw[i];
         const float im = inv_mass[i];
-        const GaussConst g = gauss_const(k);
+        const GaussConst g = gauss_const(k, dim);
         float w0;
         v3f unused;
-        kernel_eval(hid, gid, mk(0, 0, 0), k, g, true, false, &w0, &unused);
+        kernel_eval(hid, gid, dim, mk(0, 0, 0), k, g, true, false, &w0, &unused);
         const uint32_t self = to_u32(w0 / im * INCOMP_RES);
-        gc[i] = make_float4(g.ivd, g.norm, pow_f(2.0f * radius[i], DIM) * im, __builtin_bit_cast(float, self));
+        gc[i] = make_float4(g.ivd, g.norm, pow_f(2.0f * radius[i], dim) * im, __builtin_bit_cast(float, self));
     }
 }
 
@@ -319,9 +319,9 @@ void launch_set_n_plus_1(uint32_t* counts, hipStream_t s) {
     hipLaunchKernelGGL(k_set_n_plus_1, dim3(1), dim3(64), 0, s, counts);
 }
 void launch_kernel_consts(const float* kw, const float* radius, const float* inv_mass, const uint32_t* counts,
-                          uint32_t cap, int hid, int gid, float4* gc,
+                          uint32_t cap, int hid, int gid, float dim, float4* gc,
                           hipStream_t s) {
-    LAUNCH(k_kernel_consts, stride_grid(cap), kw, radius, inv_mass, counts, hid, gid, gc);
+    LAUNCH(k_kernel_consts, stride_grid(cap), kw, radius, inv_mass, counts, hid, gid, dim, gc);
 }
 void launch_box(int4* pos, const float* radius, const uint32_t* counts, uint32_t cap,
                 const BoxSet& boxes, hipStream_t s, const SegmentTable* seg) {

@@ -84,9 +84,9 @@ __device__ __forceinline__ uint64_t group_mask() {
 // Kernel evaluation with the height / gradient kernel ids fixed at compile
 // time (KH, KG >= 0) or read at run time (-1).
 template <int KH, int KG>
-__device__ __forceinline__ void kernel_eval_k(int hid, int gid, v3f r, float kw, GaussConst g, bool need_w,
-                                              bool need_g, float* w, v3f* grad) {
-    kernel_eval(KH >= 0 ? KH : hid, KG >= 0 ? KG : gid, r, kw, g, need_w, need_g, w, grad);
+__device__ __forceinline__ void kernel_eval_k(int hid, int gid, float dim, v3f r, float kw, GaussConst g,
+                                              bool need_w, bool need_g, float* w, v3f* grad) {
+    kernel_eval(KH >= 0 ? KH : hid, KG >= 0 ? KG : gid, dim, r, kw, g, need_w, need_g, w, grad);
 }
 
 // Gradients of the first C entries of a lane's share of the row stay in
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                 const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
                 float w;
                 v3f grad;
-                kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+                kernel_eval_k<KH, KG>(a.hid, a.gid, a.dim, diff, kwi, g, true, true, &w, &grad);
                 const float gg = dot(grad, grad);
                 float wq, ggq;
                 v3f wg;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                         const int4 pj = pos[j];
                         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
                         float w;
-                        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, false, true, &w, &grad);
+                        kernel_eval_k<KH, KG>(a.hid, a.gid, a.dim, diff, kwi, g, false, true, &w, &grad);
                     }
                     pair3(j, grad);
                 }
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void k_features_rows(const int4* __restrict_
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
         v3f grad;
-        kernel_eval_k<KH, KG>(a.hid, a.gid, diff, kwi, g, true, true, &w, &grad);
+        kernel_eval_k<KH, KG>(a.hid, a.gid, a.dim, diff, kwi, g, true, true, &w, &grad);
         const float gg = dot(grad, grad);
         float wq, ggq;
         v3f wg;
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col, uint32_t pcap,
-                                                        int hid, int gid, double* __restrict__ part_sum,
+                                                        int hid, int gid, float dim, double* __restrict__ part_sum,
                                                         float* __restrict__ part_max) {
     __shared__ double ssum[BLOCK];
     __shared__ double sabs[BLOCK];
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_rows(const int4* __restrict__
     auto pair = [&](const int4 pj, const float imN) {
         const v3f diff = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z)) / POS_RES;
         float w;
-        kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
+        kernel_eval_k<KH, 0>(hid, gid, dim, diff, kwi, g, true, false, &w, &unused);
         const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
         const uint32_t ie = (ib >> 23) & 0xFFu;
         float wq;
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_density_rows(
     const int4* __restrict__ pos, const float* __restrict__ inv_mass, const float* __restrict__ kw,
     const float4* __restrict__ gc, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
     const uint32_t* __restrict__ col, uint32_t pcap, const uint32_t* __restrict__ old_bd, uint32_t* __restrict__ bd,
-    uint32_t* __restrict__ nearest, int hid, int gid, double* __restrict__ part_sum, float* __restrict__ part_max) {
+    uint32_t* __restrict__ nearest, int hid, int gid, float dim, double* __restrict__ part_sum, float* __restrict__ part_max) {
     __shared__ double ssum[BLOCK];
     __shared__ double sabs[BLOCK];
     __shared__ float smax[BLOCK];
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(BLOCK) void k_fsm12_density_rows(
         if (k >= rbd && k < red) {
             const v3f diff = vec_i(dx, dy, dz) / POS_RES;
             float w;
-            kernel_eval_k<KH, 0>(hid, gid, diff, kwi, g, true, false, &w, &unused);
+            kernel_eval_k<KH, 0>(hid, gid, dim, diff, kwi, g, true, false, &w, &unused);
             const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
             const uint32_t ie = (ib >> 23) & 0xFFu;
             float wq;
@@ -1055,26 +1055,26 @@ void launch_solve_stats(const uint32_t* counts, unsigned long long* stats, uint3
 uint32_t density_rows_blocks(uint32_t cap) { return row_grid(cap) < 2048u ? row_grid(cap) : 2048u; }
 void launch_density_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                          const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                         const uint32_t* col, uint32_t pcap, int hid, int gid, double* part_sum,
+                         const uint32_t* col, uint32_t pcap, int hid, int gid, float dim, double* part_sum,
                          float* part_max, hipStream_t s) {
     if (hid == 1)
         hipLaunchKernelGGL((k_density_rows<ROW_LANES, 1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, dim, part_sum, part_max);
     else
         hipLaunchKernelGGL((k_density_rows<ROW_LANES, -1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s, pos,
-                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, part_sum, part_max);
+                           inv_mass, radius, kw, gc, counts, row, col, pcap, hid, gid, dim, part_sum, part_max);
 }
 void launch_fsm12_density_rows(const int4* pos, const float* inv_mass, const float* kw, const float4* gc,
                                const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,
                                uint32_t pcap, const uint32_t* old_bd, uint32_t* bd, uint32_t* nearest, int hid,
-                               int gid, double* part_sum, float* part_max, hipStream_t s) {
+                               int gid, float dim, double* part_sum, float* part_max, hipStream_t s) {
     if (hid == 1)
         hipLaunchKernelGGL((k_fsm12_density_rows<ROW_LANES, 1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s,
-                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, part_sum,
+                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, dim, part_sum,
                            part_max);
     else
         hipLaunchKernelGGL((k_fsm12_density_rows<ROW_LANES, -1>), dim3(density_rows_blocks(cap)), dim3(BLOCK), 0, s,
-                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, part_sum,
+                           pos, inv_mass, kw, gc, counts, row, col, pcap, old_bd, bd, nearest, hid, gid, dim, part_sum,
                            part_max);
 }
 void launch_fsm12_rows(const int4* pos, const uint32_t* counts, uint32_t cap, const uint32_t* row,

@@ -36,6 +36,9 @@ struct Scene {
     uint32_t* seg_start = nullptr;
     uint32_t* sid[2] = {};
     bool packed() const { return n_seg > 1 || sid[0] != nullptr; }
+    // DIMENSIONS (cpu_gpu_shared_config.h:2): 2 or 3 (init maps 0 to 3)
+    uint32_t dims() const { return P.dimensions; }
+    float dim() const { return (float)P.dimensions; }
     SegmentTable seg_table() const {
         return packed() ? SegmentTable{d_grids, d_boxes, seg_start, sid[cur], seg_shift}
                         : SegmentTable{nullptr, nullptr, nullptr, nullptr, 0u};

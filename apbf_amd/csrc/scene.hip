@@ -57,13 +57,16 @@ ParticleArrays Scene::arrays(int set) {
 
 apbf_status Scene::init(const apbf_scene_params& p) {
     P = p;
-    if (p.cap_particles == 0 || p.hash_res_log2 == 0 || p.hash_res_log2 > 10 || p.n_boxes > APBF_MAX_BOXES)
+    if (P.dimensions == 0u) P.dimensions = 3u;
+    if (P.dimensions != 2u && P.dimensions != 3u) return APBF_ERR_INVALID_ARGUMENT;
+    if (p.cap_particles == 0 || !hash_res_ok(p.hash_res_log2, dims()) || p.n_boxes > APBF_MAX_BOXES)
         return APBF_ERR_INVALID_ARGUMENT;
     cap = p.cap_particles;
     tcap = std::max<uint32_t>(p.cap_transfers, 1);
     pcap = std::max<uint32_t>(p.cap_pairs, 1);
-    n_cells = 1u << (3u * p.hash_res_log2);
+    n_cells = 1u << (dims() * p.hash_res_log2);
     grid.res = p.hash_res_log2;
+    grid.dims = dims();
     for (int d = 0; d < 3; ++d) {
         grid.mn[d] = p.hash_min[d];
         grid.mx[d] = p.hash_max[d];
@@ -202,9 +205,10 @@ apbf_status Scene::init_segments(uint32_t n, const apbf_segment_desc* segs) {
     bool any_box = false;
     for (uint32_t k = 0; k < n; ++k) {
         const apbf_segment_desc& d = segs[k];
-        if (d.hash_res_log2 == 0 || d.hash_res_log2 > 10 || d.n_boxes > APBF_MAX_BOXES) return APBF_ERR_INVALID_ARGUMENT;
+        if (!hash_res_ok(d.hash_res_log2, dims()) || d.n_boxes > APBF_MAX_BOXES) return APBF_ERR_INVALID_ARGUMENT;
         max_res = std::max(max_res, d.hash_res_log2);
         g[k].res = d.hash_res_log2;
+        g[k].dims = dims();
         std::memset(&b[k], 0, sizeof(BoxSet));
         b[k].n = d.n_boxes;
         any_box = any_box || d.n_boxes;
@@ -220,9 +224,9 @@ apbf_status Scene::init_segments(uint32_t n, const apbf_segment_desc* segs) {
     }
     uint32_t bits = 0;
     while ((1u << bits) < n) ++bits;
-    if (3u * max_res + bits > 31u) return APBF_ERR_CAPACITY;  // scene index above the cell hash in a u32 key
+    if (dims() * max_res + bits > 31u) return APBF_ERR_CAPACITY;  // scene index above the cell hash in a u32 key
     n_seg = n;
-    seg_shift = 3u * max_res;
+    seg_shift = dims() * max_res;
     n_cells = n << seg_shift;
     boxes.n = any_box ? 1u : 0u;  // the per-scene boxes are in the table; this only says "run the colliders"
     hipFree(cell_start);
@@ -250,7 +254,7 @@ apbf_status Scene::init_segments(uint32_t n, const apbf_segment_desc* segs) {
 void Scene::neighborhood_green(float range_scale, bool spread) {
     hipStream_t s = stream;
     const SegmentTable seg0 = seg_table();
-    uint32_t upper = packed() ? (n_seg << seg_shift) : (1u << (grid.res * 3u));
+    uint32_t upper = packed() ? (n_seg << seg_shift) : (1u << (grid.res * dims()));
     launch_hash_iota(pos[cur], counts + C_N, cap, grid, keys, vals, s, &seg0);
     radix_sort_pairs(keys, vals, counts + C_N, cap, upper, skeys, perm, ws_mem, s, sort_shape);
     ParticleArrays src = arrays(cur), dst = arrays(1 - cur);
@@ -372,7 +376,8 @@ SolverArgs Scene::solver_args() const {
     a.b_speed = P.s.mBoundarinessAdaptionSpeed;
     a.b_sgl_factor = P.s.mBoundarinessSelfGradLengthFactor;
     a.b_under_factor = P.s.mBoundarinessUnderpressureFactor;
-    a.lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, DIM);
+    a.lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, dim());
+    a.dim = dim();
     a.it_err = nullptr;
     a.lam_override = nullptr;
     a.lam_acc = nullptr;
@@ -401,7 +406,7 @@ void Scene::prepare_solver() {
         long_rows = (uint64_t)seen_pairs > 62ull * (uint64_t)seen_n;
     }
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                         P.s.mGradientKernelId, gc, stream);
+                         P.s.mGradientKernelId, dim(), gc, stream);
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
 }
 
@@ -522,9 +527,9 @@ void Scene::accumulate_density_error() {  // after the solver loop of update()
     hipStream_t s = stream;
     if (P.solver_iterations <= 0)  // otherwise the solver loop computed gc from the same state
         launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                             P.s.mGradientKernelId, gc, s);
+                             P.s.mGradientKernelId, dim(), gc, s);
     launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
-                        P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+                        P.s.mHeightKernelId, P.s.mGradientKernelId, dim(), part_sum, part_max, s);
     launch_density_error_finish(part_sum, part_max, err_blocks, counts, err_stats, s);
 }
 
@@ -560,9 +565,9 @@ apbf_status Scene::density_error(double* mean_err, double* max_err) {
     if (!pairs_valid) return APBF_ERR_INVALID_ARGUMENT;
     hipStream_t s = stream;
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
-                         P.s.mGradientKernelId, gc, s);
+                         P.s.mGradientKernelId, dim(), gc, s);
     launch_density_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap,
-                        P.s.mHeightKernelId, P.s.mGradientKernelId, part_sum, part_max, s);
+                        P.s.mHeightKernelId, P.s.mGradientKernelId, dim(), part_sum, part_max, s);
     std::vector<double> hs(err_blocks);
     std::vector<float> hm(err_blocks);
     uint32_t n = 0;
@@ -815,10 +820,11 @@ apbf_status apbf_scene_set_range_scale(void* scene, float scale) {
 }
 
 apbf_status apbf_scene_set_position_range(void* scene, const float* min3, const float* max3, uint32_t res_log2) {
-    if (!scene || !min3 || !max3 || res_log2 == 0 || res_log2 > 10) return APBF_ERR_INVALID_ARGUMENT;
+    if (!scene || !min3 || !max3) return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
+    if (!apbf::hash_res_ok(res_log2, s->dims())) return APBF_ERR_INVALID_ARGUMENT;
     if (s->packed()) return APBF_ERR_INVALID_ARGUMENT;  // packed batches: the grids of their segments
-    const uint32_t cells = 1u << (3u * res_log2);
+    const uint32_t cells = 1u << (s->dims() * res_log2);
     if (cells != s->n_cells) {  // new cell tables (outside any substep)
         if (hipStreamSynchronize(s->stream) != hipSuccess) return APBF_ERR_HIP;
         uint32_t *a = nullptr, *b = nullptr;

@@ -52,6 +52,7 @@ struct GridDesc {
     float mn[3];
     float mx[3];
     uint32_t res;
+    uint32_t dims;  // DIMENSIONS (cpu_gpu_shared_config.h:2): 2 or 3
 };
 
 // The scene's colliders: up to APBF_MAX_BOXES boxes (box_collision) and one
@@ -83,6 +84,7 @@ struct SegmentTable {
 struct SolverArgs {
     int hid, gid, method, update_b;
     float b_speed, b_sgl_factor, b_under_factor, lambda_scale_base;
+    float dim;  // DIMENSIONS, 2 or 3
     double* it_err;  // when set: += mean density error of the positions this iteration starts from
     const float* lam_override;  // when set: shift with these lambdas, no boundariness update or filter
     float* lam_acc;             // when set: += min(lambda, 0) per particle
@@ -102,10 +104,25 @@ APBF_HD uint32_t spread_bits3(uint32_t v) {  // bit i -> bit 3i, for the low 10 
     v = (v | (v << 2)) & 0x09249249u;
     return v;
 }
-// z-curve of calculate_position_hash.comp:29-36: bit i of axis d -> bit 3i+d, i < res.
-APBF_HD uint32_t zhash(uint32_t gx, uint32_t gy, uint32_t gz, uint32_t res) {
-    uint32_t m = (res >= 32u) ? 0xFFFFFFFFu : ((1u << res) - 1u);
-    return spread_bits3(gx & m) | (spread_bits3(gy & m) << 1) | (spread_bits3(gz & m) << 2);
+APBF_HD uint32_t spread_bits2(uint32_t v) {  // bit i -> bit 2i, for the low 16 bits
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+// The z-curve of calculate_position_hash.comp:29-36 (bit i of axis d -> bit
+// D i + d for i < res, d < D), split by axis: the bits of cell coordinate c
+// on axis d. The hash of a cell is the OR over its axes.
+APBF_HD uint32_t zbits(uint32_t c, uint32_t d, uint32_t res, uint32_t dims) {
+    const uint32_t m = (res >= 32u) ? 0xFFFFFFFFu : ((1u << res) - 1u);
+    return dims == 2u ? spread_bits2(c & m) << d : spread_bits3(c & m) << d;
+}
+// The z-curve hash of a cell; in 2D the z coordinate is not part of it.
+APBF_HD uint32_t zhash(uint32_t gx, uint32_t gy, uint32_t gz, uint32_t res, uint32_t dims) {
+    const uint32_t h = zbits(gx, 0u, res, dims) | zbits(gy, 1u, res, dims);
+    return dims == 2u ? h : h | zbits(gz, 2u, res, dims);
 }
 APBF_HD v3f pos_world(int4 p) { return vec_i(p.x, p.y, p.z) / POS_RES; }
 

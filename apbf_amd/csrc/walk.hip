@@ -103,6 +103,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         hi[0] = grid_cell(p.x + range, g.mn[0], g.mx[0], g.res);
         hi[1] = grid_cell(p.y + range, g.mn[1], g.mx[1], g.res);
         hi[2] = grid_cell(p.z + range, g.mn[2], g.mx[2], g.res);
+        // neighborhood_green.comp:36: in 2D the walk stays in z cell 0
+        if (g.dims == 2u) lo[2] = hi[2] = 0u;
         // Bound the walk to 4 * 2^res cells per axis: wider walks only revisit
         // aliased cells, and an unbounded span (infinite range) would not end.
         const uint32_t span = 4u << g.res;
@@ -192,20 +194,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
         // in its registers. A candidate's position and (spreading) stored
         // width are loaded together.
         constexpr int CW = L < 4 ? L : 4;
-        const uint32_t hmask = (g.res >= 32u) ? 0xFFFFFFFFu : ((1u << g.res) - 1u);
         const uint32_t nline = hi[0] >= lo[0] ? hi[0] - lo[0] + 1u : 0u;
         for (uint32_t z = lo[2]; more && z <= hi[2]; ++z) {
             const float dz2 = axis_d2(2, z, p.z);
             for (uint32_t y = lo[1]; more && y <= hi[1] && dz2 <= thr; ++y) {
                 const float dyz2 = dz2 + axis_d2(1, y, p.y);
-                const uint32_t hyz = (spread_bits3(y & hmask) << 1) | (spread_bits3(z & hmask) << 2);
+                const uint32_t hyz = zbits(y, 1u, g.res, g.dims) | (g.dims == 2u ? 0u : zbits(z, 2u, g.res, g.dims));
                 for (uint32_t c0 = 0; more && c0 < nline && dyz2 <= thr; c0 += CW) {
                     const uint32_t nc = min(nline - c0, (uint32_t)CW);
                     uint32_t cstart = 0, clen = 0;
                     if ((uint32_t)gl < nc) {
                         const uint32_t x = lo[0] + c0 + (uint32_t)gl;
                         if (!(dyz2 + axis_d2(0, x, p.x) > thr)) {
-                            const uint32_t h = cbase + (spread_bits3(x & hmask) | hyz);
+                            const uint32_t h = cbase + (zbits(x, 0u, g.res, g.dims) | hyz);
                             cstart = cs[h];
                             const uint32_t cend = ce[h];
                             clen = cend > cstart ? cend - cstart : 0u;

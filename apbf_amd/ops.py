@@ -80,13 +80,13 @@ def radix_sort(keys: torch.Tensor, values: torch.Tensor, upper_bound: int = 0xFF
     return ok, ov
 
 
-def position_hash(position: torch.Tensor, mn, mx, res: int) -> torch.Tensor:
+def position_hash(position: torch.Tensor, mn, mx, res: int, dims: int = 3) -> torch.Tensor:
     L = _lib.lib()
     n = position.shape[0]
     out = torch.empty(n, dtype=torch.int32, device=position.device)
     fmn = (C.c_float * 3)(*mn)
     fmx = (C.c_float * 3)(*mx)
-    check(L.apbf_calculate_position_hash(_p(position), _p(_len(n, position.device)), n, fmn, fmx, res,
+    check(L.apbf_calculate_position_hash(_p(position), _p(_len(n, position.device)), n, fmn, fmx, res, dims,
                                          _p(out), _stream()), "apbf_calculate_position_hash")
     _sync()
     return out
@@ -113,7 +113,7 @@ def find_value_ranges(sorted_keys: torch.Tensor, n_cells: int):
     return cs, ce
 
 
-def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx, res, pair_cap):
+def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx, res, pair_cap, dims=3):
     """CSR neighbour lists (row offsets, neighbour ids, pair count)."""
     L = _lib.lib()
     n = position.shape[0]
@@ -125,7 +125,7 @@ def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx,
     fmn = (C.c_float * 3)(*mn)
     fmx = (C.c_float * 3)(*mx)
     check(L.apbf_neighborhood_green(_p(position), _p(rng), _p(_len(n, dev)), n, float(range_scale), fmn, fmx,
-                                    res, _p(cell_start), _p(cell_end), _p(row), _p(col), pair_cap, _p(npairs),
+                                    res, dims, _p(cell_start), _p(cell_end), _p(row), _p(col), pair_cap, _p(npairs),
                                     _p(ws), ws.numel(), _stream()), "apbf_neighborhood_green")
     _sync()
     return row, col, npairs

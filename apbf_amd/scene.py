@@ -276,9 +276,10 @@ class Scene:
 
 def _group_key(p: dict):
     """Scenes that can share a packed list: the same settings, flags, solver,
-    gravity and time step (grids and boxes may differ)."""
+    gravity, time step and dimension count (grids and boxes may differ)."""
     return (tuple(sorted(p["settings"].items())), int(p["basic_pbf"]), int(p["solver_iterations"]),
-            float(p["split_duration"]), tuple(float(g) for g in p["gravity"]), float(p.get("dt", 0.0)))
+            float(p["split_duration"]), tuple(float(g) for g in p["gravity"]), float(p.get("dt", 0.0)),
+            int(p.get("dimensions", 3)))
 
 
 def _split_states(st: dict, ids: np.ndarray, n_scenes: int) -> list:

@@ -81,6 +81,11 @@ typedef struct apbf_scene_params {
     uint32_t cap_particles;
     uint32_t cap_transfers;
     uint32_t cap_pairs;
+    /* DIMENSIONS (cpu_gpu_shared_config.h:2): 3, or 2 for the reference's
+     * default planar build (one particle layer in z, 2D z-curve hash,
+     * 2D kernel normalisations, rest density and split/merge volumes);
+     * 0 means 3. */
+    uint32_t dimensions;
 } apbf_scene_params;
 
 /* ------------------------------------------------------------------ scene */
@@ -260,11 +265,12 @@ apbf_status apbf_radix_sort_noalloc(const uint32_t* keys, const uint32_t* values
 
 /* z-order hash of each position in the grid [min, max] with 2^res cells per
  * axis (shader_provider::calculate_position_hash, shader_provider.cpp:805;
- * calculate_position_hash.comp:23-46). */
+ * calculate_position_hash.comp:23-46) over the first `dimensions` (2 or 3,
+ * the reference's compile-time DIMENSIONS) axes; res * dimensions <= 30. */
 apbf_status apbf_calculate_position_hash(const int32_t* position, const uint32_t* d_len,
                                          uint32_t capacity, const float* host_min3,
                                          const float* host_max3, uint32_t res_log2,
-                                         uint32_t* out_hash, apbf_stream stream);
+                                         uint32_t dimensions, uint32_t* out_hash, apbf_stream stream);
 
 /* One 32-bit section (0..2) of the 96-bit Morton code of the raw positions
  * (shader_provider::calculate_position_code, shader_provider.cpp:782;
@@ -286,12 +292,14 @@ apbf_status apbf_find_value_ranges(const uint32_t* sorted_keys, const uint32_t* 
  * row_offset[i+1] holds the neighbours of particle i in the reference's cell-
  * walk order; pairs beyond pair_capacity are dropped (the reference's clamp,
  * neighbor_add.glsl:22-23) and *d_n_pairs receives the clamped total.
- * `workspace` >= apbf_neighborhood_green_workspace_size(capacity). */
+ * With dimensions 2 the cells are those of the x-y grid (the walk stays in
+ * z cell 0, neighborhood_green.comp:36); the distance test is 3D as in the
+ * reference. `workspace` >= apbf_neighborhood_green_workspace_size(capacity). */
 size_t apbf_neighborhood_green_workspace_size(uint32_t capacity);
 apbf_status apbf_neighborhood_green(const int32_t* position, const float* range,
                                     const uint32_t* d_len, uint32_t capacity, float range_scale,
                                     const float* host_min3, const float* host_max3,
-                                    uint32_t res_log2, const uint32_t* cell_start,
+                                    uint32_t res_log2, uint32_t dimensions, const uint32_t* cell_start,
                                     const uint32_t* cell_end, uint32_t* row_offset,
                                     uint32_t* neighbors, uint32_t pair_capacity,
                                     uint32_t* d_n_pairs, void* workspace, size_t workspace_bytes,

@@ -37,7 +37,6 @@ constexpr float KW_RES = 262144.0f;
 constexpr float INCOMP_RES = 262144.0f;
 constexpr float KERNEL_SCALE = 4.0f;
 constexpr float KW_PROPAGATION = 0.5f;
-constexpr float DIM = 3.0f;
 constexpr float PI_F = 3.14159265f;  // kernels.glsl:1
 constexpr float INCOMP_EPSILON = 0.01f;  // incompressibility_2.comp:6
 // initialize_split_particles.comp:26 writes the GLSL integer expression 1/0,
@@ -104,24 +103,28 @@ V3 cubic_kernel_gradient(V3 r, float h) {  // kernels.glsl:33-46
     float factor = 1.0f - q;
     return muls(gradq, l * (-factor * factor));
 }
-float cone_kernel_height(V3 r, float h) {  // kernels.glsl:48-52
-    float height = 3.0f / (PI_F * pow_f(h, DIM));
+// D is DIMENSIONS (cpu_gpu_shared_config.h:2), 2 or 3, as a float.
+float cone_kernel_height(V3 r, float h, float D) {  // kernels.glsl:48-52
+    float height = 3.0f / (PI_F * pow_f(h, D));
     return gmax(0.0f, (1.0f - length(r) / h) * height);
 }
-V3 cone_kernel_gradient(V3 r, float h) {  // kernels.glsl:54-60
-    float steep = 3.0f / (PI_F * pow_f(h, DIM + 1.0f));
+V3 cone_kernel_gradient(V3 r, float h, float D) {  // kernels.glsl:54-60
+    float steep = 3.0f / (PI_F * pow_f(h, D + 1.0f));
     float dist = length(r);
     if (dist > h || dist < 0.0001f) return v3(0, 0, 0);
     return muls(neg(r), steep / dist);
 }
-float quadratic_spike_kernel_height(V3 r, float h) {  // kernels.glsl:62-70 (3D branch)
-    float a = 15.0f / (2.0f * PI_F * pow_f(h, 5.0f));
+float quadratic_spike_a(float h, float D) {  // kernels.glsl:64-68, 76-80
+    return D == 3.0f ? 15.0f / (2.0f * PI_F * pow_f(h, 5.0f)) : 6.0f / (PI_F * pow_f(h, 4.0f));
+}
+float quadratic_spike_kernel_height(V3 r, float h, float D) {  // kernels.glsl:62-70
+    float a = quadratic_spike_a(h, D);
     return a * pow_f(gmin(0.0f, length(r) - h), 2.0f);
 }
-V3 quadratic_spike_kernel_gradient(V3 r, float h) {  // kernels.glsl:72-82 (3D branch)
+V3 quadratic_spike_kernel_gradient(V3 r, float h, float D) {  // kernels.glsl:72-82
     float dist = length(r);
     if (dist > h || dist < 0.0001f) return v3(0, 0, 0);
-    float a = 15.0f / (2.0f * PI_F * pow_f(h, 5.0f));
+    float a = quadratic_spike_a(h, D);
     return muls(r, -2.0f * a * gmax(0.0f, h - dist) / dist);
 }
 
@@ -132,10 +135,10 @@ struct Gauss {
     float ivd;   // invDoubleVariance = pow(height, 2/D) * PI
     float norm;  // pow(invDoubleVarianceWithoutPi, D/2)
 };
-inline Gauss gauss_of_width(float kw) {
-    float height = 0.6f / pow_f(kw / 2.0f, DIM);       // kernels.glsl:104,117
-    float ivdwp = pow_f(height, 2.0f / DIM);           // kernels.glsl:86
-    return Gauss{ivdwp * PI_F, pow_f(ivdwp, DIM / 2.0f)};
+inline Gauss gauss_of_width(float kw, float D) {
+    float height = 0.6f / pow_f(kw / 2.0f, D);       // kernels.glsl:104,117
+    float ivdwp = pow_f(height, 2.0f / D);           // kernels.glsl:86
+    return Gauss{ivdwp * PI_F, pow_f(ivdwp, D / 2.0f)};
 }
 inline float gauss_height(V3 r, const Gauss& g) {  // kernels.glsl:84-89
     return exp_f(-dot(r, r) * g.ivd) * g.norm;
@@ -149,6 +152,7 @@ inline V3 gauss_gradient(V3 r, const Gauss& g, float height_value) {  // kernels
 
 struct KernelEval {
     int height_id, gradient_id;
+    float D;
     // returns W(r) and grad W(r) for kernel width kw with precomputed Gauss constants g
     void eval(V3 r, float kw, const Gauss& g, float* w, V3* grad, bool need_w, bool need_g) const {
         float gh = 0.0f;
@@ -158,8 +162,8 @@ struct KernelEval {
                 case 0: *w = cubic_kernel_height(r, kw); break;
                 case 1: gh = gauss_height(r, g); gh_done = true; *w = gh; break;
                 case 2: *w = poly6_kernel_height(r, kw); break;
-                case 3: *w = cone_kernel_height(r, kw); break;
-                case 4: *w = quadratic_spike_kernel_height(r, kw); break;
+                case 3: *w = cone_kernel_height(r, kw, D); break;
+                case 4: *w = quadratic_spike_kernel_height(r, kw, D); break;
                 default: *w = 0.0f;
             }
         }
@@ -172,8 +176,8 @@ struct KernelEval {
                     break;
                 }
                 case 2: *grad = spiky_kernel_gradient(r, kw); break;
-                case 3: *grad = cone_kernel_gradient(r, kw); break;
-                case 4: *grad = quadratic_spike_kernel_gradient(r, kw); break;
+                case 3: *grad = cone_kernel_gradient(r, kw, D); break;
+                case 4: *grad = quadratic_spike_kernel_gradient(r, kw, D); break;
                 default: *grad = v3(0, 0, 0);
             }
         }
@@ -203,22 +207,26 @@ void radix_sort(const uint32_t* keys, const uint32_t* vals, uint32_t n, uint32_t
     }
 }
 
-// calculate_position_hash.comp:23-36 / neighborhood_green.comp:35-47
+// calculate_position_hash.comp:23-36 / neighborhood_green.comp:35-47 for
+// `dims` = DIMENSIONS. neighborhood_green.comp:36 zeroes the cell coordinates
+// of the axes past DIMENSIONS (so in 2D the walk stays in z cell 0); the
+// position hash never reads them, so zeroing them here changes no hash.
 struct Grid {
     V3 mn, mx;
     uint32_t res;
+    uint32_t dims;
     uint32_t cell_of(float p, float lo, float hi) const {
         return to_u32((p - lo) / (hi - lo) * (float)(1u << res));
     }
     void map(V3 p, uint32_t g[3]) const {
         g[0] = cell_of(p.x, mn.x, mx.x);
-        g[1] = cell_of(p.y, mn.y, mx.y);
-        g[2] = cell_of(p.z, mn.z, mx.z);
+        g[1] = dims > 1u ? cell_of(p.y, mn.y, mx.y) : 0u;
+        g[2] = dims > 2u ? cell_of(p.z, mn.z, mx.z) : 0u;
     }
     uint32_t hash(const uint32_t g[3]) const {
         uint32_t result = 0u;
         for (uint32_t i = 0; i < res; ++i)
-            for (uint32_t d = 0; d < 3u; ++d) result += ((g[d] >> i) & 1u) << (i * 3u + d);
+            for (uint32_t d = 0; d < dims; ++d) result += ((g[d] >> i) & 1u) << (i * dims + d);
         return result;
     }
 };
@@ -312,7 +320,10 @@ struct Scene {
     std::vector<uint32_t> fsm_bd_fill, fsm_min_dist, fsm_nearest;
     float last_dt = 1.0f;  // velocity_handling.h:17
 
-    KernelEval kern() const { return KernelEval{P.s.mHeightKernelId, P.s.mGradientKernelId}; }
+    KernelEval kern() const { return KernelEval{P.s.mHeightKernelId, P.s.mGradientKernelId, dim()}; }
+    // DIMENSIONS (cpu_gpu_shared_config.h:2): 2 or 3; 0 in the params means 3.
+    uint32_t dims() const { return P.dimensions == 2u ? 2u : 3u; }
+    float dim() const { return (float)dims(); }
     uint32_t H() const { return (uint32_t)position.size(); }
     uint32_t F() const { return (uint32_t)fidx.size(); }
 
@@ -401,6 +412,7 @@ struct Scene {
             uint32_t idS = t_source[id], idT = t_target[id];
             float rS = radius[idS], rT = radius[idT];
             float imS = inverse_mass[idS], imT = inverse_mass[idT];
+            const float DIM = MUT(14) ? 3.0f : dim();  // mutation 14: 3D volumes in every build
             float volS = pow_f(rS, DIM), volT = pow_f(rT, DIM);
             float factorS = merge ? gmin(1.0f, dt / ttl) : dt * (volS - volT) / (2.0f * ttl * volS);
             float transfVol = factorS * volS;
@@ -450,11 +462,11 @@ struct Scene {
     // --------------------------------------------- neighborhood_green::apply
     Grid grid() const {
         return Grid{v3(P.hash_min[0], P.hash_min[1], P.hash_min[2]),
-                    v3(P.hash_max[0], P.hash_max[1], P.hash_max[2]), P.hash_res_log2};
+                    v3(P.hash_max[0], P.hash_max[1], P.hash_max[2]), P.hash_res_log2, dims()};
     }
     void neighborhood(float range_scale) {  // neighborhood_green.cpp:27-77
         Grid g = grid();
-        uint32_t maxHash = 1u << (g.res * 3u);
+        uint32_t maxHash = 1u << (g.res * g.dims);
         uint32_t nh = H();
         std::vector<uint32_t> iota(nh), hashes(nh);
         for (uint32_t i = 0; i < nh; ++i) iota[i] = i;
@@ -787,7 +799,7 @@ struct Scene {
         inc.assign(nf, Incomp{{0, 0, 0}, 0u, 0u});
         com.assign(nf, Com{0, 0, 0, 0});
         std::vector<Gauss> gk(nf);
-        for (uint32_t id = 0; id < nf; ++id) gk[id] = gauss_of_width(kernel_width[id]);
+        for (uint32_t id = 0; id < nf; ++id) gk[id] = gauss_of_width(kernel_width[id], dim());
         for (uint32_t id = 0; id < nf; ++id) {  // incompressibility_0.comp:33-45
             uint32_t idx = fidx[id];
             float w;
@@ -843,7 +855,7 @@ struct Scene {
         double sum = 0.0;
         for (uint32_t id = 0; id < F(); ++id) {
             uint32_t idx = fidx[id];
-            float ird = pow_f(2.0f * radius[idx], DIM) * inverse_mass[idx];
+            float ird = pow_f(2.0f * radius[idx], dim()) * inverse_mass[idx];
             float density = (float)inc[id].density / INCOMP_RES;
             sum += (double)gmax(0.0f, density * ird - 1.0f);
         }
@@ -867,13 +879,13 @@ struct Scene {
         if (action == RESTORE) position = saved_position;
         uint32_t nf = F();
         lambda.assign(nf, 0.0f);
-        float lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, DIM);
+        float lambda_scale_base = pow_f(2.0f * P.s.mSmallestTargetRadius, dim());
         for (uint32_t id = 0; id < nf; ++id) {  // incompressibility_2.comp:72-110
             uint32_t idx = fidx[id];
             float r = radius[idx];
             float im = inverse_mass[idx];
             float kw = kernel_width[id];
-            float ird = pow_f(2.0f * r, DIM) * im;
+            float ird = pow_f(2.0f * r, MUT(15) ? 3.0f : dim()) * im;  // mutation 15: the 3D rest density
             float density = (float)inc[id].density / INCOMP_RES;
             V3 wgs = divs(vec_of(inc[id].g[0], inc[id].g[1], inc[id].g[2]), INCOMP_RES);
             float sgs = (float)inc[id].sq / INCOMP_RES;
@@ -947,7 +959,7 @@ struct Scene {
         double sum = 0.0, mx = 0.0;
         for (uint32_t id = 0; id < F(); ++id) {
             uint32_t idx = fidx[id];
-            float ird = pow_f(2.0f * radius[idx], DIM) * inverse_mass[idx];
+            float ird = pow_f(2.0f * radius[idx], dim()) * inverse_mass[idx];
             float density = (float)inc[id].density / INCOMP_RES;
             float e = gmax(0.0f, density * ird - 1.0f);
             sum += (double)e;
@@ -993,7 +1005,7 @@ struct Scene {
         uint32_t maxT = P.cap_transfers, maxS = P.cap_particles;
         split_list.clear();
         uint32_t slen = 0;
-        float split_factor = pow_f(2.0f, 1.0f / DIM) * 0.99f;
+        float split_factor = pow_f(2.0f, 1.0f / (MUT(13) ? 3.0f : dim())) * 0.99f;  // mutation 13: the 3D factor
         for (uint32_t id = 0; id < nf; ++id) {  // find_split_and_merge_3.comp:42-117
             uint32_t idx = fidx[id];
             const I4& pos = position[idx];
@@ -1026,7 +1038,7 @@ struct Scene {
             boundariness[id] = b;
             bool split = P.s.mSplit && (tr * split_factor <= r);
             bool merge = P.s.mMerge && (MUT(6) || nnDist < r) &&  // mutation 6: no distance condition
-                         pow_f(r, DIM) + pow_f(nnR, DIM) <= pow_f(tr, DIM);
+                         pow_f(r, dim()) + pow_f(nnR, dim()) <= pow_f(tr, dim());
             if (!merge && !split) continue;
             uint32_t s = (merge && nnLarger) ? nnIdx : idx;
             uint32_t t = (merge && nnLarger) ? idx : nnIdx;
@@ -1347,8 +1359,8 @@ void orc_prefix_sum(uint32_t* values, uint32_t n) {  // prefix_sum_apply_on_bloc
 }
 
 void orc_position_hash(const int32_t* position, uint32_t n, const float* mn, const float* mx,
-                       uint32_t res, uint32_t* out) {
-    Grid g{v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), res};
+                       uint32_t res, uint32_t dims, uint32_t* out) {
+    Grid g{v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), res, dims};
     const I4* p = reinterpret_cast<const I4*>(position);
     for (uint32_t id = 0; id < n; ++id) {
         uint32_t c[3];
@@ -1372,8 +1384,8 @@ void orc_find_value_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t n_c
 
 uint32_t orc_neighborhood_green(const int32_t* position, const float* range, uint32_t n,
                                 const uint32_t* cs, const uint32_t* ce, float range_scale,
-                                const float* mn, const float* mx, uint32_t res, uint32_t* pairs,
-                                uint32_t cap) {
+                                const float* mn, const float* mx, uint32_t res, uint32_t dims,
+                                uint32_t* pairs, uint32_t cap) {
     Scene s;
     std::memset(&s.P, 0, sizeof(s.P));
     s.P.cap_pairs = cap;
@@ -1382,7 +1394,7 @@ uint32_t orc_neighborhood_green(const int32_t* position, const float* range, uin
     s.kernel_width.assign(range, range + n);
     s.fidx.resize(n);
     for (uint32_t i = 0; i < n; ++i) s.fidx[i] = i;
-    Grid g{v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), res};
+    Grid g{v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), res, dims};
     s.green(g, cs, ce, range_scale);
     std::memcpy(pairs, s.pairs.data(), 8 * s.pairs.size());
     return (uint32_t)s.pairs.size();
@@ -1426,9 +1438,10 @@ uint32_t orc_delete_edit_list(const uint32_t* deleted, uint32_t n_deleted, uint3
 float orc_exp(float x) { return exp_f(x); }
 float orc_pow(float x, float y) { return pow_f(x, y); }
 
-void orc_kernel(int kernel_id, const float* r3, float kw, float* height, float* grad3) {
-    KernelEval ke{kernel_id, kernel_id};
-    Gauss g = gauss_of_width(kw);
+void orc_kernel(int kernel_id, const float* r3, float kw, uint32_t dims, float* height, float* grad3) {
+    const float D = dims == 2u ? 2.0f : 3.0f;
+    KernelEval ke{kernel_id, kernel_id, D};
+    Gauss g = gauss_of_width(kw, D);
     V3 gr;
     ke.eval(v3(r3[0], r3[1], r3[2]), kw, g, height, &gr, true, true);
     grad3[0] = gr.x;

@@ -72,6 +72,7 @@ typedef struct orc_scene_params {
     uint32_t cap_particles;        /* hidden particle / fluid requested length */
     uint32_t cap_transfers;        /* MAX_TRANSFERS */
     uint32_t cap_pairs;            /* neighbour list requested length */
+    uint32_t dimensions;           /* DIMENSIONS (cpu_gpu_shared_config.h:2): 2 or 3; 0 means 3 */
 } orc_scene_params;
 
 void* orc_create(const orc_scene_params* p);
@@ -141,7 +142,7 @@ void orc_stable_radix_sort(const uint32_t* keys, const uint32_t* values, uint32_
                            uint32_t upper_bound, uint32_t* out_keys, uint32_t* out_values);
 void orc_prefix_sum(uint32_t* values, uint32_t n);
 void orc_position_hash(const int32_t* position, uint32_t n, const float* min3, const float* max3,
-                       uint32_t res_log2, uint32_t* out_hash);
+                       uint32_t res_log2, uint32_t dimensions, uint32_t* out_hash);
 void orc_position_code(const int32_t* position, uint32_t n, uint32_t section, uint32_t* out_code);
 void orc_find_value_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t n_cells,
                            uint32_t* start, uint32_t* end);
@@ -150,7 +151,7 @@ void orc_find_value_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t n_c
 uint32_t orc_neighborhood_green(const int32_t* position, const float* range, uint32_t n,
                                 const uint32_t* cell_start, const uint32_t* cell_end,
                                 float range_scale, const float* min3, const float* max3,
-                                uint32_t res_log2, uint32_t* pairs, uint32_t cap);
+                                uint32_t res_log2, uint32_t dimensions, uint32_t* pairs, uint32_t cap);
 uint32_t orc_neighborhood_brute_force(const int32_t* position, const float* range, uint32_t n,
                                       float range_scale, uint32_t* pairs, uint32_t cap);
 /* Index-list rebuild after a hidden-list edit (indexed_list.h:289-308); returns
@@ -164,7 +165,8 @@ uint32_t orc_delete_edit_list(const uint32_t* deleted, uint32_t n_deleted, uint3
 /* Scalar math helpers (pinned against libm in the tests). */
 float orc_exp(float x);
 float orc_pow(float x, float y);
-void orc_kernel(int kernel_id, const float* r3, float kernel_width, float* height, float* grad3);
+void orc_kernel(int kernel_id, const float* r3, float kernel_width, uint32_t dimensions, float* height,
+                float* grad3);
 
 #ifdef __cplusplus
 }

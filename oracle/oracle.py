@@ -76,6 +76,7 @@ class OrcSceneParams(C.Structure):
         ("cap_particles", C.c_uint32),
         ("cap_transfers", C.c_uint32),
         ("cap_pairs", C.c_uint32),
+        ("dimensions", C.c_uint32),
     ]
 
 
@@ -108,11 +109,11 @@ def _declare(L):
     L.orc_density_error.argtypes = [vp, _p(f64), _p(f64)]
     L.orc_stable_radix_sort.argtypes = [pu, pu, u32, u32, pu, pu]
     L.orc_prefix_sum.argtypes = [pu, u32]
-    L.orc_position_hash.argtypes = [pi, u32, pf, pf, u32, pu]
+    L.orc_position_hash.argtypes = [pi, u32, pf, pf, u32, u32, pu]
     L.orc_position_code.argtypes = [pi, u32, u32, pu]
     L.orc_find_value_ranges.argtypes = [pu, u32, u32, pu, pu]
     L.orc_neighborhood_green.restype = u32
-    L.orc_neighborhood_green.argtypes = [pi, pf, u32, pu, pu, f32, pf, pf, u32, pu, u32]
+    L.orc_neighborhood_green.argtypes = [pi, pf, u32, pu, pu, f32, pf, pf, u32, u32, pu, u32]
     L.orc_neighborhood_brute_force.restype = u32
     L.orc_neighborhood_brute_force.argtypes = [pi, pf, u32, f32, pu, u32]
     L.orc_apply_hidden_edit.restype = u32
@@ -123,7 +124,7 @@ def _declare(L):
     L.orc_exp.argtypes = [f32]
     L.orc_pow.restype = f32
     L.orc_pow.argtypes = [f32, f32]
-    L.orc_kernel.argtypes = [C.c_int, pf, f32, pf, pf]
+    L.orc_kernel.argtypes = [C.c_int, pf, f32, u32, pf, pf]
 
 
 def _ptr(a: np.ndarray, t):
@@ -169,6 +170,7 @@ def make_params(p: dict) -> OrcSceneParams:
     out.cap_particles = int(p["cap_particles"])
     out.cap_transfers = int(p["cap_transfers"])
     out.cap_pairs = int(p["cap_pairs"])
+    out.dimensions = int(p.get("dimensions", 3))
     return out
 
 
@@ -325,11 +327,11 @@ def prefix_sum(values):
     return v
 
 
-def position_hash(position, mn, mx, res):
+def position_hash(position, mn, mx, res, dims=3):
     p = _i32(position).reshape(-1, 4)
     out = np.zeros(len(p), np.uint32)
     lib().orc_position_hash(_ptr(p, C.c_int32), len(p), _ptr(_f32(mn), C.c_float),
-                            _ptr(_f32(mx), C.c_float), res, _ptr(out, C.c_uint32))
+                            _ptr(_f32(mx), C.c_float), res, dims, _ptr(out, C.c_uint32))
     return out
 
 
@@ -349,14 +351,14 @@ def find_value_ranges(sorted_keys, n_cells):
     return s, e
 
 
-def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx, res, cap):
+def neighborhood_green(position, rng, cell_start, cell_end, range_scale, mn, mx, res, cap, dims=3):
     p = _i32(position).reshape(-1, 4)
     r = _f32(rng)
     cs, ce = _u32(cell_start), _u32(cell_end)
     out = np.zeros((cap, 2), np.uint32)
     n = lib().orc_neighborhood_green(_ptr(p, C.c_int32), _ptr(r, C.c_float), len(p),
                                      _ptr(cs, C.c_uint32), _ptr(ce, C.c_uint32), range_scale,
-                                     _ptr(_f32(mn), C.c_float), _ptr(_f32(mx), C.c_float), res,
+                                     _ptr(_f32(mn), C.c_float), _ptr(_f32(mx), C.c_float), res, dims,
                                      _ptr(out, C.c_uint32), cap)
     return out[:n]
 
@@ -394,9 +396,9 @@ def pow(x: float, y: float) -> float:  # noqa: A001 - mirrors the GLSL name
     return lib().orc_pow(x, y)
 
 
-def kernel(kernel_id: int, r3, kernel_width: float):
+def kernel(kernel_id: int, r3, kernel_width: float, dims: int = 3):
     r = _f32(r3)
     h = C.c_float()
     g = np.zeros(3, np.float32)
-    lib().orc_kernel(kernel_id, _ptr(r, C.c_float), kernel_width, C.byref(h), _ptr(g, C.c_float))
+    lib().orc_kernel(kernel_id, _ptr(r, C.c_float), kernel_width, dims, C.byref(h), _ptr(g, C.c_float))
     return h.value, g

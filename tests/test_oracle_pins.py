@@ -32,15 +32,20 @@ def _dist_fixed(pos, a, b):
     return np.floor(np.sqrt(sq.astype(F))).astype(np.uint64), np.sqrt(sq.astype(F))
 
 
-def _case(bd_mode: bool, offset: float = 3.0, seed: int = 11, library=None, substeps: int = 0):
+# (boundary-distance mode, DIMENSIONS) of the split/merge cases
+CASES = [(False, 3), (True, 3), (False, 2), (True, 2)]
+
+
+def _case(bd_mode: bool, offset: float = 3.0, seed: int = 11, library=None, substeps: int = 0, dims: int = 3):
     """A small water drop whose drop is squeezed to 0.45 of its size (nearest
     neighbours closer than the radius: many merges competing for the same
     particles) and whose every third pool particle has radius 1.4 (splits
-    wherever the target radius is below 1.4 / 1.247), run stage by stage up to
-    update_transfers after `substeps` full substeps. Returns the parameters,
-    the scene, the states before and after update_transfers and the
-    intermediates of the find."""
-    p, st = cfg.small_water_drop(seed=seed)
+    wherever the target radius is below 1.4 / 1.247; in 2D 1.6, below
+    1.6 / 1.400), run
+    stage by stage up to update_transfers after `substeps` full substeps, in
+    3D or in the planar build. Returns the parameters, the scene, the states
+    before and after update_transfers and the intermediates of the find."""
+    p, st = cfg.small_water_drop(seed=seed, dims=dims)
     n = len(st["radius"])
     p["cap_transfers"] = 8 * n  # no claim is dropped by a capacity clamp
     p["cap_particles"] = 8 * n
@@ -59,8 +64,8 @@ def _case(bd_mode: bool, offset: float = 3.0, seed: int = 11, library=None, subs
     st["pos_backup"] = st["position"].copy()
     big = (~fine) & (np.arange(n) % 3 == 0)
     st["radius"] = st["radius"].copy()
-    st["radius"][big] = 1.4
-    st["inverse_mass"] = (np.float32(1.0) / (np.float32(2.0) * st["radius"]) ** 3).astype(np.float32)
+    st["radius"][big] = 1.4 if dims == 3 else 1.6
+    st["inverse_mass"] = (np.float32(1.0) / (np.float32(2.0) * st["radius"]) ** dims).astype(np.float32)
     o = OracleScene(p, library=library)
     o.set_state(st)
     dt = p["dt"]
@@ -86,8 +91,8 @@ def pin_flood_fill(library=None):
     """find_split_and_merge_1: the flood-filled distance and the shortest
     neighbour distance equal a brute-force minimum over each particle's pairs;
     particles without pairs keep UINT_MAX."""
-    for bd_mode in (False, True):
-        p, o, before, after, (bd_fill, min_d, _) = _case(bd_mode, library=library)
+    for bd_mode, dims in CASES:
+        p, o, before, after, (bd_fill, min_d, _) = _case(bd_mode, library=library, dims=dims)
         assert np.array_equal(before["fluid_index"], np.arange(len(before["fluid_index"])))
         pairs = before["pairs"].astype(np.int64)
         a, b = pairs[:, 0], pairs[:, 1]
@@ -106,8 +111,8 @@ def pin_nearest(library=None):
     """find_split_and_merge_2: the nearest neighbour of a particle with pairs is
     one of its neighbours at the shortest distance (ties: any of them; the
     oracle takes the last in list order); without pairs it is the particle."""
-    for bd_mode in (False, True):
-        p, o, before, after, (_, min_d, nearest) = _case(bd_mode, library=library)
+    for bd_mode, dims in CASES:
+        p, o, before, after, (_, min_d, nearest) = _case(bd_mode, library=library, dims=dims)
         pairs = before["pairs"].astype(np.int64)
         a, b = pairs[:, 0], pairs[:, 1]
         dist, _ = _dist_fixed(before["position"], a, b)
@@ -127,8 +132,8 @@ def pin_nearest(library=None):
         assert np.array_equal(nearest[has].astype(np.int64), last[has])
 
 
-def _split_factor():
-    return F(F(2.0 ** float(F(1.0) / F(3.0))) * F(0.99))  # pow(2.0, 1.0 / DIMENSIONS) * 0.99
+def _split_factor(dims):
+    return F(F(2.0 ** float(F(1.0) / F(dims))) * F(0.99))  # pow(2.0, 1.0 / DIMENSIONS) * 0.99
 
 
 def _target_radius(s, bdist, bd_mode):
@@ -144,8 +149,8 @@ def pin_find_closed_forms(library=None):
     particles (:79-81), the snapped boundariness, and the split and merge
     predicates (:90-91) of every new split and merge, except where the deciding
     quantity lies within 1e-6 (relative) of its threshold."""
-    for bd_mode in (False, True):
-        p, o, before, after, (bd_fill, _, nearest) = _case(bd_mode, library=library)
+    for bd_mode, dims in CASES:
+        p, o, before, after, (bd_fill, _, nearest) = _case(bd_mode, library=library, dims=dims)
         s = p["settings"]
         nf = len(before["fluid_index"])
         r = before["radius"].astype(F)
@@ -158,13 +163,13 @@ def pin_find_closed_forms(library=None):
         assert np.array_equal(after["boundary_distance"][:nf].astype(np.uint64), want_bd)
         assert (~bnd).sum() > 0 and bnd.sum() > 0
         # predicates, float64 with a margin around each threshold
-        split_q = tr.astype(np.float64) * float(_split_factor()) - r
+        split_q = tr.astype(np.float64) * float(_split_factor(dims)) - r
         idx = np.arange(nf)
         _, nn_len = _dist_fixed(before["position"], idx, nearest.astype(np.int64))
         nn_d = nn_len.astype(np.float64) / 262144.0
         rn = r[nearest].astype(np.float64)
-        vol_q = r.astype(np.float64) ** 3 + rn ** 3 - tr.astype(np.float64) ** 3
-        near = (np.abs(split_q) <= 1e-6 * r) | (np.abs(vol_q) <= 1e-6 * tr.astype(np.float64) ** 3) | \
+        vol_q = r.astype(np.float64) ** dims + rn ** dims - tr.astype(np.float64) ** dims
+        near = (np.abs(split_q) <= 1e-6 * r) | (np.abs(vol_q) <= 1e-6 * tr.astype(np.float64) ** dims) | \
                (np.abs(nn_d - r) <= 1e-6 * r)
         split = split_q <= 0
         merge = (nn_d < r) & (vol_q <= 0)
@@ -195,8 +200,8 @@ def pin_claims_legal_and_maximal(library=None):
     a particle already flagged (source or target of its transfer). A particle
     without pairs is its own nearest neighbour; its merge claims the same
     particle twice, which the second exchange refuses, so it is left out."""
-    for bd_mode in (False, True):
-        p, o, before, after, (bd_fill, _, nearest) = _case(bd_mode, library=library)
+    for bd_mode, dims in CASES:
+        p, o, before, after, (bd_fill, _, nearest) = _case(bd_mode, library=library, dims=dims)
         s = p["settings"]
         nf = len(before["fluid_index"])
         src, tgt, ttl = after["t_source"], after["t_target"], after["t_time_left"]
@@ -212,9 +217,9 @@ def pin_claims_legal_and_maximal(library=None):
         idx = np.arange(nf)
         _, nn_len = _dist_fixed(before["position"], idx, nearest.astype(np.int64))
         nn_d = nn_len.astype(np.float64) / 262144.0
-        vol_q = r.astype(np.float64) ** 3 + r[nearest].astype(np.float64) ** 3 - tr.astype(np.float64) ** 3
-        split_q = tr.astype(np.float64) * float(_split_factor()) - r
-        margin = (np.abs(split_q) <= 1e-6 * r) | (np.abs(vol_q) <= 1e-6 * tr.astype(np.float64) ** 3)
+        vol_q = r.astype(np.float64) ** dims + r[nearest].astype(np.float64) ** dims - tr.astype(np.float64) ** dims
+        split_q = tr.astype(np.float64) * float(_split_factor(dims)) - r
+        margin = (np.abs(split_q) <= 1e-6 * r) | (np.abs(vol_q) <= 1e-6 * tr.astype(np.float64) ** dims)
         lonely = nearest == idx
         merge = (nn_d < r) & (vol_q <= 0) & ~margin & ~lonely
         split = (split_q <= 0) & ~((nn_d < r) & (vol_q <= 0)) & ~margin
@@ -414,7 +419,54 @@ def pin_warm_start(library=None):
     assert np.array_equal(bad.get_state()["position"], staged(iters - 1))
 
 
+def pin_planar_transfer(library=None):
+    """particle_transfer.comp:37-84 in the planar build (DIMENSIONS 2): the
+    volume a transfer moves is r^2, so the total of r^2 and the total mass
+    are conserved up to float rounding (r^3 is not)."""
+    p, st = cfg.small_water_drop(seed=9, active=True, dims=2)
+    o = OracleScene(p, library=library)
+    o.set_state(st)
+    for _ in range(6):
+        o.update(p["dt"])
+        if o.counts()[2] > 0:
+            break
+    before = o.get_state()
+    assert len(before["t_source"]) > 0
+    o.stage("particle_transfer", p["dt"])
+    after = o.get_state()
+    m0 = (1.0 / before["inverse_mass"].astype(np.float64)).sum()
+    m1 = (1.0 / after["inverse_mass"].astype(np.float64)).sum()
+    a0 = (before["radius"].astype(np.float64) ** 2).sum()
+    a1 = (after["radius"].astype(np.float64) ** 2).sum()
+    assert abs(m1 - m0) <= 1e-5 * m0
+    assert abs(a1 - a0) <= 1e-5 * a0
+    assert not np.allclose(before["radius"][: len(after["radius"])], after["radius"])
+
+
+def pin_planar_rest_density(library=None):
+    """The planar rest density (incompressibility_2.comp:81, pow(2r, D) m^-1
+    with D = 2) and the 2D Gaussian normalisation (kernels.glsl:84-89,104):
+    a single layer of particles on a square lattice at spacing 2r with mass
+    (2r)^2 is a Riemann sum of the unit-area kernel, so every particle's
+    density is within 3 % of the rest density and one solver iteration moves
+    no particle by more than 0.05 r."""
+    p, st = cfg.pool_scene(dims=2, mn=(-20.0, -10.0, -20.0), mx=(20.0, 10.0, -10.0))
+    o = OracleScene(p, library=library)
+    o.set_state(st)
+    o.stage("neighborhood", p["dt"])
+    mean_err, max_err = o.density_error()
+    assert mean_err < 0.03 and max_err < 0.03, (mean_err, max_err)
+    before = o.get_state()
+    o.stage("incompressibility", p["dt"])
+    after = o.get_state()
+    d = np.abs(after["position"][:, :3].astype(np.int64) - before["position"][:, :3].astype(np.int64)) / 262144.0
+    assert d.max() <= 0.05 * float(st["radius"][0]), d.max()
+    assert np.array_equal(after["position"][:, 2], before["position"][:, 2])  # the layer stays planar
+
+
 PINS = {
+    "planar_transfer": pin_planar_transfer,
+    "planar_rest_density": pin_planar_rest_density,
     "tolerance_loop": pin_tolerance_loop,
     "warm_start": pin_warm_start,
     "flood_fill": pin_flood_fill,
@@ -448,6 +500,9 @@ MUTATIONS = {
     10: "tolerance_loop",     # the iteration that reaches the tolerance still shifts
     11: "warm_start",         # the warm shift is undone when it helped, kept when it hurt
     12: "warm_start",         # no box collision between the warm shift and the first iteration
+    13: "find_closed_forms",  # the 3D split factor 2^(1/3) in the planar build
+    14: "planar_transfer",    # 3D volumes r^3 moved by planar transfers
+    15: "planar_rest_density",  # the 3D rest density (2r)^3 m^-1 in the planar solver
 }
 
 
