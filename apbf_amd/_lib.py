@@ -227,6 +227,7 @@ PROTOTYPES = {
     "apbf_snapshot_destroy": (_i32, [_vp]),
     "apbf_scene_save_particle_info": (_i32, [_vp, C.c_char_p]),
     "apbf_scene_save_measurements": (_i32, [_vp, C.c_char_p, _u32, _f32, _f32, _f32]),
+    "apbf_scene_save_svg": (_i32, [_vp, C.c_char_p, _u32, _pf, _pf, _f32, _f32, _i32, _i32]),
     "apbf_check_math": (_i32, [_vp, _vp]),
     "apbf_check_conversions": (_i32, [_u32, C.POINTER(C.c_uint64), _pu, _vp, _vp]),
 }

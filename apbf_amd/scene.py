@@ -139,6 +139,17 @@ class Scene:
         """save_particle_info::apply: the reference's text dumps of the fluid."""
         check(self._L.apbf_scene_save_particle_info(self._h, folder.encode()), "save_particle_info")
 
+    def save_svg(self, folder: str, svg_id: int, view_min, view_max, render_scale: float = 0.7,
+                 max_expected_boundary_distance: float = 100.0, color: int = 0, render_boxes: bool = True) -> None:
+        """save_particle_info::save_as_svg's drawing: particles_<svg_id>.svg of
+        a planar scene (nothing in 3D). color: settings::color (0 boundariness,
+        1 boundary-distance gradient, 2 transfers, 3 with kernel circles)."""
+        vmin = (C.c_float * 2)(*view_min)
+        vmax = (C.c_float * 2)(*view_max)
+        check(self._L.apbf_scene_save_svg(self._h, folder.encode(), int(svg_id), vmin, vmax, float(render_scale),
+                                          float(max_expected_boundary_distance), int(color), int(bool(render_boxes))),
+              "save_svg")
+
     def save_measurements(self, folder: str, svg_id: int, step_ms: float = 0.0, neighborhood_ms: float = 0.0,
                           solver_ms: float = 0.0) -> None:
         """save_particle_info::save_additional_info: measurements_<svg_id>.txt."""

@@ -682,6 +682,20 @@ apbf_status apbf_scene_save_particle_info(void* scene, const char* folder);
  * only this file). Synchronises. */
 apbf_status apbf_scene_save_measurements(void* scene, const char* folder, uint32_t id, float step_ms,
                                          float neighborhood_ms, float solver_ms);
+/* save_particle_info::save_as_svg (save_particle_info.cpp:130-254, boxes_to_svg
+ * :256-270), the drawing: folder/particles_<svg_id>.svg of a planar scene
+ * (dimensions 2) in the view box [view_min2, view_max2], with the particle
+ * circles scaled by render_scale (settings::particleRenderScale), coloured by
+ * `color` (settings::color: 0 boundariness, 1 boundary-distance gradient up to
+ * 0.8 max_expected_boundary_distance, 2 transfer pairs, 3 boundariness plus
+ * kernel-width circles) and the scene's boxes when render_boxes
+ * (settings::renderBoxes). Numbers as std::format "{}" prints them. The
+ * reference's save_as_svg also writes measurements_<id>.txt first
+ * (apbf_scene_save_measurements); in 3D it draws nothing, and neither does
+ * this call. Synchronises. */
+apbf_status apbf_scene_save_svg(void* scene, const char* folder, uint32_t svg_id, const float* view_min2,
+                                const float* view_max2, float render_scale, float max_expected_boundary_distance,
+                                int32_t color, int32_t render_boxes);
 
 /* ------------------------------------------------------------ utilities */
 /* Library version string and the device it was built for ("gfx950"). */

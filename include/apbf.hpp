@@ -562,7 +562,16 @@ private:
     std::deque<float> dts_;
 };
 
-/* save_particle_info.h:8-23 (3D: the SVG dump writes measurements_<id>.txt only) */
+/* save_particle_info.h:8-23. save_as_svg writes measurements_<id>.txt and, in
+ * a planar scene, particles_<id>.svg (3D: the measurements only). The
+ * reference's globals it reads (settings::color, settings::renderBoxes, the
+ * measurement timers) come in through svg_options. */
+struct svg_options {
+    int32_t color = 0;          /* settings::color (settings.cpp:29) */
+    bool render_boxes = true;   /* settings::renderBoxes (settings.cpp:15) */
+    float step_ms = 0.0f, neighborhood_ms = 0.0f, solver_ms = 0.0f;
+    const char* folder = "particle_svg";  /* SVG_FOLDER_NAME (cpu_gpu_shared_config.h:22) */
+};
 class save_particle_info {
 public:
     save_particle_info& set_data(scene* s) {
@@ -572,10 +581,15 @@ public:
     void apply(const char* folder = "particle_data") {
         check(apbf_scene_save_particle_info(s_->handle(), folder), "save_particle_info");
     }
-    void save_as_svg(uint32_t svg_id, float step_ms = 0.0f, float neighborhood_ms = 0.0f, float solver_ms = 0.0f,
-                     const char* folder = "particle_svg") {
-        check(apbf_scene_save_measurements(s_->handle(), folder, svg_id, step_ms, neighborhood_ms, solver_ms),
+    void save_as_svg(uint32_t svg_id, const float view_box_min[2], const float view_box_max[2],
+                     float render_scale = 1.0f, float max_expected_boundary_dist = 100.0f,
+                     const svg_options& o = svg_options()) {
+        check(apbf_scene_save_measurements(s_->handle(), o.folder, svg_id, o.step_ms, o.neighborhood_ms,
+                                           o.solver_ms),
               "save_measurements");
+        check(apbf_scene_save_svg(s_->handle(), o.folder, svg_id, view_box_min, view_box_max, render_scale,
+                                  max_expected_boundary_dist, o.color, o.render_boxes ? 1 : 0),
+              "save_svg");
     }
 
 private:

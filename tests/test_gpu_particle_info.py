@@ -59,3 +59,39 @@ def test_particle_info_files_hold_the_state(device, tmp_path):
     assert m[2:6] == ["Simulation Step     : 1.5ms", "Neighborhood Search : 0.25ms", "Constraint Solver   : 0.75ms",
                       "Neighbor Search Type: neighborhood_binary_search.h"]
     s.close()
+
+
+@pytest.mark.parametrize("color", [0, 1, 2, 3])
+def test_planar_svg_drawing(device, tmp_path, color):
+    """save_as_svg's drawing of a planar scene (save_particle_info.cpp:130-270)
+    equals the restatement from the downloaded state, character for
+    character, in every colour mode (transfers in flight for mode 2)."""
+    from apbf_amd.scene import Scene
+    from svg_text import svg_text
+    p, st = cfg.small_water_drop(seed=3, active=True, dims=2)
+    s = Scene(p)
+    s.set_state(st)
+    for _ in range(2):
+        s.update(p["dt"])
+    a = s.get_state(pairs=False)
+    assert color != 2 or len(a["t_source"]) > 0
+    vmin, vmax = (-10.0, -8.0), (170.5, 120.25)
+    folder = str(tmp_path / "particle_svg")
+    s.save_svg(folder, 4, vmin, vmax, render_scale=0.7, max_expected_boundary_distance=30.0, color=color)
+    got = open(os.path.join(folder, "particles_4.svg")).read()
+    want = svg_text(a, p["boxes"], vmin, vmax, 0.7, 30.0, color, True)
+    assert got == want
+    s.save_svg(folder, 5, vmin, vmax, color=color, render_boxes=False)
+    assert open(os.path.join(folder, "particles_5.svg")).read() == svg_text(a, p["boxes"], vmin, vmax, 0.7, 100.0,
+                                                                               color, False)
+    s.close()
+
+
+def test_svg_drawing_skipped_in_3d(device, tmp_path):
+    from apbf_amd.scene import Scene
+    p, st = cfg.small_water_drop(seed=2)
+    s = Scene(p)
+    s.set_state(st)
+    s.save_svg(str(tmp_path), 1, (0.0, 0.0), (10.0, 10.0))
+    assert not os.path.exists(tmp_path / "particles_1.svg")
+    s.close()
