@@ -179,7 +179,7 @@ apbf_status apbf_batch_synchronize(apbf_batch* b) {
 }
 
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes) {
-    if (!scene || (walk_lanes != 0 && walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16) ||
+    if (!scene || (walk_lanes != 0 && walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16 && walk_lanes != 64) ||
         (solve_lanes != 8 && solve_lanes != 16 && solve_lanes != 32))
         return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
@@ -256,7 +256,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     s->walk_auto = false;
     s->timed_mask = 0;
     if (st == APBF_OK) copy_stats(true);
-    const int walks[3] = {4, 8, 16}, solves[3] = {8, 16, 32};
+    const int walks[3] = {4, 8, 64}, solves[3] = {8, 16, 32};
     float trial_ms[3][3];
     for (int r = 0; r < 2 && st == APBF_OK; ++r)
         for (int a = 0; a < 3 && st == APBF_OK; ++a)

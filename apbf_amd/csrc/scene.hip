@@ -277,9 +277,12 @@ void Scene::neighborhood_green(float range_scale, bool spread) {
 void Scene::build_pairs(float range_scale, bool spread) {
     hipStream_t s = stream;
     const SegmentTable seg = seg_table();
-    if (walk_auto) {  // short rows: fewer lanes per particle fill a wave better (large pool: 22.1 vs 22.6 ms)
+    if (walk_auto) {
+        // short rows: 4 lanes per particle fill a wave better (large pool: 12.6 ms per substep against 13.5
+        // with 8 and 15.3 with whole waves); long rows: whole waves per particle (water drop: 6.79 against
+        // 7.15 ms with 8)
         refresh_seen_counts();
-        walk_lanes = (seen_n != 0u && (uint64_t)seen_pairs < 62ull * (uint64_t)seen_n) ? 4 : 8;
+        walk_lanes = (seen_n != 0u && (uint64_t)seen_pairs < 62ull * (uint64_t)seen_n) ? 4 : 64;
     }
     const int on_t = P.s.mBaseKernelWidthOnTargetRadius;
     uint32_t* row_raw = spread ? u0 : row;  // offsets of the full (unfiltered) list

@@ -305,6 +305,324 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
     }
 }
 
+// Inclusive scan over the 64 lanes of a wavefront (every lane active) of an
+// associative, commutative op with identity 0 on unsigned values (sum, max):
+// DPP row shifts inside each 16-lane row, then the row totals by readlane.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, Op op) {
+    constexpr int ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118;
+    auto shr = [](uint32_t x, auto ctrl) {  // lanes whose source leaves the row read 0
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, decltype(ctrl)::value, 0xF, 0xF, false);
+    };
+    v = op(v, shr(v, std::integral_constant<int, ROW_SHR1>{}));
+    v = op(v, shr(v, std::integral_constant<int, ROW_SHR2>{}));
+    v = op(v, shr(v, std::integral_constant<int, ROW_SHR4>{}));
+    v = op(v, shr(v, std::integral_constant<int, ROW_SHR8>{}));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1 = op(r0, (uint32_t)__builtin_amdgcn_readlane((int)v, 31));
+    const uint32_t r2 = op(r1, (uint32_t)__builtin_amdgcn_readlane((int)v, 47));
+    const uint32_t row = (threadIdx.x & 63u) >> 4;
+    const uint32_t carry = row == 0u ? 0u : (row == 1u ? r0 : (row == 2u ? r1 : r2));
+    return op(v, carry);
+}
+
+// q = c / d and r = c % d for a divisor shared by the wave: m = floor(2^32 / d)
+// (0xFFFFFFFF for d = 1) underestimates the quotient by at most one.
+struct Div32 {
+    uint32_t d, m;
+    __device__ __forceinline__ static uint32_t multiplier(uint32_t dv) {
+        return dv <= 1u ? 0xFFFFFFFFu : (uint32_t)(0x100000000ull / dv);
+    }
+    __device__ __forceinline__ Div32(uint32_t dv, uint32_t mv) : d(dv), m(mv) {}
+    __device__ __forceinline__ uint32_t div(uint32_t c, uint32_t& r) const {
+        uint32_t q = __umulhi(c, m);
+        r = c - q * d;
+        if (r >= d) {
+            ++q;
+            r -= d;
+        }
+        return q;
+    }
+};
+
+// Wave-uniform copy of lane k's value (v_readlane into a scalar register).
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int k) {
+    static_assert(sizeof(T) == 4, "32-bit values");
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+
+// The same walk with a whole wavefront per particle and the cells of the
+// search box flattened into one stream. Waves take batches of WAVE_BATCH
+// consecutive particles (grid-stride; small batches keep the uneven cost of
+// particles spread over the waves: 2.99 ms per water-drop walk at 4, 3.10 at
+// 8, 3.60 at 16, 3.15 at 2; a queue counter shared by all waves was slower
+// still, its atomics serialising). The per-particle set-up (box, exact
+// distance bounds, culling constants) runs for a batch at once, one particle
+// per lane; the wave then takes the particles one by one with their values
+// read from those lanes. 64 cells (in the reference order: x fastest,
+// then y, then z) are looked up at a time, one per lane, culled and scanned
+// into stream offsets; the wave then tests 64 consecutive candidates of that
+// stream per step across cell ends, so the lanes stay busy whatever the
+// cells hold. Cells whose particles lie back to back in memory form one run;
+// a lane takes its candidate's run from the run holding the step's first
+// candidate and the (few) runs starting inside the step, read from their
+// lanes. Outputs, order and clamps are those of k_walk_rows.
+constexpr uint32_t WAVE_BATCH = 4;
+template <int MODE, bool SPREAD, bool SEG>
+__global__ __launch_bounds__(BLOCK) void k_walk_wave(
+    const int4* __restrict__ pos, const float* __restrict__ kw, const float* __restrict__ radius,
+    const float* __restrict__ trad, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+    uint32_t* __restrict__ counts, WalkArgs wa, uint32_t* __restrict__ cnt_raw, uint32_t* __restrict__ cnt_f,
+    const uint32_t* __restrict__ row_raw, const uint32_t* __restrict__ row_f, uint32_t* __restrict__ col,
+    uint32_t* __restrict__ ukw) {
+    constexpr bool FILL = MODE == WALK_FILL;
+    constexpr bool PAD = MODE == WALK_PAD;
+    bool only_long_rows = false;
+    if (FILL && wa.gate) {
+        if (wa.gate[C_FALLBACK] == 0u) {
+            if (wa.gate[C_ROWFIX] == 0u) return;
+            only_long_rows = true;
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    auto vadd = [](uint32_t a, uint32_t b) { return a + b; };
+    const uint32_t n = counts[C_N];
+    const uint32_t stride = wa.stride ? slot_stride(wa.pcap, n) : 0u;
+    const uint32_t waves = BLOCK / 64;
+    for (uint32_t base = (blockIdx.x * waves + threadIdx.x / 64) * WAVE_BATCH; base < n;
+         base += gridDim.x * waves * WAVE_BATCH) {
+        // ---- set-up of particle base + lane, lanes < WAVE_BATCH
+        const uint32_t ip = base + lane;
+        bool act = lane < WAVE_BATCH && ip < n;
+        if (act && only_long_rows && row_f[ip + 1] - row_f[ip] <= stride) act = false;
+        uint32_t s_sc = 0, s_lo[3] = {0, 0, 0}, s_n[3] = {0, 0, 0}, s_umax = 0, s_cull = 0, s_braw = 0, s_bf = 0;
+        uint32_t s_mx = 0, s_my = 0;  // Div32 multipliers of the box's x and y sizes
+        int32_t s_pi[3] = {0, 0, 0};
+        float s_trange = 0.0f, s_okw = 0.0f, s_tcut = 0.0f, s_tokw = 0.0f, s_thr = 0.0f;
+        if (act) {
+            s_sc = SEG ? wa.seg.sid[ip] : 0u;
+            const GridDesc& g = SEG ? wa.seg.grids[s_sc] : wa.g;
+            const int4 pi = pos[ip];
+            const v3f p = pos_world(pi);
+            const float kwi = kw[ip];
+            const float range = kwi * wa.range_scale;
+            uint32_t lo[3], hi[3];
+            lo[0] = grid_cell(p.x - range, g.mn[0], g.mx[0], g.res);
+            lo[1] = grid_cell(p.y - range, g.mn[1], g.mx[1], g.res);
+            lo[2] = grid_cell(p.z - range, g.mn[2], g.mx[2], g.res);
+            hi[0] = grid_cell(p.x + range, g.mn[0], g.mx[0], g.res);
+            hi[1] = grid_cell(p.y + range, g.mn[1], g.mx[1], g.res);
+            hi[2] = grid_cell(p.z + range, g.mn[2], g.mx[2], g.res);
+            if (g.dims == 2u) lo[2] = hi[2] = 0u;  // neighborhood_green.comp:36
+            const uint32_t span = 4u << g.res;
+            for (int d = 0; d < 3; ++d) {
+                if (hi[d] - lo[d] >= span && hi[d] >= lo[d]) hi[d] = lo[d] + span - 1u;
+                s_lo[d] = lo[d];
+                s_n[d] = hi[d] >= lo[d] ? hi[d] - lo[d] + 1u : 0u;
+            }
+            s_mx = Div32::multiplier(s_n[0] ? s_n[0] : 1u);
+            s_my = Div32::multiplier(s_n[1] ? s_n[1] : 1u);
+            s_pi[0] = pi.x;
+            s_pi[1] = pi.y;
+            s_pi[2] = pi.z;
+            // the tests of k_walk_rows, on unscaled coordinates
+            s_trange = ((range != range) ? __builtin_inff() : sqrt_le_bound(range)) * 0x1p36f;
+            if (SPREAD) {
+                const float tr = wa.on_target ? trad[ip] : 0.0f;
+                s_okw = fmax_g(radius[ip], tr) * KERNEL_SCALE;
+                s_tcut = sqrt_le_bound(fmax_g(s_okw, kwi)) * 0x1p36f;
+                s_tokw = sqrt_le_bound(s_okw) * 0x1p36f;
+                s_umax = to_u32(s_okw * KW_RES);
+            }
+            const uint32_t last_cell = (1u << g.res) - 1u;
+            const bool cull = s_trange < __builtin_inff() && range >= 0.0f && hi[0] <= last_cell &&
+                              hi[1] <= last_cell && hi[2] <= last_cell;
+            s_cull = cull ? 1u : 0u;
+            s_thr = cull ? range * range * 1.001f + 1e-30f : __builtin_inff();
+            if (FILL) {
+                s_braw = row_raw[ip];
+                s_bf = SPREAD ? row_f[ip] : s_braw;
+            }
+        }
+        // ---- the batch's particles one at a time
+        for (uint64_t am = __ballot(act); am; am &= am - 1ull) {
+            const int k = (int)__builtin_ctzll(am);
+            const uint32_t i = base + (uint32_t)k;
+            const uint32_t sc = lane_value(s_sc, k);
+            const GridDesc& g = SEG ? wa.seg.grids[sc] : wa.g;
+            const uint32_t cbase = SEG ? (sc << wa.seg.shift) : 0u;
+            const int4 pi = make_int4(lane_value(s_pi[0], k), lane_value(s_pi[1], k), lane_value(s_pi[2], k), 0);
+            const v3f p = pos_world(pi);
+            const v3f pr = vec_i(pi.x, pi.y, pi.z);
+            const float t_range = lane_value(s_trange, k);
+            const float okw = lane_value(s_okw, k), t_cut = lane_value(s_tcut, k), t_okw = lane_value(s_tokw, k);
+            const uint32_t umax = lane_value(s_umax, k);
+            const bool cull = lane_value(s_cull, k) != 0u;
+            const float thr = lane_value(s_thr, k);
+            const uint32_t lo0 = lane_value(s_lo[0], k), lo1 = lane_value(s_lo[1], k), lo2 = lane_value(s_lo[2], k);
+            const uint32_t nx = lane_value(s_n[0], k), ny = lane_value(s_n[1], k), nz = lane_value(s_n[2], k);
+            auto spread_to = [&](uint32_t j, float d2r, uint32_t cur) {  // kernel_width.comp:41-53
+                if (umax <= cur) return;
+                uint32_t uval;
+                if (d2r <= t_okw) {
+                    uval = umax;
+                } else {
+                    const float dfk = __builtin_sqrtf(d2r * 0x1p-36f) - okw;
+                    const float infl = fmax_g(0.0f, 1.0f - fmax_g(0.0f, dfk / (okw * KW_PROPAGATION)));
+                    uval = to_u32(okw * infl * KW_RES);
+                }
+                if (uval > cur) atomicMax(&ukw[j], uval);
+            };
+            // the cell culling of k_walk_rows (the same cell bounds and margins)
+            const uint32_t last_cell = (1u << g.res) - 1u;
+            float cw[3], cm[3];
+            for (int d = 0; d < 3; ++d) {
+                cw[d] = (g.mx[d] - g.mn[d]) / (float)(1u << g.res);
+                cm[d] = 1e-4f * cw[d] + 1e-6f * (__builtin_fabsf(g.mn[d]) + __builtin_fabsf(g.mx[d]));
+            }
+            auto axis_d2 = [&](int d, uint32_t c, float pc) -> float {
+                if (!cull || c == 0u || c >= last_cell) return 0.0f;
+                const float a = g.mn[d] + (float)c * cw[d] - cm[d];
+                const float b = g.mn[d] + (float)(c + 1u) * cw[d] + cm[d];
+                const float t = pc < a ? a - pc : (pc > b ? pc - b : 0.0f);
+                return t * t;
+            };
+            uint32_t craw = 0, cf = 0, braw = 0, bf = 0;
+            uint64_t tested = 0;
+            bool more = true;
+            if (FILL) {
+                braw = lane_value(s_braw, k);
+                bf = lane_value(s_bf, k);
+                more = braw < wa.pcap;
+            }
+            const size_t slot = PAD ? (size_t)i * stride : 0;
+            const uint64_t ncell = (uint64_t)nx * ny * nz;
+            const bool planar = g.dims == 2u;
+            const Div32 divx(nx ? nx : 1u, lane_value(s_mx, k)), divy(ny ? ny : 1u, lane_value(s_my, k));
+            for (uint64_t c0 = 0; more && c0 < ncell; c0 += 64) {
+                // one cell per lane
+                uint32_t cstart = 0, clen = 0;
+                const uint64_t c = c0 + lane;
+                if (c < ncell) {
+                    uint32_t x, y, z;
+                    if (ncell <= 0xFFFFFFFFull) {
+                        const uint32_t q = divx.div((uint32_t)c, x);
+                        z = divy.div(q, y);
+                    } else {  // a box of more than 2^32 cells (unbounded ranges at high resolutions)
+                        x = (uint32_t)(c % nx);
+                        y = (uint32_t)((c / nx) % ny);
+                        z = (uint32_t)(c / ((uint64_t)nx * ny));
+                    }
+                    x += lo0;
+                    y += lo1;
+                    z += lo2;
+                    const float d2 = (axis_d2(2, z, p.z) + axis_d2(1, y, p.y)) + axis_d2(0, x, p.x);
+                    if (!(d2 > thr)) {
+                        const uint32_t h = cbase + (planar ? (zbits(x, 0u, g.res, 2u) | zbits(y, 1u, g.res, 2u))
+                                                           : (zbits(x, 0u, g.res, 3u) | zbits(y, 1u, g.res, 3u) |
+                                                              zbits(z, 2u, g.res, 3u)));
+                        cstart = cs[h];
+                        const uint32_t cend = ce[h];
+                        clen = cend > cstart ? cend - cstart : 0u;
+                    }
+                }
+                const uint32_t incl = wave_scan(clen, vadd);
+                const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                const uint32_t off = incl - clen;  // the cell's first candidate in the stream
+                // candidate = stream position + delta inside the cell; a cell whose
+                // particles directly follow those of the previous non-empty cell
+                // (the same delta) continues its run
+                const uint32_t delta = cstart - off;
+                const uint64_t nonempty = __ballot(clen > 0u);
+                const uint64_t prev_mask = nonempty & below;
+                const uint32_t prev = prev_mask ? 63u - (uint32_t)__clzll(prev_mask) : lane;
+                const uint32_t prev_delta = (uint32_t)__shfl((int)delta, (int)prev, 64);
+                const bool start = clen > 0u && (prev_mask == 0ull || prev_delta != delta);
+                if (MODE == WALK_COUNT) tested += total;
+                for (uint32_t w0 = 0; more && w0 < total; w0 += 64) {
+                    // the delta of candidate w0 + lane: that of the run containing
+                    // w0, replaced by each run starting inside the step from its
+                    // first candidate on
+                    const uint64_t le = __ballot(start && off <= w0);
+                    const int r0 = 63 - (int)__clzll(le);  // le != 0: w0 < total
+                    uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)delta, r0);
+                    uint64_t inside = __ballot(start && off > w0 && off - w0 < 64u);
+                    while (inside) {
+                        const int q = (int)__builtin_ctzll(inside);
+                        inside &= inside - 1ull;
+                        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)off, q) - w0;
+                        const uint32_t dq = (uint32_t)__builtin_amdgcn_readlane((int)delta, q);
+                        if (lane >= o) d = dq;
+                    }
+                    const uint32_t t = w0 + lane;
+                    const uint32_t j = t + d;
+                    bool green = false, keep = false;
+                    float d2 = 0.0f;
+                    uint32_t uj = 0u;
+                    if (t < total && j != i) {
+                        const int4 pj = pos[j];
+                        if (SPREAD && MODE != WALK_COUNT) uj = ukw[j];
+                        const v3f u = pr - vec_i(pj.x, pj.y, pj.z);
+                        green = !(dot(u, u) > t_range);
+                        if (SPREAD && green) {
+                            const v3f v = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z));
+                            d2 = dot(v, v);
+                            keep = d2 <= t_cut;
+                        }
+                    }
+                    const uint64_t rbal = __ballot(green);
+                    if (PAD) {
+                        if (SPREAD && green) spread_to(j, d2, uj);
+                        const bool kept = SPREAD ? (green && keep) : green;
+                        const uint64_t kbal = SPREAD ? __ballot(kept) : rbal;
+                        const uint32_t kk = cf + (uint32_t)__popcll(kbal & below);
+                        if (kept && kk < stride) col[slot + kk] = j;
+                        cf += (uint32_t)__popcll(kbal);
+                        craw += (uint32_t)__popcll(rbal);
+                    } else if (!FILL) {
+                        craw += (uint32_t)__popcll(rbal);
+                        if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep));
+                    } else {
+                        const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
+                        const bool incl_r = green && ridx < wa.pcap;
+                        if (SPREAD) {
+                            if (incl_r) spread_to(j, d2, uj);
+                            const bool kept = incl_r && keep;
+                            const uint64_t fbal = __ballot(kept);
+                            if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
+                            cf += (uint32_t)__popcll(fbal);
+                        } else if (incl_r) {
+                            col[ridx] = j;
+                        }
+                        craw += (uint32_t)__popcll(rbal);
+                        more = braw + craw < wa.pcap;
+                    }
+                }
+            }
+            if (lane == 0) {
+                if (PAD) {
+                    cnt_raw[i] = craw;
+                    if (SPREAD) cnt_f[i] = cf;
+                    if (cf > stride) atomicMax(&counts[C_MAXROW], cf);
+                } else if (!FILL) {
+                    cnt_raw[i] = craw;
+                    if (SPREAD) cnt_f[i] = cf;
+                    if (wa.cand) atomicAdd(wa.cand, (unsigned long long)tested);
+                } else if (SPREAD && braw <= wa.pcap && row_raw[i + 1] > wa.pcap) {
+                    counts[C_CLAMP_ROW] = i;
+                    counts[C_CLAMP_KEPT] = cf;
+                }
+            }
+        }
+    }
+    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt_raw[n] = 0;
+        if (SPREAD) cnt_f[n] = 0;
+    }
+}
+
 // After a PAD walk and the scan of its raw counts: take the exact path when a
 // row outgrew its slot or the list exceeds the capacity.
 __global__ void k_walk_decide(uint32_t* __restrict__ counts, uint32_t pcap) {
@@ -439,9 +757,40 @@ void launch_walk(int mode, bool spread, const int4* pos, const float* kw, const 
     wa.cand = cand;
     wa.seg = seg ? *seg : SegmentTable{nullptr, nullptr, nullptr, nullptr, 0u};
     const bool packed = seg && seg->sid;
-    if (lanes != 4 && lanes != 16) lanes = WALK_LANES;
-    if (packed) lanes = WALK_LANES;  // packed batches walk with the default group width
+    if (lanes != 4 && lanes != 16 && lanes != 64) lanes = WALK_LANES;
+    if (packed && lanes != 64) lanes = WALK_LANES;  // packed batches: the default group width or whole waves
     const dim3 grid(walk_grid(cap, (uint32_t)lanes)), block(BLOCK);
+    if (lanes == 64) {
+        const dim3 wgrid(walk_grid(div_up(cap ? cap : 1u, WAVE_BATCH), 64u));
+#define WAVE(F, S, G)                                                                                                  \
+    hipLaunchKernelGGL((k_walk_wave<F, S, G>), wgrid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
+                       cnt_f, row_raw, row_f, col, ukw)
+#define WAVE_G(G)                         \
+    if (mode == WALK_PAD) {               \
+        if (spread)                       \
+            WAVE(WALK_PAD, true, G);      \
+        else                              \
+            WAVE(WALK_PAD, false, G);     \
+    } else if (mode == WALK_FILL) {       \
+        if (spread)                       \
+            WAVE(WALK_FILL, true, G);     \
+        else                              \
+            WAVE(WALK_FILL, false, G);    \
+    } else {                              \
+        if (spread)                       \
+            WAVE(WALK_COUNT, true, G);    \
+        else                              \
+            WAVE(WALK_COUNT, false, G);   \
+    }
+        if (packed) {
+            WAVE_G(true)
+        } else {
+            WAVE_G(false)
+        }
+#undef WAVE_G
+#undef WAVE
+        return;
+    }
 #define WALK(L, F, S, G)                                                                                                 \
     hipLaunchKernelGGL((k_walk_rows<L, F, S, G>), grid, block, 0, s, pos, kw, radius, trad, cs, ce, counts, wa, cnt_raw, \
                        cnt_f, row_raw, row_f, col, ukw)

@@ -579,10 +579,12 @@ apbf_status apbf_batch_update(apbf_batch* batch, float dt);
 apbf_status apbf_batch_capture(apbf_batch* batch, float dt);
 apbf_status apbf_batch_replay(apbf_batch* batch, uint32_t times);
 apbf_status apbf_batch_synchronize(apbf_batch* batch);
-/* Launch shapes: lanes per particle in the neighbour walk (4, 8, 16) and per
- * row in the solver (8, 16, 32); every shape computes identical results.
- * Until a shape is set or tuned, the walk takes 4 lanes when a recent
- * substep's mean neighbour row is below 62 entries and 8 otherwise, and the
+/* Launch shapes: lanes per particle in the neighbour walk (4, 8, 16, or 64:
+ * a whole wavefront per particle over a flattened stream of the box's cells)
+ * and per row in the solver (8, 16, 32); every shape computes identical
+ * results. Until a shape is set or tuned, the walk takes 4 lanes when a
+ * recent substep's mean neighbour row is below 62 entries and whole
+ * wavefronts otherwise, and the
  * solver 16; walk_lanes 0 returns the walk to that choice. The recent
  * substep's counts reach the host by a copy that is never waited for; a
  * graph captured by apbf_scene_capture keeps the choices of the state it

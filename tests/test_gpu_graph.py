@@ -68,7 +68,7 @@ def test_batch_graph_matches_individual_scenes(device):
     batch.close()
 
 
-@pytest.mark.parametrize("walk,solve", [(4, 8), (4, 16), (8, 8), (8, 32), (16, 8), (16, 16)])
+@pytest.mark.parametrize("walk,solve", [(4, 8), (4, 16), (8, 8), (8, 32), (16, 8), (16, 16), (64, 16), (64, 32)])
 def test_launch_shapes_match_the_oracle(device, walk, solve):
     from oracle.oracle import OracleScene
     p, st = cfg.small_water_drop(seed=4, active=True)
@@ -123,7 +123,7 @@ def test_autotune_keeps_results(device):
         for _ in range(3):
             s.update(dt)
     w, s, ms = tuned.autotune(dt, substeps=2)
-    assert w in (4, 8, 16) and s in (8, 16, 32) and ms > 0
+    assert w in (4, 8, 64) and s in (8, 16, 32) and ms > 0
     bits, items = tuned.sort_shape()
     assert bits in (4, 8, 11) and items in (4, 8, 16)
     bad = compare(tuned.get_state(), plain.get_state(), check_pairs=False)
