@@ -180,12 +180,13 @@ apbf_status apbf_batch_synchronize(apbf_batch* b) {
 
 apbf_status apbf_scene_set_launch_shape(void* scene, int32_t walk_lanes, int32_t solve_lanes) {
     if (!scene || (walk_lanes != 0 && walk_lanes != 4 && walk_lanes != 8 && walk_lanes != 16 && walk_lanes != 64) ||
-        (solve_lanes != 8 && solve_lanes != 16 && solve_lanes != 32))
+        (solve_lanes != 0 && solve_lanes != 8 && solve_lanes != 16 && solve_lanes != 32))
         return APBF_ERR_INVALID_ARGUMENT;
     Scene* s = static_cast<Scene*>(scene);
     s->walk_auto = walk_lanes == 0;  // 0: lanes from a recent mean row length again
     if (walk_lanes != 0) s->walk_lanes = walk_lanes;
-    s->solve_lanes = solve_lanes;
+    s->solve_auto = solve_lanes == 0;
+    if (solve_lanes != 0) s->solve_lanes = solve_lanes;
     drop_graph(s);  // a captured graph holds the old shapes
     return APBF_OK;
 }
@@ -252,8 +253,9 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     };
     const uint32_t timed_mask = s->timed_mask;
     const int walk0 = s->walk_lanes, solve0 = s->solve_lanes;
-    const bool auto0 = s->walk_auto;
+    const bool auto0 = s->walk_auto, sauto0 = s->solve_auto;
     s->walk_auto = false;
+    s->solve_auto = false;
     s->timed_mask = 0;
     if (st == APBF_OK) copy_stats(true);
     const int walks[3] = {4, 8, 64}, solves[3] = {8, 16, 32};
@@ -329,6 +331,7 @@ apbf_status apbf_scene_autotune(void* scene, float dt, uint32_t substeps, int32_
     s->timed_mask = timed_mask;
     s->walk_lanes = st == APBF_OK ? bw : walk0;
     s->walk_auto = st == APBF_OK ? false : auto0;
+    s->solve_auto = st == APBF_OK ? false : sauto0;
     s->solve_lanes = st == APBF_OK ? bs : solve0;
     if (st == APBF_OK) s->sort_shape = best_sort;
     drop_graph(s);

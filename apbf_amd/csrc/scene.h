@@ -132,9 +132,10 @@ struct Scene {
     uint32_t *t_flag = nullptr, *t_scan = nullptr;
     int claim_rounds = 6;
     // launch shapes picked by the autotuner (all compute identical results)
-    int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8 or 16
-    bool walk_auto = true;  // walk lanes from a recent mean row length (4 below 62 entries, else 8)
-    int solve_lanes = 16;  // lanes per row in the solver: 8 or 16
+    int walk_lanes = 8;    // lanes per particle in the neighbour walk: 4, 8, 16 or 64 (a whole wave)
+    bool walk_auto = true;  // walk lanes from a recent mean row length (4 below 62 entries, else 64)
+    int solve_lanes = 16;  // lanes per row in the solver: 8, 16 or 32
+    bool solve_auto = true;  // solver lanes from a recent mean row length (8 below 62 entries, else 16)
     SortShape sort_shape = SORT_DEFAULT;  // digit width and tile of the hash sorts
     bool exact_walks = false;  // pair build by the two exact walks only (no fixed-stride slots)
     bool count_candidates = false;  // diagnostic: a counting walk adds its tested candidates to C_CANDIDATES

@@ -425,7 +425,8 @@ def main():
     scene.set_error_measurement(True)
     log(f"scene {params.get('workload')} with {len(state['radius'])} particles ready "
         f"({time.perf_counter() - t_setup:.1f} s of set-up, not timed)")
-    shape = "walk 8 lanes (4 when rows average under 62 entries), solver 16 lanes (library default)"
+    shape = ("library defaults: when rows average under 62 entries 4 walk lanes per particle and 8 solver lanes "
+             "per row, otherwise a whole wavefront per particle and 16 solver lanes")
     if args.launch_shape:
         w, v = (int(x) for x in args.launch_shape.split(","))
         scene.set_launch_shape(w, v)

@@ -584,8 +584,8 @@ apbf_status apbf_batch_synchronize(apbf_batch* batch);
  * and per row in the solver (8, 16, 32); every shape computes identical
  * results. Until a shape is set or tuned, the walk takes 4 lanes when a
  * recent substep's mean neighbour row is below 62 entries and whole
- * wavefronts otherwise, and the
- * solver 16; walk_lanes 0 returns the walk to that choice. The recent
+ * wavefronts otherwise, and the solver 8 lanes for such short rows and 16
+ * otherwise; 0 for either returns it to that choice. The recent
  * substep's counts reach the host by a copy that is never waited for; a
  * graph captured by apbf_scene_capture keeps the choices of the state it
  * was captured from.
