@@ -369,6 +369,11 @@ __device__ __forceinline__ T lane_value(T v, int k) {
 // candidate and the (few) runs starting inside the step, read from their
 // lanes. Outputs, order and clamps are those of k_walk_rows.
 constexpr uint32_t WAVE_BATCH = 4;
+// Candidates per lane and step (the step covers 64 * WAVE_CANDS consecutive
+// candidates): the run lookup is shared and the loads of all of them are in
+// flight together (water drop: 2.82 ms per walk at 2 and 3, 2.99 at 1, 3.08
+// at 4).
+constexpr uint32_t WAVE_CANDS = 2;
 template <int MODE, bool SPREAD, bool SEG>
 __global__ __launch_bounds__(BLOCK) void k_walk_wave(
     const int4* __restrict__ pos, const float* __restrict__ kw, const float* __restrict__ radius,
@@ -541,63 +546,86 @@ __global__ __launch_bounds__(BLOCK) void k_walk_wave(
                 const uint32_t prev_delta = (uint32_t)__shfl((int)delta, (int)prev, 64);
                 const bool start = clen > 0u && (prev_mask == 0ull || prev_delta != delta);
                 if (MODE == WALK_COUNT) tested += total;
-                for (uint32_t w0 = 0; more && w0 < total; w0 += 64) {
-                    // the delta of candidate w0 + lane: that of the run containing
-                    // w0, replaced by each run starting inside the step from its
-                    // first candidate on
+                for (uint32_t w0 = 0; more && w0 < total; w0 += 64u * WAVE_CANDS) {
+                    // WAVE_CANDS candidates per lane: w0 + 64 c + lane. The delta of
+                    // each: that of the run containing w0, replaced by each run
+                    // starting inside the step from its first candidate on
                     const uint64_t le = __ballot(start && off <= w0);
                     const int r0 = 63 - (int)__clzll(le);  // le != 0: w0 < total
-                    uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)delta, r0);
-                    uint64_t inside = __ballot(start && off > w0 && off - w0 < 64u);
+                    uint32_t d[WAVE_CANDS];
+                    d[0] = (uint32_t)__builtin_amdgcn_readlane((int)delta, r0);
+#pragma unroll
+                    for (uint32_t c = 1; c < WAVE_CANDS; ++c) d[c] = d[0];
+                    uint64_t inside = __ballot(start && off > w0 && off - w0 < 64u * WAVE_CANDS);
                     while (inside) {
                         const int q = (int)__builtin_ctzll(inside);
                         inside &= inside - 1ull;
                         const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)off, q) - w0;
                         const uint32_t dq = (uint32_t)__builtin_amdgcn_readlane((int)delta, q);
-                        if (lane >= o) d = dq;
+#pragma unroll
+                        for (uint32_t c = 0; c < WAVE_CANDS; ++c)
+                            if (lane + 64u * c >= o) d[c] = dq;
                     }
-                    const uint32_t t = w0 + lane;
-                    const uint32_t j = t + d;
-                    bool green = false, keep = false;
-                    float d2 = 0.0f;
-                    uint32_t uj = 0u;
-                    if (t < total && j != i) {
-                        const int4 pj = pos[j];
-                        if (SPREAD && MODE != WALK_COUNT) uj = ukw[j];
-                        const v3f u = pr - vec_i(pj.x, pj.y, pj.z);
-                        green = !(dot(u, u) > t_range);
-                        if (SPREAD && green) {
-                            const v3f v = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z));
-                            d2 = dot(v, v);
-                            keep = d2 <= t_cut;
+                    // every candidate's loads first, then the tests in stream order
+                    uint32_t jc[WAVE_CANDS], uc[WAVE_CANDS];
+                    int4 pc[WAVE_CANDS];
+                    bool vc[WAVE_CANDS];
+#pragma unroll
+                    for (uint32_t c = 0; c < WAVE_CANDS; ++c) {
+                        const uint32_t t = w0 + 64u * c + lane;
+                        jc[c] = t + d[c];
+                        vc[c] = t < total && jc[c] != i;
+                        pc[c] = make_int4(0, 0, 0, 0);
+                        uc[c] = 0u;
+                        if (vc[c]) {
+                            pc[c] = pos[jc[c]];
+                            if (SPREAD && MODE != WALK_COUNT) uc[c] = ukw[jc[c]];
                         }
                     }
-                    const uint64_t rbal = __ballot(green);
-                    if (PAD) {
-                        if (SPREAD && green) spread_to(j, d2, uj);
-                        const bool kept = SPREAD ? (green && keep) : green;
-                        const uint64_t kbal = SPREAD ? __ballot(kept) : rbal;
-                        const uint32_t kk = cf + (uint32_t)__popcll(kbal & below);
-                        if (kept && kk < stride) col[slot + kk] = j;
-                        cf += (uint32_t)__popcll(kbal);
-                        craw += (uint32_t)__popcll(rbal);
-                    } else if (!FILL) {
-                        craw += (uint32_t)__popcll(rbal);
-                        if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep));
-                    } else {
-                        const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
-                        const bool incl_r = green && ridx < wa.pcap;
-                        if (SPREAD) {
-                            if (incl_r) spread_to(j, d2, uj);
-                            const bool kept = incl_r && keep;
-                            const uint64_t fbal = __ballot(kept);
-                            if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
-                            cf += (uint32_t)__popcll(fbal);
-                        } else if (incl_r) {
-                            col[ridx] = j;
+#pragma unroll
+                    for (uint32_t c = 0; c < WAVE_CANDS; ++c) {
+                        if (c > 0 && !(w0 + 64u * c < total && more)) break;
+                        const uint32_t j = jc[c];
+                        const int4 pj = pc[c];
+                        const uint32_t uj = uc[c];
+                        bool green = false, keep = false;
+                        float d2 = 0.0f;
+                        if (vc[c]) {
+                            const v3f u = pr - vec_i(pj.x, pj.y, pj.z);
+                            green = !(dot(u, u) > t_range);
+                            if (SPREAD && green) {
+                                const v3f v = vec_i(isub(pj.x, pi.x), isub(pj.y, pi.y), isub(pj.z, pi.z));
+                                d2 = dot(v, v);
+                                keep = d2 <= t_cut;
+                            }
                         }
-                        craw += (uint32_t)__popcll(rbal);
-                        more = braw + craw < wa.pcap;
+                        const uint64_t rbal = __ballot(green);
+                        if (PAD) {
+                            if (SPREAD && green) spread_to(j, d2, uj);
+                            const bool kept = SPREAD ? (green && keep) : green;
+                            const uint64_t kbal = SPREAD ? __ballot(kept) : rbal;
+                            const uint32_t kk = cf + (uint32_t)__popcll(kbal & below);
+                            if (kept && kk < stride) col[slot + kk] = j;
+                            cf += (uint32_t)__popcll(kbal);
+                            craw += (uint32_t)__popcll(rbal);
+                        } else if (!FILL) {
+                            craw += (uint32_t)__popcll(rbal);
+                            if (SPREAD) cf += (uint32_t)__popcll(__ballot(keep));
+                        } else {
+                            const uint32_t ridx = braw + craw + (uint32_t)__popcll(rbal & below);
+                            const bool incl_r = green && ridx < wa.pcap;
+                            if (SPREAD) {
+                                if (incl_r) spread_to(j, d2, uj);
+                                const bool kept = incl_r && keep;
+                                const uint64_t fbal = __ballot(kept);
+                                if (kept) col[bf + cf + (uint32_t)__popcll(fbal & below)] = j;
+                                cf += (uint32_t)__popcll(fbal);
+                            } else if (incl_r) {
+                                col[ridx] = j;
+                            }
+                            craw += (uint32_t)__popcll(rbal);
+                            more = braw + craw < wa.pcap;
+                        }
                     }
                 }
             }
