@@ -190,19 +190,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     // A block takes STEPS consecutive groups of BLOCK / L rows, then
     // flushes the table; the trip counts are uniform per block, so every
     // thread meets the barriers.
-    const uint32_t n = counts[C_N];
+    const uint32_t n = a.list ? counts[C_LONG] : counts[C_N];
     const int gl = threadIdx.x & (L - 1);
     constexpr uint32_t groups = BLOCK / L;
     constexpr uint32_t span = groups * STEPS;
     for (uint32_t base = blockIdx.x * span; base < n; base += gridDim.x * span) {
         for (uint32_t step = 0; step < STEPS; ++step) {
-            const uint32_t i = base + step * groups + threadIdx.x / L;
-            const bool active = i < n;
+            const uint32_t slot = base + step * groups + threadIdx.x / L;
+            bool active = slot < n;
+            const uint32_t i = (a.list && active) ? a.list[slot] : slot;
             int4 pi = make_int4(0, 0, 0, 0);
             float im = 1.0f, kwi = 1.0f, ird = 1.0f;
             uint32_t self = 0;
             GaussConst g{1.0f, 1.0f};
             uint32_t rb = 0, re = 0;
+            if (active) {
+                rb = row[i];
+                re = min(row[i + 1], pcap);
+                // a row longer than split_len is left to the pass over the long-row list
+                if (a.split_len && re > rb && re - rb > a.split_len) {
+                    active = false;
+                    rb = re = 0;
+                }
+            }
             if (active) {
                 pi = pos[i];
                 im = inv_mass[i];
@@ -211,8 +221,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                 g = GaussConst{g4.x, g4.y};
                 ird = g4.z;
                 self = __builtin_bit_cast(uint32_t, g4.w);
-                rb = row[i];
-                re = min(row[i + 1], pcap);
             }
             // incompressibility_1.comp:43-68 over this lane's share of row i
             uint32_t dens = 0, sq = 0, s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
@@ -952,14 +960,63 @@ static uint32_t row_grid(uint32_t cap, uint32_t lanes = ROW_LANES) {
     return g < 8192u ? g : 8192u;
 }
 
+// Rows longer than a row group's gradient cache (SOLVE_CACHE entries per lane
+// in registers, SOLVE_SPILL in LDS) go to a pass with a whole wavefront per
+// row. In the settled water drop 6 % of the rows (the coarse interior
+// particles, 145 to 1 000 entries) hold 37 % of the pairs; a 16-lane group
+// takes them through its uncached tail loop one dependent load chain at a
+// time, with 3 idle row groups beside it in the wave.
+static uint32_t solve_split_len(int lanes) { return (uint32_t)((SOLVE_CACHE + SOLVE_SPILL) * lanes); }
+
+// Lists the rows of more than split_len entries (after the pair-capacity
+// clamp) for the whole-wave pass: one atomic per wavefront of 64 rows, the
+// rows of a wavefront in order. counts[C_LONG] is zero before.
+__global__ __launch_bounds__(BLOCK) void k_long_rows(uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
+                                                     uint32_t pcap, uint32_t split_len, uint32_t* __restrict__ list) {
+    const uint32_t n = counts[C_N];
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = (blockIdx.x * BLOCK + threadIdx.x) & ~63u; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + lane;
+        bool is_long = false;
+        if (i < n) {
+            const uint32_t rb = row[i], re = min(row[i + 1], pcap);
+            is_long = re > rb && re - rb > split_len;
+        }
+        const uint64_t bal = __ballot(is_long);
+        if (bal == 0) continue;  // the same for every lane of the wavefront
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&counts[C_LONG], (uint32_t)__popcll(bal));
+        at = (uint32_t)__shfl((int)at, 0, 64);
+        if (is_long) list[at + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
+void launch_long_rows(uint32_t* counts, uint32_t cap, const uint32_t* row, uint32_t pcap, int lanes, uint32_t* list,
+                      hipStream_t s) {
+    fill_u32(counts + C_LONG, 0u, 1, s);
+    hipLaunchKernelGGL(k_long_rows, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, counts, row, pcap,
+                       solve_split_len(lanes), list);
+}
+
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
-                       const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
-                       float* lambda, int4* acc, hipStream_t s, int lanes, bool long_rows) {
+                       const uint32_t* col, uint32_t pcap, const SolverArgs& a_in, float* boundariness,
+                       float* lambda, int4* acc, hipStream_t s, int lanes, bool long_rows,
+                       const uint32_t* long_list) {
     int* ac = reinterpret_cast<int*>(acc);
+    SolverArgs a = a_in;
+    const bool split = long_list && a.hid == 1 && a.gid == 1 && (lanes == 8 || lanes == 16 || lanes == 32);
+    if (split) a.split_len = solve_split_len(lanes);
 #define SOLVE(L, KH, KG, C, F)                                                                                       \
     hipLaunchKernelGGL((k_solve_rows<L, KH, KG, C, F>), dim3(row_grid(cap, L)), dim3(BLOCK), 0, s, pos, inv_mass,    \
                        radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac)
+    if (split) {  // the long rows (listed by launch_long_rows), a wavefront per row
+        SolverArgs al = a;
+        al.split_len = 0;
+        al.list = long_list;
+        hipLaunchKernelGGL((k_solve_rows<64, 1, 1, SOLVE_CACHE, 1>), dim3(row_grid(cap, 64)), dim3(BLOCK), 0, s,
+                           pos, inv_mass, radius, kw, gc, counts, row, col, pcap, al, boundariness, lambda, ac);
+    }
     if (a.hid == 1 && a.gid == 1 && lanes == 8)  // Gaussian height and gradient, 8 lanes per row
         SOLVE(8, 1, 1, SOLVE_CACHE, AGG_STEPS);
     else if (a.hid == 1 && a.gid == 1 && lanes == 32)  // 32 lanes: rows of up to 192 entries stay in registers

@@ -111,6 +111,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(col, pcap);
     A(col_tmp, pcap);
     A(gc, cap);
+    A(long_list, cap);
     A(lambda, cap);
     A(shift, cap);
     A(ukw, cap);
@@ -172,8 +173,8 @@ Scene::~Scene() {
     }
     void* ps_seg[] = {d_grids, d_boxes, seg_start, sid[0], sid[1]};
     for (void* q : ps_seg) hipFree(q);
-    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, lambda, shift,
-                  ukw, u0, u1, u2, u3, u4, small, part_sum,
+    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, long_list,
+                  lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_owner2, a_flag, a_scan,
                   a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps,
                   tol_err, pos_save, solve_stats};
@@ -386,6 +387,8 @@ SolverArgs Scene::solver_args() const {
     a.lam_acc = nullptr;
     a.tol_sum = nullptr;
     a.skip = nullptr;
+    a.split_len = 0;
+    a.list = nullptr;
     return a;
 }
 
@@ -412,6 +415,8 @@ void Scene::prepare_solver() {
     // short rows: 8 lanes per row (large pool, 26 entries: 1.58 against 2.14 ms per launch with 16);
     // long rows: 16 (water drop: 0.66 against 1.01 ms with 8 and 0.80 with 32)
     if (solve_auto) solve_lanes = short_rows ? 8 : 16;
+    if (P.s.mHeightKernelId == 1 && P.s.mGradientKernelId == 1)
+        launch_long_rows(counts, cap, row, pcap, solve_lanes, long_list, stream);
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
                          P.s.mGradientKernelId, dim(), gc, stream);
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
@@ -433,7 +438,7 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const 
     }
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
-                      boundariness[cur], lambda, shift, s, solve_lanes, long_rows);
+                      boundariness[cur], lambda, shift, s, solve_lanes, long_rows, long_list);
     tend(tk);
     const bool judge = decided && !lam_override;
     if (judge) launch_solve_decide(counts, tol_err, (uint32_t)it, tol, ws_mode == 2 && ws_model && it == 0, s);

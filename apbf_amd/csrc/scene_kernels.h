@@ -37,6 +37,7 @@ enum CountSlot : uint32_t {
     C_WARM_FALLBACK = 26,  // 1 when this substep's warm start was undone (the cold start judged better)
     C_MAXROW_LAST = 27,    // single-pass walk: longest row that outgrew its slot in the last walk (0: none)
     C_CANDIDATES = 28,     // 28-29: candidates tested by the diagnostic counting walks (a 64-bit sum)
+    C_LONG = 30,           // rows in the solver's long-row list (rows longer than the row groups cache)
     C_COUNT = 32
 };
 
@@ -90,6 +91,8 @@ struct SolverArgs {
     float* lam_acc;             // when set: += min(lambda, 0) per particle
     double* tol_sum;            // when set: += sum of max(0, rho/rho0 - 1) over the positions the pass starts from
     const uint32_t* skip;       // when set and *skip != 0: the pass does nothing (the solve is done)
+    uint32_t split_len;         // when nonzero: rows of more entries are left to a pass over `list`
+    const uint32_t* list;       // when set: the pass takes the rows listed here (counts[C_LONG] of them)
 };
 
 // -------------------------------------------------------- shared device code
