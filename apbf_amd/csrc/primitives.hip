@@ -398,9 +398,11 @@ __global__ __launch_bounds__(BLOCK) void k_check_conversions(uint32_t stride, un
 }
 
 // Every float bit pattern in the stated ranges through sqrt_in_range /
-// recip_in_range and through the IEEE square root and 1/x: the number of
-// patterns where the bits differ and the smallest such pattern, per function
-// (slot 0 is unused).
+// recip_in_range and through the IEEE square root and 1/x, and 2^32 quotients
+// through quotient_by_recip and the IEEE division (512 divisors, each with
+// every dividend significand): the number of cases where the bits differ and
+// the smallest such case (bit pattern, or divisor index << 23 | significand),
+// per function (slot 0 quotient, 1 square root, 2 reciprocal).
 __global__ __launch_bounds__(BLOCK) void k_check_math(unsigned long long* mismatches, uint32_t* first_bad) {
     unsigned long long bad[3] = {0, 0, 0};
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (1ull << 32);
@@ -409,11 +411,27 @@ __global__ __launch_bounds__(BLOCK) void k_check_math(unsigned long long* mismat
         const float x = bits_f(b);
         const float ax = __builtin_fabsf(x);
         bool d[3] = {false, false, false};
+        {
+            // divisor d: a hashed significand (d = 0: all ones, d = 1: 1.5) with exponents
+            // of -40 to 40, the dividend 1.m with exponents of -80 to 80 and either sign
+            const uint32_t di = b >> 23, m = b & 0x7FFFFFu;
+            uint32_t h = di * 0x9E3779B1u;
+            h ^= h >> 15;
+            h *= 0x85EBCA77u;
+            h ^= h >> 13;
+            const uint32_t bm = di == 0u ? 0x7FFFFFu : (di == 1u ? 0x400000u : (h & 0x7FFFFFu));
+            const uint32_t be = 127u + (di % 81u) - 40u;
+            const uint32_t xe = 127u + ((di * 7u + (m >> 11)) % 161u) - 80u;
+            const float dv = bits_f((be << 23) | bm);
+            const float xv = bits_f(((m & 1u) << 31) | (xe << 23) | m);
+            d[0] = __builtin_bit_cast(uint32_t, quotient_by_recip(xv, dv, recip_in_range(dv))) !=
+                   __builtin_bit_cast(uint32_t, xv / dv);
+        }
         if (x == 0.0f || (x >= 0x1p-96f && x <= 0x1p120f))
             d[1] = __builtin_bit_cast(uint32_t, sqrt_in_range(x)) != __builtin_bit_cast(uint32_t, __builtin_sqrtf(x));
         if (ax >= 0x1p-120f && ax <= 0x1p120f)
             d[2] = __builtin_bit_cast(uint32_t, recip_in_range(x)) != __builtin_bit_cast(uint32_t, 1.0f / x);
-        for (int f = 1; f < 3; ++f)
+        for (int f = 0; f < 3; ++f)
             if (d[f]) {
                 ++bad[f];
                 atomicMin(&first_bad[f], b);

@@ -167,9 +167,12 @@ struct ShiftTable {
 
 // STEPS: row groups a block takes at a time and sums in its table before a
 // flush. Long rows bring more distinct neighbours per row group, and a table
-// flushed every 32 rows overflows less (water drop, 100-160 entries per row:
-// 0.75 against 0.80 ms per launch); short rows gain from fewer flushes
-// (large pool, 33 entries per row: 4.04 against 4.31 ms).
+// flushed every 32 rows overflows less when a pass takes rows of every length
+// (water drop, 100-160 entries per row: 0.75 against 0.80 ms per launch);
+// short rows gain from fewer flushes (large pool, 33 entries per row: 4.04
+// against 4.31 ms), and so does the water drop once its rows above 144
+// entries go to the whole-wave pass (0.621 against 0.631 ms per iteration).
+// The whole-wave pass flushes after every 4 rows.
 // Six waves per SIMD (80 VGPRs, 25.6 KB of LDS per block) hide the latency
 // of the row's dependent loads.
 template <int L, int KH, int KG, int C, uint32_t STEPS>
@@ -237,7 +240,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                 // Inverse masses are powers of two unless a merge mixed two masses.
                 // Dividing by a power of two whose reciprocal is a normal float is the
                 // exact scaling by that reciprocal (one rounding of the same real
-                // value), so the quotients below are the IEEE quotients either way.
+                // value). Other divisors of 2^-40 to 2^40 take quotient_by_recip, the
+                // IEEE quotient for dividends of at least 2^-80; smaller dividends
+                // give quotients below 2^-39 either way, whose products with
+                // INCOMP_RES truncate to 0. So the integers below are those of the
+                // IEEE quotients in every case (finite w and gg; otherwise the
+                // full division runs).
                 const uint32_t ib = __builtin_bit_cast(uint32_t, imN);
                 const uint32_t ie = (ib >> 23) & 0xFFu;
                 if (__all((ib & 0x7FFFFFu) == 0u && ie >= 1u && ie <= 253u)) {
@@ -245,6 +253,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     wq = w * rim;
                     wg = grad * rim;
                     ggq = gg * rim;
+                } else if (__all(ie >= 87u && ie <= 167u && w < __builtin_inff() && gg < __builtin_inff())) {
+                    const float y = recip_in_range(imN);
+                    wq = quotient_by_recip(w, imN, y);
+                    wg = mk(quotient_by_recip(grad.x, imN, y), quotient_by_recip(grad.y, imN, y),
+                            quotient_by_recip(grad.z, imN, y));
+                    ggq = quotient_by_recip(gg, imN, y);
                 } else {
                     wq = w / imN;
                     wg = grad / imN;
@@ -1021,7 +1035,7 @@ void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radi
         SOLVE(8, 1, 1, SOLVE_CACHE, AGG_STEPS);
     else if (a.hid == 1 && a.gid == 1 && lanes == 32)  // 32 lanes: rows of up to 192 entries stay in registers
         SOLVE(32, 1, 1, SOLVE_CACHE, AGG_STEPS);
-    else if (a.hid == 1 && a.gid == 1 && long_rows)  // the reference's defaults (Gaussian), long rows
+    else if (a.hid == 1 && a.gid == 1 && long_rows && !split)  // the reference's defaults (Gaussian), long rows
         SOLVE(ROW_LANES, 1, 1, SOLVE_CACHE, AGG_STEPS / 2);
     else if (a.hid == 1 && a.gid == 1)
         SOLVE(ROW_LANES, 1, 1, SOLVE_CACHE, AGG_STEPS);

@@ -717,8 +717,11 @@ apbf_status apbf_check_conversions(uint32_t stride, uint64_t* mismatches, uint32
 /* Self-check of the device square root and reciprocal the Gaussian gradient
  * uses (IEEE sqrt and 1/x, kernels.glsl:91-97): every float bit pattern of
  * their argument ranges through the short device forms and through the full
- * IEEE forms. mismatches3[f] / first_bad3[f] for f = 1 square root,
- * 2 reciprocal (slot 0 unused). No counterpart in the reference. */
+ * IEEE forms; and of the solver's quotients by inverse masses (IEEE x / b,
+ * incompressibility_1.comp:53-59) through a reciprocal and one correction:
+ * 512 divisors, each with every dividend significand. mismatches3[f] /
+ * first_bad3[f] for f = 0 quotient, 1 square root, 2 reciprocal. No
+ * counterpart in the reference. */
 apbf_status apbf_check_math(uint64_t* mismatches3, uint32_t* first_bad3);
 
 #ifdef __cplusplus

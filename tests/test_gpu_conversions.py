@@ -38,11 +38,14 @@ def test_conversions_match_definition_on_every_input(device):
 
 def test_short_sqrt_and_reciprocal_equal_ieee(device):
     """sqrt_in_range and recip_in_range (hardware approximation plus one
-    correction) give the IEEE bits on every float of their argument ranges."""
+    correction) give the IEEE bits on every float of their argument ranges,
+    and quotient_by_recip (the solver's division by inverse masses) gives the
+    IEEE quotient for 512 divisors of 2^-40 to 2^40, each with every dividend
+    significand."""
     from apbf_amd import _lib
     L = _lib.lib()
     bad = np.full(3, 7, np.uint64)
     first = np.zeros(3, np.uint32)
     _lib.check(L.apbf_check_math(bad.ctypes.data_as(C.c_void_p), first.ctypes.data_as(C.c_void_p)), "apbf_check_math")
-    names = ("unused", "sqrt", "reciprocal")
+    names = ("quotient", "sqrt", "reciprocal")
     assert not bad.any(), {names[f]: (int(bad[f]), hex(first[f])) for f in range(3) if bad[f]}

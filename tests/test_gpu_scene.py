@@ -320,6 +320,29 @@ def test_solver_long_rows_match_oracle(device):
             assert len(gs["pairs"]) > 1500 * n, len(gs["pairs"]) / n
 
 
+def test_solver_mass_quotients_match_oracle(device):
+    """Inverse masses that are not powers of two (as after merges of unequal
+    masses) in every neighbour row: most within a factor of two of the
+    lattice's, some 2^-44 and 2^44 times it (outside the range of the
+    solver's reciprocal quotient, so their rows' slots take the full IEEE
+    division). The solver's quotients by them give the oracle's substeps bit
+    for bit."""
+    p, st = cfg.small_water_drop(seed=5, active=False)
+    n = len(st["radius"])
+    rng = np.random.default_rng(11)
+    f = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    far = rng.random(n) < 0.01
+    f[far] *= np.where(rng.random(int(far.sum())) < 0.5, np.float32(2.0 ** -44), np.float32(2.0 ** 44))
+    im = (st["inverse_mass"] * f).astype(np.float32)
+    assert ((im.view(np.uint32) & 0x7FFFFF) != 0).mean() > 0.99
+    st["inverse_mass"] = im
+    g, r = _pair(p, st)
+    for step in range(2):
+        g.update(p["dt"])
+        r.update(p["dt"])
+        _check(g, r, f"substep {step}")
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_solver_table_variants_match_oracle(device, mode):
     """The solver's two shift-table variants (flush every 64 or every 32 rows),

@@ -1,7 +1,10 @@
 """Host-side code without a GPU: the warm-start training's input
-normalisation (NumPy arrays and torch tensors) and the SVG restatement's
-float formatting."""
+normalisation (NumPy arrays and torch tensors), the SVG restatement's float
+formatting and the solver's reciprocal quotient compiled for the CPU."""
+import os
+
 import numpy as np
+import pytest
 
 
 def test_fit_norm_numpy_and_torch_agree():
@@ -34,3 +37,24 @@ def test_svg_float_format_matches_std_format():
              16777217.0: "16777216", 0.030000001: "0.030000001"}
     for v, want in cases.items():
         assert fmt32(v) == want, (v, fmt32(v), want)
+
+
+def test_reciprocal_quotient_equals_ieee_division(tmp_path):
+    """The solver divides by inverse masses through the reciprocal and one
+    fused correction (quotient_by_recip); compiled for the CPU, it gives the
+    IEEE quotient for 64 divisors, each with every dividend significand
+    (5.4e8 cases, divisor exponents -40..40, dividend exponents -80..80)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc absent")
+    exe = str(tmp_path / "quotient_check")
+    src = os.path.join(os.path.dirname(__file__), "quotient_check.c")
+    flags = ["-O2", "-ffp-contract=off"]
+    if subprocess.run([gcc, *flags, "-march=native", "-o", exe, src, "-lm"]).returncode != 0:
+        subprocess.run([gcc, *flags, "-o", exe, src, "-lm"], check=True)
+    out = subprocess.run([exe, "64"], capture_output=True, text=True, check=True, timeout=300).stdout.split()
+    bad, total = int(out[0]), int(out[1])
+    assert total == 64 * (1 << 23)
+    assert bad == 0, bad
