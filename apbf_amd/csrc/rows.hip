@@ -32,13 +32,16 @@ constexpr int ROW_LANES = 16;
 // step every lane holds the reduction over its 2, 4, 8, 16 lanes. Wider
 // groups continue with xor shuffles. Sums of integers, minima and maxima do
 // not depend on the order of the operands, so the results equal those of
-// any other reduction order.
+// any other reduction order. The moves are written as DPP updates of a zero
+// with bound_ctrl set (every source lane of these patterns lies in the same
+// row, so neither takes effect), the form the compiler folds into the
+// integer add or minimum that consumes the move: one instruction per step.
 template <int L, typename Op>
 __device__ __forceinline__ uint32_t greduce(uint32_t v, Op op) {
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "group of a power of two lanes");
     constexpr int QUAD_XOR1 = 0xB1, QUAD_XOR2 = 0x4E, ROW_HALF_MIRROR = 0x141, ROW_MIRROR = 0x140;
     auto dpp = [](uint32_t x, auto ctrl) {
-        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, decltype(ctrl)::value, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, decltype(ctrl)::value, 0xF, 0xF, true);
     };
     if (L >= 2) v = op(v, dpp(v, std::integral_constant<int, QUAD_XOR1>{}));
     if (L >= 4) v = op(v, dpp(v, std::integral_constant<int, QUAD_XOR2>{}));
