@@ -153,7 +153,7 @@ APBF_HD float recip_in_range(float x) {
 // over every significand of x for 100 000 divisor significands and over every
 // significand of b for 65 536 dividend significands, without a difference
 // (apbf_check_math repeats 512 divisors on the device). For |x| < 2^-80 both
-// quotients are below 2^-39 in magnitude. Five fused or plain operations
+// quotients are below 2^-39 in magnitude. Three operations per quotient
 // after the reciprocal against about eleven for a full IEEE division.
 APBF_HD float quotient_by_recip(float x, float b, float y) {
     const float q0 = x * y;
