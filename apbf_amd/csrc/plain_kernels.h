@@ -72,9 +72,9 @@ void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radi
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a, float* boundariness,
                        float* lambda, int4* acc, hipStream_t s, int lanes = 16, bool long_rows = false,
-                       const uint32_t* long_list = nullptr);
+                       const uint32_t* long_list = nullptr, const uint32_t* long_list2 = nullptr);
 void launch_long_rows(uint32_t* counts, uint32_t cap, const uint32_t* row, uint32_t pcap, int lanes, uint32_t* list,
-                      hipStream_t s);
+                      uint32_t* list2, hipStream_t s);
 void launch_features_rows(const int4* pos, const float4* vel, const float* inv_mass, const float* radius,
                           const float* kw, const float4* gc, const float* boundariness, const float* lam_prev,
                           const uint32_t* counts, uint32_t cap, const uint32_t* row, const uint32_t* col,

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     // A block takes STEPS consecutive groups of BLOCK / L rows, then
     // flushes the table; the trip counts are uniform per block, so every
     // thread meets the barriers.
-    const uint32_t n = a.list ? counts[C_LONG] : counts[C_N];
+    const uint32_t n = a.list ? counts[a.list_slot] : counts[C_N];
     const int gl = threadIdx.x & (L - 1);
     constexpr uint32_t groups = BLOCK / L;
     constexpr uint32_t span = groups * STEPS;
@@ -983,56 +983,84 @@ static uint32_t row_grid(uint32_t cap, uint32_t lanes = ROW_LANES) {
 // particles, 145 to 1 000 entries) hold 37 % of the pairs; a 16-lane group
 // takes them through its uncached tail loop one dependent load chain at a
 // time, with 3 idle row groups beside it in the wave.
+// Rows up to the 32-lane cache (288 entries) take 32 lanes per row, two rows
+// per wavefront, which halves the per-row work (lambda, boundariness: about
+// 280 instructions per row group) against a wavefront per row; the batch of
+// 64 reference scenes has many rows of 145 to 256 entries (28.8 ms per batch
+// substep with a wavefront for every row above 144 entries, 27.4 without
+// the split).
 static uint32_t solve_split_len(int lanes) { return (uint32_t)((SOLVE_CACHE + SOLVE_SPILL) * lanes); }
 
 // Lists the rows of more than split_len entries (after the pair-capacity
-// clamp) for the whole-wave pass: one atomic per wavefront of 64 rows, the
-// rows of a wavefront in order. counts[C_LONG] is zero before.
+// clamp) for the passes over long rows: those up to split2 entries in list,
+// the longer ones in list2. One atomic per list and wavefront of 64 rows, the
+// rows of a wavefront in order. counts[C_LONG] and counts[C_LONG2] are zero
+// before.
 __global__ __launch_bounds__(BLOCK) void k_long_rows(uint32_t* __restrict__ counts, const uint32_t* __restrict__ row,
-                                                     uint32_t pcap, uint32_t split_len, uint32_t* __restrict__ list) {
+                                                     uint32_t pcap, uint32_t split_len, uint32_t split2,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ list2) {
     const uint32_t n = counts[C_N];
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t base = (blockIdx.x * BLOCK + threadIdx.x) & ~63u; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + lane;
-        bool is_long = false;
+        uint32_t len = 0;
         if (i < n) {
             const uint32_t rb = row[i], re = min(row[i + 1], pcap);
-            is_long = re > rb && re - rb > split_len;
+            len = re > rb ? re - rb : 0u;
         }
-        const uint64_t bal = __ballot(is_long);
-        if (bal == 0) continue;  // the same for every lane of the wavefront
-        uint32_t at = 0;
-        if (lane == 0) at = atomicAdd(&counts[C_LONG], (uint32_t)__popcll(bal));
-        at = (uint32_t)__shfl((int)at, 0, 64);
-        if (is_long) list[at + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+        const bool in1 = len > split_len && len <= split2, in2 = len > split2;
+        const uint64_t b1 = __ballot(in1), b2 = __ballot(in2);
+        const uint64_t below = (1ull << lane) - 1ull;
+        if (b1) {  // the same for every lane of the wavefront
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&counts[C_LONG], (uint32_t)__popcll(b1));
+            at = (uint32_t)__shfl((int)at, 0, 64);
+            if (in1) list[at + (uint32_t)__popcll(b1 & below)] = i;
+        }
+        if (b2) {
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&counts[C_LONG2], (uint32_t)__popcll(b2));
+            at = (uint32_t)__shfl((int)at, 0, 64);
+            if (in2) list2[at + (uint32_t)__popcll(b2 & below)] = i;
+        }
     }
 }
 
 void launch_long_rows(uint32_t* counts, uint32_t cap, const uint32_t* row, uint32_t pcap, int lanes, uint32_t* list,
-                      hipStream_t s) {
-    fill_u32(counts + C_LONG, 0u, 1, s);
-    hipLaunchKernelGGL(k_long_rows, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, counts, row, pcap,
-                       solve_split_len(lanes), list);
+                      uint32_t* list2, hipStream_t s) {
+    static_assert(C_LONG2 == C_LONG + 1, "adjacent counters");
+    fill_u32(counts + C_LONG, 0u, 2, s);
+    const uint32_t s1 = solve_split_len(lanes), s2 = solve_split_len(32);
+    hipLaunchKernelGGL(k_long_rows, dim3(stride_grid(cap)), dim3(BLOCK), 0, s, counts, row, pcap, s1,
+                       s2 > s1 ? s2 : s1, list, list2);
 }
 
 void launch_solve_rows(const int4* pos, const float* inv_mass, const float* radius, const float* kw,
                        const float4* gc, const uint32_t* counts, uint32_t cap, const uint32_t* row,
                        const uint32_t* col, uint32_t pcap, const SolverArgs& a_in, float* boundariness,
                        float* lambda, int4* acc, hipStream_t s, int lanes, bool long_rows,
-                       const uint32_t* long_list) {
+                       const uint32_t* long_list, const uint32_t* long_list2) {
     int* ac = reinterpret_cast<int*>(acc);
     SolverArgs a = a_in;
-    const bool split = long_list && a.hid == 1 && a.gid == 1 && (lanes == 8 || lanes == 16 || lanes == 32);
+    const bool split = long_list && long_list2 && a.hid == 1 && a.gid == 1 &&
+                       (lanes == 8 || lanes == 16 || lanes == 32);
     if (split) a.split_len = solve_split_len(lanes);
 #define SOLVE(L, KH, KG, C, F)                                                                                       \
     hipLaunchKernelGGL((k_solve_rows<L, KH, KG, C, F>), dim3(row_grid(cap, L)), dim3(BLOCK), 0, s, pos, inv_mass,    \
                        radius, kw, gc, counts, row, col, pcap, a, boundariness, lambda, ac)
-    if (split) {  // the long rows (listed by launch_long_rows), a wavefront per row
+    if (split) {  // the long rows (listed by launch_long_rows): 32 lanes per row, then a wavefront per row
         SolverArgs al = a;
         al.split_len = 0;
-        al.list = long_list;
+        al.list = long_list2;
+        al.list_slot = C_LONG2;
         hipLaunchKernelGGL((k_solve_rows<64, 1, 1, SOLVE_CACHE, 1>), dim3(row_grid(cap, 64)), dim3(BLOCK), 0, s,
                            pos, inv_mass, radius, kw, gc, counts, row, col, pcap, al, boundariness, lambda, ac);
+        if (lanes != 32) {  // with 32-lane groups the first list is empty
+            al.list = long_list;
+            al.list_slot = C_LONG;
+            hipLaunchKernelGGL((k_solve_rows<32, 1, 1, SOLVE_CACHE, 1>), dim3(row_grid(cap, 32)), dim3(BLOCK), 0, s,
+                               pos, inv_mass, radius, kw, gc, counts, row, col, pcap, al, boundariness, lambda, ac);
+        }
     }
     if (a.hid == 1 && a.gid == 1 && lanes == 8)  // Gaussian height and gradient, 8 lanes per row
         SOLVE(8, 1, 1, SOLVE_CACHE, AGG_STEPS);

@@ -119,7 +119,8 @@ struct Scene {
     uint32_t* col_tmp = nullptr;
     // solver
     float4* gc = nullptr;  // per particle: Gaussian constants, rest-density factor, own density term
-    uint32_t* long_list = nullptr;  // rows the solver takes with a wavefront per row (counts[C_LONG])
+    uint32_t* long_list = nullptr;   // rows the solver takes with 32 lanes per row (counts[C_LONG])
+    uint32_t* long_list2 = nullptr;  // rows the solver takes with a wavefront per row (counts[C_LONG2])
     float* lambda = nullptr;
     int4* shift = nullptr;  // per-particle shift accumulator of the solver (x, y, z, unused)
     // adaptive scratch

@@ -112,6 +112,7 @@ apbf_status Scene::init(const apbf_scene_params& p) {
     A(col_tmp, pcap);
     A(gc, cap);
     A(long_list, cap);
+    A(long_list2, cap);
     A(lambda, cap);
     A(shift, cap);
     A(ukw, cap);
@@ -173,7 +174,7 @@ Scene::~Scene() {
     }
     void* ps_seg[] = {d_grids, d_boxes, seg_start, sid[0], sid[1]};
     for (void* q : ps_seg) hipFree(q);
-    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, long_list,
+    void* ps[] = {counts, keys, vals, skeys, perm, cell_start, cell_end, cnt, row, col, col_tmp, gc, long_list, long_list2,
                   lambda, shift, ukw, u0, u1, u2, u3, u4, small, part_sum,
                   part_max, ws_mem, a_type, a_s, a_t, a_state, a_owner, a_owner2, a_flag, a_scan,
                   a_list, a_t0, t_flag, t_scan, err_stats, iter_err, feat, lam_acc, lam_hat, particle_substeps,
@@ -389,6 +390,7 @@ SolverArgs Scene::solver_args() const {
     a.skip = nullptr;
     a.split_len = 0;
     a.list = nullptr;
+    a.list_slot = C_LONG;
     return a;
 }
 
@@ -416,7 +418,7 @@ void Scene::prepare_solver() {
     // long rows: 16 (water drop: 0.66 against 1.01 ms with 8 and 0.80 with 32)
     if (solve_auto) solve_lanes = short_rows ? 8 : 16;
     if (P.s.mHeightKernelId == 1 && P.s.mGradientKernelId == 1)
-        launch_long_rows(counts, cap, row, pcap, solve_lanes, long_list, stream);
+        launch_long_rows(counts, cap, row, pcap, solve_lanes, long_list, long_list2, stream);
     launch_kernel_consts(kernel_width[cur], radius[cur], inv_mass[cur], counts, cap, P.s.mHeightKernelId,
                          P.s.mGradientKernelId, dim(), gc, stream);
     fill_u32_dev(reinterpret_cast<uint32_t*>(shift), 0u, counts + C_N4, 4 * cap, stream);
@@ -438,7 +440,7 @@ void Scene::incompressibility_iteration(const BoxSet& next_boxes, int it, const 
     }
     bool tk = tbegin(K_SOLVE_ROWS);
     launch_solve_rows(pos[cur], inv_mass[cur], radius[cur], kernel_width[cur], gc, counts, cap, row, col, pcap, a,
-                      boundariness[cur], lambda, shift, s, solve_lanes, long_rows, long_list);
+                      boundariness[cur], lambda, shift, s, solve_lanes, long_rows, long_list, long_list2);
     tend(tk);
     const bool judge = decided && !lam_override;
     if (judge) launch_solve_decide(counts, tol_err, (uint32_t)it, tol, ws_mode == 2 && ws_model && it == 0, s);

@@ -37,7 +37,8 @@ enum CountSlot : uint32_t {
     C_WARM_FALLBACK = 26,  // 1 when this substep's warm start was undone (the cold start judged better)
     C_MAXROW_LAST = 27,    // single-pass walk: longest row that outgrew its slot in the last walk (0: none)
     C_CANDIDATES = 28,     // 28-29: candidates tested by the diagnostic counting walks (a 64-bit sum)
-    C_LONG = 30,           // rows in the solver's long-row list (rows longer than the row groups cache)
+    C_LONG = 30,           // rows in the solver's first long-row list (longer than a row group's cache)
+    C_LONG2 = 31,          // rows in its second list (longer than a 32-lane group's cache)
     C_COUNT = 32
 };
 
@@ -92,7 +93,8 @@ struct SolverArgs {
     double* tol_sum;            // when set: += sum of max(0, rho/rho0 - 1) over the positions the pass starts from
     const uint32_t* skip;       // when set and *skip != 0: the pass does nothing (the solve is done)
     uint32_t split_len;         // when nonzero: rows of more entries are left to a pass over `list`
-    const uint32_t* list;       // when set: the pass takes the rows listed here (counts[C_LONG] of them)
+    const uint32_t* list;       // when set: the pass takes the rows listed here (counts[list_slot] of them)
+    uint32_t list_slot;         // C_LONG or C_LONG2
 };
 
 // -------------------------------------------------------- shared device code
