@@ -175,7 +175,8 @@ struct ShiftTable {
 // short rows gain from fewer flushes (large pool, 33 entries per row: 4.04
 // against 4.31 ms), and so does the water drop once its rows above 144
 // entries go to the whole-wave pass (0.621 against 0.631 ms per iteration).
-// The whole-wave pass flushes after every 4 rows.
+// The passes over the long-row lists flush after every row group (8 rows
+// at 32 lanes per row, 4 at a wavefront per row).
 // Six waves per SIMD (80 VGPRs, 25.6 KB of LDS per block) hide the latency
 // of the row's dependent loads.
 template <int L, int KH, int KG, int C, uint32_t STEPS>
